@@ -1,0 +1,224 @@
+/*
+ * mmca.h — C ABI of the MI355X-native cellular-automaton (CA) step of the
+ * muMatScale solidification model.
+ *
+ * The reference has no plugin system: its operator API is the set of C
+ * functions that `doiteration()` (calculate.c:193-352) and `loop()`
+ * (loop.c:186-272) call on one subblock, passing state through globals
+ * (`bp`, globals.h:422; `lsp`, calculate.c:40; `grain_cache`, grain.c:49).
+ * This header replaces those globals by an opaque context that owns the
+ * device buffers of one subblock, and exports one entry point per reference
+ * operator plus a fused multi-step entry point.  Every function returns an
+ * `int` status (MMCA_OK or an MMCA_ERR_* code) where the reference calls
+ * `error()` (debug.h:54-58, which aborts the MPI job).
+ *
+ * Layout convention (same as the reference, globals.h:71-74, growth.c:70):
+ * host arrays are padded boxes of (nx+2)*(ny+2)*(nz+2) cells, x fastest,
+ * idx = i + j*(nx+2) + k*(nx+2)*(ny+2), interior i in [1,nx] etc.
+ *
+ * Threading: a context is used by one host thread at a time; all work is
+ * ordered on the context's single HIP stream.
+ */
+#ifndef MMCA_H
+#define MMCA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMCA_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------- */
+#define MMCA_OK                    0
+#define MMCA_ERR_ARG               1  /* bad argument / size                  */
+#define MMCA_ERR_HIP               2  /* HIP runtime error                    */
+#define MMCA_ERR_COURANT           3  /* growth.c:43-50  rs|rl > CourantLimit */
+#define MMCA_ERR_NEGATIVE_CE       4  /* growth.c:163-165 ce < 0 (CPU build)  */
+#define MMCA_ERR_GRAIN_CAPACITY    5  /* grain.c:361-366 grain id > MaxTotGrains */
+#define MMCA_ERR_NUC_THRESHOLD     6  /* grain.c:141-154 rejection loop does not end */
+#define MMCA_ERR_NO_DEVICE         7  /* no usable GPU                        */
+#define MMCA_ERR_STATE             8  /* call out of order                    */
+
+/* ---- faces, order of the reference FACE_* constants (globals.h:46-51) ---- */
+#define MMCA_FACE_TOP    0  /* +z */
+#define MMCA_FACE_BOTTOM 1  /* -z */
+#define MMCA_FACE_LEFT   2  /* -x */
+#define MMCA_FACE_RIGHT  3  /* +x */
+#define MMCA_FACE_FRONT  4  /* -y */
+#define MMCA_FACE_BACK   5  /* +y */
+
+/* ---- temperature models (temperature.c:24-126) --------------------------- */
+#define MMCA_TEMP_UNIFORM   0  /* T = T0 - t*rate           (:36-39)  */
+#define MMCA_TEMP_GRADIENT  1  /* moving linear gradient      (:41-61)  */
+#define MMCA_TEMP_AM_BEAM   2  /* Gaussian-beam time integral (:62-123) */
+
+/* ---- storage precision of the solute fields ------------------------------ */
+#define MMCA_PREC_FP64      0  /* ce, cl, fs in FP64: bit-exact path */
+#define MMCA_PREC_FP32DEV   1  /* ce, cl stored in FP32 as deviations from c0 */
+
+/*
+ * Parameters of one subblock: the part of BB_struct (globals.h:273-403) the
+ * CA step reads, with the same meaning and units (SI).  Derived constants
+ * are folded by the library exactly as the reference folds them
+ * (growth.c:38-40, 181-189; read_ctrl.c:551).
+ */
+typedef struct mmca_params {
+    int32_t nx, ny, nz;            /* NCellsPerSB                           */
+    int32_t pad_bndy[3];           /* FaceCtrl per axis x,y,z: 1 = pad (mold wall), 0 = wrap */
+    int32_t sb_coords[3];          /* subblock coordinates (0,0,0 for one block) */
+    int32_t gdim[3];               /* global domain size in cells           */
+    double  cell_size;             /* CellSize                              */
+    double  origin[3];             /* origin_offset                         */
+    double  dt;                    /* TimeStep                              */
+    double  courant_limit;         /* CourantLimit (default 0.2)            */
+    /* alloy */
+    double  c_init;                /* Cinit                                 */
+    double  liquidus_temp;         /* LiquidusTemp                          */
+    double  solidus_temp;          /* SolidusTemp                           */
+    double  m_solute[3];           /* m_solute0 a b c; slope used = c       */
+    double  part_coef;             /* part_coef0 c                          */
+    double  d_sol, d_liq;          /* Dsol, Dliq                            */
+    double  fsgrow;                /* FSGrow (rev_fsgrow = 1.0/FSGrow)      */
+    /* temperature */
+    int32_t temp_model;            /* MMCA_TEMP_*                           */
+    int32_t am_layers;             /* NoOfLayers                            */
+    int32_t am_tracks;             /* NoOfTracks                            */
+    int32_t gnsbx;                 /* subblocks along x (AM track length)   */
+    double  t_init;                /* InitialTemperature                    */
+    double  cool_rate;             /* CoolingRate                           */
+    double  gradient, grad_coef;   /* Gradient, Grad_Coef                   */
+    double  velocity, velo_coef;   /* Velocity, Velo_Coef                   */
+    double  laser_power, laser_speed, laser_spot, laser_absorp;
+    double  laser_hatch, sub_init_temp, th_conductivity, density, specific_heat;
+    double  laser_start[3];        /* LaserStartLocation                    */
+    /* grains */
+    int64_t max_total_grains;      /* MaxTotGrains                          */
+    int32_t check_negative_ce;     /* 1: CPU-build abort on ce<0 (growth.c:163) */
+    int32_t precision;             /* MMCA_PREC_*                           */
+    uint64_t timestep;             /* starting timestep (0 for a fresh run) */
+} mmca_params;
+
+/* FixedNuc entry (read_ctrl.c:298-310): position [m], angles [deg]. */
+typedef struct mmca_fixed_nuc {
+    double x, y, z;
+    double ax, ay, az;
+    double threshold;
+} mmca_fixed_nuc;
+
+/* NucVolume entry (read_ctrl.c:311-330). */
+typedef struct mmca_nuc_volume {
+    double density;
+    double minx, miny, minz, maxx, maxy, maxz;
+    double threshold;
+} mmca_nuc_volume;
+
+/*
+ * Nucleation set-up (init_nucleation_sites grain.c:603-713 and
+ * init_sb_nucleation grain.c:931-1010).  Lists are given in control-file
+ * order; the library applies the reference's head-insertion order
+ * (ll.c:59) itself.
+ */
+typedef struct mmca_nucleation_cfg {
+    uint32_t random_seed;          /* RandSeedVal; stream = srand(rank+1+seed) */
+    int32_t  rank;                 /* MPI rank whose stream this block uses */
+    int32_t  oriented;             /* GNOriented                            */
+    int32_t  do_surface;           /* MouldNuc (default 1)                  */
+    double   bulk_density, bulk_mean, bulk_sigma;   /* MaxGrainDensity, GNGaussCentre/Sigma */
+    double   surf_density, surf_mean, surf_sigma;   /* ...Surf                   */
+    int32_t  n_fixed;
+    int32_t  n_volumes;
+    const mmca_fixed_nuc  *fixed;
+    const mmca_nuc_volume *volumes;
+} mmca_nucleation_cfg;
+
+/* Grain record, same layout and meaning as grain_t (globals.h:244-254). */
+typedef struct mmca_grain {
+    double   nuc_x, nuc_y, nuc_z;
+    uint64_t nuc_timestep;
+    double   nuc_threshold;
+    double   rotmat[3][3];
+    double   rotang[3];
+} mmca_grain;
+
+/* Per-call report of mmca_step. */
+typedef struct mmca_report {
+    uint64_t timestep;        /* bp->timestep after the call               */
+    int32_t  num_grains;      /* bp->num_grains (includes grain 0)         */
+    int32_t  new_grains;      /* grains created during the call            */
+    int32_t  gindex;          /* interface-list length of the last step    */
+    int32_t  status;          /* MMCA_OK or error of the failing step      */
+    uint64_t failed_step;     /* timestep at which status was raised       */
+    double   solid_sum;       /* sum of min(1,fs) over the interior (growth.c:541-566), if requested */
+    double   solute_sum;      /* sum of ce over the interior, if requested */
+} mmca_report;
+
+/* Field identifiers for upload/download (padded host arrays). */
+#define MMCA_FIELD_CE         0  /* double */
+#define MMCA_FIELD_CL         1  /* double */
+#define MMCA_FIELD_FS         2  /* double */
+#define MMCA_FIELD_GR         3  /* int32  */
+#define MMCA_FIELD_DC         4  /* double[3] per cell, packed x,y,z (decentered_t) */
+#define MMCA_FIELD_D          5  /* double, download only: half-diagonal of solid cells */
+#define MMCA_FIELD_TEMP       6  /* double, download only: temperature at current timestep */
+#define MMCA_FIELD_THRESHOLD  7  /* float, download only: INFINITY where no site */
+#define MMCA_FIELD_MOLD       8  /* int8, download only */
+#define MMCA_FIELD_OCE        9  /* double: ce before the last diffusion */
+
+typedef struct mmca_ctx mmca_ctx;
+
+/* Library / device */
+int  mmca_abi_version(void);
+const char *mmca_status_string(int status);
+int  mmca_device_count(int *count);
+
+/*
+ * allocateFields + SetupInitialConditions (setup.c:113-236, 42-96) and
+ * tempPrepareSB (temperature.c:156-215): ce = cl = Cinit, thresholds = INF,
+ * other fields zero, mold on padded faces.
+ */
+int  mmca_create(const mmca_params *params, int device, mmca_ctx **out);
+int  mmca_destroy(mmca_ctx *ctx);
+
+/* init_nucleation_sites + init_sb_nucleation (grain.c:603-713, 931-1010). */
+int  mmca_init_nucleation(mmca_ctx *ctx, const mmca_nucleation_cfg *cfg);
+
+/*
+ * `loop()` body without IO (loop.c:222-227): for each of nsteps,
+ * timestep++ then doiteration().  The sums in the report are computed after
+ * the last step when want_sums != 0 (computeFS, loop.c:173-182).
+ */
+int  mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report);
+
+/* --- single operators of doiteration(), for stepping phase by phase ------ */
+int  mmca_set_timestep(mmca_ctx *ctx, uint64_t timestep);          /* bp->timestep       */
+int  mmca_cell_nucleation(mmca_ctx *ctx, int32_t *n_new);            /* grain.c:1222-1318  */
+int  mmca_activate_new_grains(mmca_ctx *ctx);                        /* grain.c:841-916    */
+int  mmca_sb_diffuse_alloy_decentered(mmca_ctx *ctx);               /* growth.c:30-170    */
+int  mmca_fs_change_diffuse(mmca_ctx *ctx);                          /* growth.c:177-276   */
+int  mmca_grow_octahedra(mmca_ctx *ctx);                             /* growth.c:339-407   */
+int  mmca_grow_cell_reduction(mmca_ctx *ctx, int32_t *gindex);       /* growth.c:461-538   */
+int  mmca_capture_octahedra_diffuse(mmca_ctx *ctx);                  /* growth.c:574-807   */
+int  mmca_doiteration(mmca_ctx *ctx);                                /* calculate.c:193-352 */
+int  mmca_solid_volume(mmca_ctx *ctx, double *volume);               /* growth.c:971-978   */
+
+/* --- state transfer ------------------------------------------------------- */
+int  mmca_upload_field(mmca_ctx *ctx, int field, const void *host);
+int  mmca_download_field(mmca_ctx *ctx, int field, void *host);
+int  mmca_get_grains(mmca_ctx *ctx, mmca_grain *out, int32_t max, int32_t *num_grains);
+int  mmca_get_interface_list(mmca_ctx *ctx, int32_t *out, int32_t max, int32_t *gindex);
+int  mmca_get_timestep(mmca_ctx *ctx, uint64_t *timestep);
+int  mmca_padded_cells(mmca_ctx *ctx, int64_t *cells);
+
+/* Average duration [ms] of the dominant kernel over the last mmca_step call,
+ * measured with HIP events on the context's stream (0 when not timed). */
+int  mmca_set_kernel_timing(mmca_ctx *ctx, int enable);
+int  mmca_kernel_time_ms(mmca_ctx *ctx, double *avg_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MMCA_H */

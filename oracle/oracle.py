@@ -1,0 +1,220 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of the CPU restatement in oracle/mmca_oracle.c.  Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg import this module,
+and only as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from mumatscale_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-f", os.path.join(HERE, "oracle.mk")],
+                          cwd=os.path.dirname(HERE))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [C.POINTER(abi.Params)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_init_nucleation.argtypes = [vp, C.POINTER(abi.NucleationCfg)]
+        for name in ("orc_step",):
+            getattr(L, name).argtypes = [vp, C.c_int]
+        for name in ("orc_doiteration", "orc_cell_nucleation", "orc_activate_new_grains",
+                     "orc_diffuse", "orc_grow_cell_reduction", "orc_num_grains",
+                     "orc_gindex", "orc_status", "orc_nnuc", "orc_dc_tmp_size",
+                     "orc_taint_nucleations"):
+            getattr(L, name).argtypes = [vp]
+        for name in ("orc_temp_update", "orc_copy_oce", "orc_fs_change",
+                     "orc_grow_octahedra", "orc_capture"):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = None
+        L.orc_field.restype = vp
+        L.orc_field.argtypes = [vp, C.c_int]
+        L.orc_diff_id.restype = C.POINTER(C.c_int)
+        L.orc_diff_id.argtypes = [vp]
+        L.orc_grains.restype = C.POINTER(abi.Grain)
+        L.orc_grains.argtypes = [vp]
+        L.orc_dc_taint.restype = C.POINTER(C.c_uint8)
+        L.orc_dc_taint.argtypes = [vp]
+        L.orc_dc_tmp.restype = C.POINTER(C.c_double)
+        L.orc_dc_tmp.argtypes = [vp]
+        L.orc_reduce_fs.restype = C.c_double
+        L.orc_reduce_fs.argtypes = [vp]
+        L.orc_reduce_ce.restype = C.c_double
+        L.orc_reduce_ce.argtypes = [vp]
+        L.orc_timestep.restype = C.c_uint64
+        L.orc_timestep.argtypes = [vp]
+        L.orc_failed_step.restype = C.c_uint64
+        L.orc_failed_step.argtypes = [vp]
+        L.orc_set_timestep.argtypes = [vp, C.c_uint64]
+        L.orc_set_timestep.restype = None
+        L.orc_fnv1a.restype = C.c_uint64
+        L.orc_fnv1a.argtypes = [C.c_void_p, C.c_size_t]
+        L.orc_rng_test.argtypes = [C.c_uint, C.c_int, C.POINTER(C.c_int32)]
+        L.orc_rng_test.restype = None
+        for name in ("orc_sizeof_grain", "orc_sizeof_params", "orc_sizeof_nuc_cfg"):
+            getattr(L, name).restype = C.c_size_t
+        _lib = L
+    return _lib
+
+
+_DTYPES = {
+    abi.FIELD_CE: (np.float64, 1), abi.FIELD_CL: (np.float64, 1),
+    abi.FIELD_FS: (np.float64, 1), abi.FIELD_GR: (np.int32, 1),
+    abi.FIELD_DC: (np.float64, 3), abi.FIELD_D: (np.float64, 1),
+    abi.FIELD_TEMP: (np.float64, 1), abi.FIELD_THRESHOLD: (np.float32, 1),
+    abi.FIELD_MOLD: (np.int8, 1), abi.FIELD_OCE: (np.float64, 1),
+}
+
+
+def fnv1a(arr):
+    """FNV-1a 64 of an array's bytes (the hash oracle/ref_hook.c writes)."""
+    a = np.ascontiguousarray(arr)
+    return int(lib().orc_fnv1a(a.ctypes.data, a.nbytes))
+
+
+def rng_sequence(seed, n):
+    out = (C.c_int32 * n)()
+    lib().orc_rng_test(seed, n, out)
+    return np.frombuffer(out, dtype=np.int32).copy()
+
+
+class OracleSubblock:
+    """One subblock advanced by the CPU restatement."""
+
+    def __init__(self, params, nucleation=None):
+        self.params = params
+        self._h = lib().orc_create(C.byref(params))
+        if not self._h:
+            raise abi.MmcaError(abi.ERR_ARG, "orc_create")
+        self.shape = (params.nz + 2, params.ny + 2, params.nx + 2)
+        if nucleation is not None:
+            st = lib().orc_init_nucleation(self._h, C.byref(nucleation))
+            if st:
+                raise abi.MmcaError(st, "orc_init_nucleation")
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def field(self, which):
+        dt, comps = _DTYPES[which]
+        ptr = lib().orc_field(self._h, which)
+        n = int(np.prod(self.shape)) * comps
+        buf = (C.c_char * (n * np.dtype(dt).itemsize)).from_address(ptr)
+        a = np.frombuffer(buf, dtype=dt, count=n)
+        shp = self.shape + ((3,) if comps == 3 else ())
+        return a.reshape(shp)
+
+    def step(self, n=1):
+        return lib().orc_step(self._h, n)
+
+    def doiteration(self):
+        return lib().orc_doiteration(self._h)
+
+    @property
+    def status(self):
+        return lib().orc_status(self._h)
+
+    @property
+    def failed_step(self):
+        return lib().orc_failed_step(self._h)
+
+    @property
+    def timestep(self):
+        return lib().orc_timestep(self._h)
+
+    @timestep.setter
+    def timestep(self, t):
+        lib().orc_set_timestep(self._h, t)
+
+    @property
+    def num_grains(self):
+        return lib().orc_num_grains(self._h)
+
+    @property
+    def gindex(self):
+        return lib().orc_gindex(self._h)
+
+    def diff_id(self):
+        g = self.gindex
+        p = lib().orc_diff_id(self._h)
+        return np.ctypeslib.as_array(p, shape=(g + 1,))[1:].copy() if g else np.zeros(0, np.int32)
+
+    def grains(self):
+        n = self.num_grains
+        p = lib().orc_grains(self._h)
+        return [p[i] for i in range(n)]
+
+    def rotmats(self):
+        n = self.num_grains
+        p = lib().orc_grains(self._h)
+        arr = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(n * 17,))
+        return arr.reshape(n, 17)[:, 5:14].copy()
+
+    def dc_taint(self):
+        n = int(np.prod(self.shape))
+        return np.ctypeslib.as_array(lib().orc_dc_taint(self._h), shape=(n,)).reshape(self.shape).copy()
+
+    @property
+    def taint_nucleations(self):
+        return lib().orc_taint_nucleations(self._h)
+
+    @property
+    def dc_tmp_size(self):
+        return lib().orc_dc_tmp_size(self._h)
+
+    def solid_sum(self):
+        return lib().orc_reduce_fs(self._h)
+
+    def solute_sum(self):
+        return lib().orc_reduce_ce(self._h)
+
+    # single operators
+    def temp_update(self):
+        lib().orc_temp_update(self._h)
+
+    def cell_nucleation(self):
+        return lib().orc_cell_nucleation(self._h)
+
+    def activate_new_grains(self):
+        return lib().orc_activate_new_grains(self._h)
+
+    def copy_oce(self):
+        lib().orc_copy_oce(self._h)
+
+    def diffuse(self):
+        return lib().orc_diffuse(self._h)
+
+    def fs_change(self):
+        lib().orc_fs_change(self._h)
+
+    def grow_octahedra(self):
+        lib().orc_grow_octahedra(self._h)
+
+    def grow_cell_reduction(self):
+        return lib().orc_grow_cell_reduction(self._h)
+
+    def capture(self):
+        lib().orc_capture(self._h)

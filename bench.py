@@ -1,0 +1,158 @@
+"""Throughput of the CA step on one MI355X.
+
+Workload: directional solidification with multiple grains on a
+256 x 256 x 512 block (dx = 1 um), frozen linear temperature field
+T = T_liq - 5 K + G (z - V t) with G = 1e4 K/m and V = 1e-3 m/s, surface
+nucleation 1e10 m^-2 (mean 1 K, sigma 0.2 K) on every padded face, bulk
+nucleation 1e13 m^-3 (mean 5 K, sigma 0.5 K), random orientations from seed
+1, c0 = 4 wt%, k = 0.14, m = -2.6 K/wt%, T_liq = 933 K, D_l = 3e-9, D_s = 1e-12
+m^2/s, dt = 0.9 dx^2 / (6 D_l) = 5e-5 s, FSGrow 0.66, plain FP64.
+
+A "step" is one doiteration() (timestep++, temperature, nucleation, grain
+activation, diffusion, partitioning, growth, interface list, capture) over
+the whole block, with the state resident in HBM.  At this dt the reference
+CPU build stops at the first negative ce (growth.c:163-165, step ~25); the
+timed run uses the reference GPU build's behaviour, which has no such check.
+
+Prints one JSON line on stdout.  --gpus N with N > 1 is refused: this build
+runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mumatscale_amd import abi  # noqa: E402
+from mumatscale_amd.ctrl import Config  # noqa: E402
+
+METRIC = "Mcell-updates/s & % HBM roofline; batched/FP32 vs plain FP64 at stated error"
+HBM_PEAK_GBS = 8000.0
+B_ALG_FP64 = 56.0      # {ce, cl, fs, gr} read + written per cell-update
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def directional_config(nx=256, ny=256, nz=512):
+    c = Config()
+    c.nsub = [1, 1, 1]
+    c.sbdim = [nx, ny, nz]
+    c.pad_bndy = [1, 1, 1]
+    c.cell_size = 1.0e-6
+    c.fsgrow = 0.66
+    c.dt = 0.9 * c.cell_size ** 2 / (6 * 3.0e-9)
+    c.random_seed = 1
+    c.liquidus_temp = 933.0
+    c.solidus_temp = 850.0
+    c.c_init = 4.0
+    c.m_solute = [0.0, 0.0, -2.6]
+    c.part_coef = 0.14
+    c.d_sol = 1.0e-12
+    c.d_liq = 3.0e-9
+    c.t_init = 933.0 - 5.0
+    c.gradient = 1.0e4
+    c.velocity = 1.0e-3
+    c.bulk_density = 1.0e13
+    c.bulk_mean = 5.0
+    c.bulk_sigma = 0.5
+    c.surf_density = 1.0e10
+    c.surf_mean = 1.0
+    c.surf_sigma = 0.2
+    c.oriented = 1
+    c.max_total_grains = 1000000
+    return c
+
+
+def cpu_baseline(cfg, target_s):
+    """Oracle (scalar C port of the reference path), one thread, bounded sample."""
+    from oracle.oracle import OracleSubblock
+    t0 = time.time()
+    orc = OracleSubblock(cfg.params(check_negative_ce=False), cfg.nucleation())
+    log("cpu baseline: oracle set-up %.1f s" % (time.time() - t0))
+    n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
+    t0 = time.time()
+    orc.step(1)
+    one = time.time() - t0
+    steps = max(1, min(50, int(target_s / max(one, 1e-3))))
+    t0 = time.time()
+    orc.step(steps)
+    dt = time.time() - t0
+    orc.close()
+    return {"value": round(n * steps / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": "same workload at full size, steps 2-%d after set-up (%.1f s), one thread"
+                      % (steps + 1, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--size", type=str, default="256x256x512")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    nx, ny, nz = (int(v) for v in args.size.split("x"))
+    cfg = directional_config(nx, ny, nz)
+
+    from mumatscale_amd.lib import Subblock
+    t0 = time.time()
+    gpu = Subblock(cfg.params(check_negative_ce=False), cfg.nucleation(), device=0)
+    log("gpu set-up %.1f s, %d sites" % (time.time() - t0, 0))
+    n = nx * ny * nz
+    if args.warmup:
+        gpu.step(args.warmup)
+    gpu.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    rep = gpu.step(args.steps)        # synchronises on return
+    t1 = time.perf_counter()
+    gpu.set_kernel_timing(False)
+    k_ms, k_n = gpu.kernel_time_ms()
+    wall = t1 - t0
+    value = n * args.steps / wall / 1e6
+    log("gpu: %d steps in %.3f s, grains %d, gindex %d, k_solute avg %.4f ms over %d launches"
+        % (args.steps, wall, rep.num_grains, rep.gindex, k_ms, k_n))
+    achieved = n * B_ALG_FP64 / (k_ms * 1e-3) / 1e9 if k_ms > 0 else None
+    roof = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None, "kernel": "k_solute<true,true>",
+            "bytes_per_cell": B_ALG_FP64}
+    gpu.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, args.cpu_seconds)
+        log("cpu baseline:", cpu)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "directional solidification %dx%dx%d, plain FP64" % (nx, ny, nz),
+                   "cells": n, "dt": cfg.dt, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

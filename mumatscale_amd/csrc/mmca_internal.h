@@ -1,0 +1,93 @@
+// Internal definitions shared by the HIP kernels and the host side of the
+// CA step library.  Not part of the C ABI (include/mmca.h is).
+#pragma once
+
+#include <stdint.h>
+#include <vector>
+
+#include "../../include/mmca.h"
+
+namespace mmca {
+
+// Grid geometry and per-run constants, passed to every kernel by value.
+// The folded constants are computed on the host with the same operations
+// and order as the reference (growth.c:38-40, 181-189; read_ctrl.c:551).
+struct Consts {
+    int nx, ny, nz;         // interior cells
+    int sx;                 // nx + 2 (row pitch in cells)
+    long long sxy;          // (nx+2)*(ny+2) (plane pitch in cells)
+    long long P;            // padded cells
+    int w64;                // 64-bit mask words per padded row
+    int pad[3];             // FaceCtrl per axis
+    int off[3];             // subblock offset in global cells (sb_coords * n)
+    int gdimz;              // global z extent (2-D runs use y as the axis)
+    // diffusion (growth.c:38-40)
+    double rs, rl;
+    // partitioning (growth.c:181-189)
+    double liq_slope, k_inv, km, km_inv, m_inv, D_xt, cinit, Tliq, Tsol;
+    // nucleation undercooling m(T) = a T^2 + b T + c (grain.c:1265-1267)
+    double ma, mb, mc;
+    double rev_fsgrow;
+    long long max_total_grains;
+    int check_neg;
+    // temperature (temperature.c:24-126)
+    int am;                 // 1: Gaussian-beam model
+    int am_layers, am_tracks, gnsbx;
+    double dt, cell_size, origin[3];
+    double t_init, cool_rate, gradient, grad_coef, velocity, velo_coef;
+    double lpow, lv, lsd, lab, lhs, liniT, lthcon, rho, lcp, x0, y0, z0;
+};
+
+// Device-resident counters of one subblock.
+struct Counters {
+    unsigned long long timestep;     // bp->timestep
+    unsigned long long failed_step;  // timestep of the first error
+    int num_grains;                  // bp->num_grains (includes grain 0)
+    int random_used;                 // random orientations consumed
+    int gindex;                      // interface-list length
+    int dc_tmp_size;                 // emulated size of the reference's dc_tmp
+    int realloc_old;                 // slots zeroed this step (old size), -1 if none
+    int status;                      // MMCA_OK or first error
+    int n_events;                    // nucleation events of the last detection
+    int new_grains;                  // grains created since the host last reset it
+    int pad_[2];
+};
+
+// Device buffers of one subblock.
+struct DevBufs {
+    double *ce;
+    double *cl[2];
+    double *fs[2];
+    int *gr;
+    double *dcx, *dcy, *dcz;         // decentered octahedron centres (SoA)
+    unsigned long long *mask;        // 1 bit per padded cell: gr > 0
+    // interface list and dc_tmp emulation
+    int *diff_id;                    // slots 1..gindex
+    double *dtx, *dty, *dtz;         // dc_tmp slots 0..capacity
+    int *blk_count;                  // per-block member counts -> offsets
+    int nblk_list;
+    // nucleation sites, sorted by padded index
+    int *site_idx;
+    float *site_thr;
+    int *site_fixed;                 // index into the fixed orientation table, -1 if random
+    int *site_ls;                    // AM lsindex of the site cell
+    int *events;                     // site numbers that nucleate this step, in order
+    int nsites;
+    // grains
+    double *g_rot;                   // 9 per grain, rotmat row-major
+    int *g_cell;                     // padded index of the nucleation cell
+    unsigned long long *g_step;      // nucleation timestep
+    int *g_src;                      // >=0: fixed table index; <0: -(k+1) random draw k
+    double *fixed_rot;               // 9 per fixed nucleation point
+    double *rand_rot;                // 9 per random orientation draw
+    int rand_cap;                    // random orientations uploaded
+    // single-operator buffers (allocated on first use)
+    double *oce;
+    double *d;
+    // reductions
+    double *red_part;                // 2 * nblk_red partial sums
+    int nblk_red;
+    Counters *cnt;
+};
+
+}  // namespace mmca

@@ -1,0 +1,900 @@
+// HIP kernels of the CA step for gfx950 (CDNA4, wave64).
+//
+// One step (doiteration, calculate.c:193-352) on one subblock is five
+// launches on the context's stream:
+//   k_nucleate      cell_nucleation + activateNewGrains (grain.c:1222-1318,
+//                   841-916) over the sorted site list, one workgroup
+//   k_solute        diffusion + partitioning (growth.c:30-170, 177-276),
+//                   fused, and a 1-bit "solid" mask of gr > 0
+//   k_list_count    interface-list membership counts from the mask
+//                   (grow_cell_reduction, growth.c:461-538)
+//   k_list_scan     exclusive scan of the counts -> raster-order slots
+//   k_capture       capture_octahedra_diffuse (growth.c:574-807) with the
+//                   octahedron half-diagonal of each candidate neighbour
+//                   computed on the fly (grow_octahedra, growth.c:339-407)
+//
+// Arithmetic follows the reference's operation order and is compiled with
+// -ffp-contract=off: FP64 results are bit-identical with the reference CPU
+// build (except the Gaussian-beam temperature, which calls the device exp).
+
+#include <hip/hip_runtime.h>
+
+#include "mmca_internal.h"
+
+namespace mmca {
+
+// MIN(x,y) of globals.h:19
+__device__ __forceinline__ double ref_min(double x, double y) { return (x < y) ? x : y; }
+
+// direct_temp_calc (temperature.c:24-126); x,y,z are padded indices
+__device__ double
+temp_at(const Consts &c, double sim_time, int x, int y, int z)
+{
+    if (!c.am && c.gradient == 0)
+        return c.t_init - sim_time * c.cool_rate;
+    double rx = c.origin[0] + ((c.off[0] + x) + 0.5) * c.cell_size;
+    double ry = c.origin[1] + ((c.off[1] + y) + 0.5) * c.cell_size;
+    double rz = c.origin[2] + ((c.off[2] + z) + 0.5) * c.cell_size;
+    if (!c.am)
+    {
+        double axis = ((c.gdimz == 1) ? ry : rz) - c.cell_size * 0.5;
+        double tgrad = c.gradient + c.grad_coef * sim_time;
+        double v = c.velocity + c.velo_coef * sim_time;
+        double vt = 0.5 * (v + c.velocity) * sim_time;
+        return c.t_init + tgrad * (axis - vt);
+    }
+    double lv = c.lv;
+    double invlv = 1. / lv;
+    double ls = c.lsd / (2. * 1.414);
+    double xlen = c.gnsbx * c.nx * c.cell_size;
+    xlen = xlen + c.lsd;
+    double tline = xlen * invlv;
+    double tpause = 1.5e-4 * invlv;
+    tline = tline + tpause;
+    double trackno = 0;
+    if (c.am_tracks > 1)
+        trackno = floor(sim_time / tline);
+    double xnow = c.x0 + (sim_time - trackno * tline) * lv;
+    double ynow = c.y0 + trackno * c.lhs;
+    double znow = c.z0;
+    double lpow = c.lpow;
+    double lx = rx - xnow;
+    double ly = ry - ynow;
+    double lz = rz - znow;
+    if (!(rz <= znow + c.cell_size))
+        return 0.0;
+    const int nt = 400;
+    double Tinteg = 0.0;
+    double pi15 = 5.568328;
+    double lalpha = c.lthcon / (c.rho * c.lcp);
+    double tsteady = 20 * lalpha * invlv * invlv;
+    double ldt = tsteady / nt;
+    if (sim_time < 1e-4)
+        lpow = 0;
+    for (int i = 0; i < nt; i++)
+    {
+        double lt = (0.5 + i) * ldt;
+        double Tlocal = sqrt(lalpha * lt) * (ls * ls + 4. * lalpha * lt);
+        Tinteg += ldt *
+            exp(-((lx + lv * lt) * (lx + lv * lt) + ly * ly) / (ls * ls + 4. * lalpha * lt) -
+                (lz * lz / (4. * lalpha * lt))) / Tlocal;
+    }
+    double Tfront = c.lab * lpow * lalpha / (pi15 * c.lthcon);
+    return c.liniT + Tfront * Tinteg;
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide exclusive scan of 0/1 flags with wave ballots.  NW = waves/block.
+template <int NW>
+__device__ __forceinline__ int
+block_exscan_flag(int flag, int *total, int *s_w)
+{
+    unsigned long long bal = __ballot(flag);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int r = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0)
+        s_w[w] = __popcll(bal);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+    {
+        int v = s_w[i];
+        off += (i < w) ? v : 0;
+        tot += v;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + r;
+}
+
+// Block-wide exclusive scan of small non-negative ints.
+template <int NW>
+__device__ __forceinline__ int
+block_exscan_int(int v, int *total, int *s_w)
+{
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        int t = __shfl_up(incl, d, 64);
+        if (lane >= d)
+            incl += t;
+    }
+    if (lane == 63)
+        s_w[w] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+    {
+        int x = s_w[i];
+        off += (i < w) ? x : 0;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+// ---------------------------------------------------------------------------
+// cell_nucleation + activateNewGrains/createGrains on one rank.
+// mode: 1 = detect, 2 = activate, 4 = advance the timestep first.
+constexpr int NUC_BS = 256;
+
+__global__ void __launch_bounds__(NUC_BS)
+k_nucleate(DevBufs b, Consts c, int mode, int cur)
+{
+    __shared__ int s_w[NUC_BS / 64];
+    __shared__ int s_nev;
+    Counters *cnt = b.cnt;
+    if (cnt->status != MMCA_OK)
+        return;
+    const int tid = threadIdx.x;
+    unsigned long long ts = cnt->timestep;
+    if (mode & 4)
+        ts += 1;
+    __syncthreads();
+    if ((mode & 4) && tid == 0)
+        cnt->timestep = ts;
+    const double sim_time = ts * c.dt;
+
+    if (mode & 1)
+    {
+        int nev = 0;
+        for (int base = 0; base < b.nsites; base += NUC_BS)
+        {
+            int s = base + tid;
+            int flag = 0;
+            if (s < b.nsites)
+            {
+                int idx = b.site_idx[s];
+                int z = (int) (idx / c.sxy);
+                int rem = (int) (idx - z * c.sxy);
+                int y = rem / c.sx;
+                int x = rem - y * c.sx;
+                double T = temp_at(c, sim_time, x, y, z);
+                int layerno = 1, initnuc = 1;
+                if (c.am)
+                {
+                    // tempUpdateSB_int sets lsindex before nucleation (temperature.c:256-259)
+                    if (T > c.Tliq)
+                        b.site_ls[s] = 1;
+                    if (c.am_layers == 1)
+                        layerno = 1;
+                    else if (c.am_layers > 1 && b.site_ls[s] == 1)
+                        layerno = 1;
+                    else
+                        layerno = 0;
+                    if (sim_time <= 1e-4)
+                        initnuc = 1;
+                    else if (T >= c.Tsol)
+                        initnuc = 1;
+                    else
+                        initnuc = 0;
+                }
+                if (b.gr[idx] <= 0 && layerno && initnuc)
+                {
+                    double Tunder = c.Tliq - T;
+                    double m_solute0 = c.ma * T * T + c.mb * T + c.mc;
+                    Tunder += m_solute0 * (b.cl[cur][idx] - c.cinit);
+                    if (Tunder > (double) b.site_thr[s])
+                        flag = 1;
+                }
+            }
+            int tot;
+            int r = block_exscan_flag<NUC_BS / 64>(flag, &tot, s_w);
+            if (flag)
+                b.events[nev + r] = s;
+            nev += tot;
+        }
+        if (tid == 0)
+        {
+            s_nev = nev;
+            cnt->n_events = nev;
+        }
+        __syncthreads();
+    }
+    else
+    {
+        if (tid == 0)
+            s_nev = cnt->n_events;
+        __syncthreads();
+    }
+
+    if (!(mode & 2))
+        return;
+    const int nev = s_nev;
+    if (nev == 0)
+        return;
+    const int base_id = cnt->num_grains;
+    const int rnd0 = cnt->random_used;
+    const long long maxg = c.max_total_grains;
+    int rused = 0;
+    for (int base = 0; base < nev; base += NUC_BS)
+    {
+        int e = base + tid;
+        int isev = e < nev;
+        int s = isev ? b.events[e] : 0;
+        int fixed = isev ? b.site_fixed[s] : -1;
+        int israndom = isev && fixed < 0;
+        int rtot;
+        int rrank = block_exscan_flag<NUC_BS / 64>(israndom, &rtot, s_w);
+        if (isev)
+        {
+            long long id = (long long) base_id + e;
+            if (id <= maxg)
+            {
+                int idx = b.site_idx[s];
+                b.gr[idx] = (int) id;
+                const double *rot;
+                int src;
+                if (fixed >= 0)
+                {
+                    rot = b.fixed_rot + 9 * (long long) fixed;
+                    src = fixed;
+                }
+                else
+                {
+                    int k = rnd0 + rused + rrank;
+                    rot = b.rand_rot + 9 * (long long) k;
+                    src = -(k + 1);
+                }
+                for (int q = 0; q < 9; ++q)
+                    b.g_rot[9 * id + q] = rot[q];
+                b.g_cell[id] = idx;
+                b.g_step[id] = ts;
+                b.g_src[id] = src;
+            }
+        }
+        rused += rtot;
+    }
+    __syncthreads();
+    if (tid == 0)
+    {
+        long long total = (long long) base_id + nev;
+        if (total > maxg + 1)
+        {
+            // grain.c:868-888: with every slot taken nothing is activated;
+            // otherwise createGrains runs past MaxTotGrains and aborts
+            // (grain.c:361-366) after creating the grains that fit.
+            if ((long long) base_id != maxg + 1)
+            {
+                cnt->status = MMCA_ERR_GRAIN_CAPACITY;
+                cnt->failed_step = ts;
+            }
+        }
+        else
+        {
+            cnt->num_grains = (int) total;
+            cnt->random_used = rnd0 + rused;
+            cnt->new_grains += nev;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diffusion (growth.c:85-113, order +x,-x,+y,-y,+z,-z) and partitioning
+// (growth.c:221-273).  DIFF && FSC: fused step reading buffers `cur`,
+// writing `nxt`.  DIFF only: ce += nbsum, oce = old ce.  FSC only: in place
+// on `cur` with ce_old = oce.  FSC writes the solid mask (gr > 0).
+constexpr int SOL_BX = 64;
+constexpr int SOL_BY = 4;
+
+template <bool DIFF, bool FSC>
+__global__ void __launch_bounds__(SOL_BX *SOL_BY)
+k_solute(DevBufs b, Consts c, int cur, int nxt)
+{
+    Counters *cnt = b.cnt;
+    if (cnt->status != MMCA_OK)
+        return;
+    const int x = blockIdx.x * SOL_BX + threadIdx.x;   // padded x
+    const int j = blockIdx.y * SOL_BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (j > c.ny)
+        return;                                          // whole wave
+    const bool active = (x >= 1 && x <= c.nx);
+    int g_new = 0;
+    if (active)
+    {
+        const long long idx = k * c.sxy + (long long) j * c.sx + x;
+        const double *clc = b.cl[cur];
+        const double *fsc = b.fs[cur];
+        const double ce_old = b.ce[idx];
+        const double conc = clc[idx];
+        const double lspfs = fsc[idx];
+        double ce_new = ce_old;
+        if (DIFF)
+        {
+            const long long nb[6] = { idx + 1, idx - 1, idx + c.sx, idx - c.sx, idx + c.sxy, idx - c.sxy };
+            // mold of a neighbour: a padded face of the box (interior cells are never mold)
+            const int mo[6] = { (x == c.nx) ? c.pad[0] : 0, (x == 1) ? c.pad[0] : 0,
+                                (j == c.ny) ? c.pad[1] : 0, (j == 1) ? c.pad[1] : 0,
+                                (k == c.nz) ? c.pad[2] : 0, (k == 1) ? c.pad[2] : 0 };
+            double nbsum = 0;
+#pragma unroll
+            for (int n = 0; n < 6; ++n)
+            {
+                double fs_av = 0.5 * (lspfs + fsc[nb[n]]);
+                double r = (c.rs * fs_av + c.rl * (1 - fs_av)) * (1 - mo[n]);
+                nbsum += r * (clc[nb[n]] - conc);
+            }
+            ce_new = ce_old + nbsum;
+            if (c.check_neg && ce_new < 0)
+            {
+                if (atomicCAS(&cnt->status, MMCA_OK, MMCA_ERR_NEGATIVE_CE) == MMCA_OK)
+                    cnt->failed_step = cnt->timestep;
+            }
+            b.ce[idx] = ce_new;
+            if (!FSC)
+                b.oce[idx] = ce_old;
+        }
+        if (FSC)
+        {
+            const double ce_prev = DIFF ? ce_old : b.oce[idx];
+            const int g_old = b.gr[idx];
+            int g = g_old;
+            double cl_out, fs_out;
+            if (g <= 0)
+            {
+                cl_out = ce_new;
+                g = 0;
+                fs_out = 0;
+            }
+            else
+            {
+                const double Tcell = temp_at(c, cnt->timestep * c.dt, x, j, k);
+                double c_int = c.cinit + c.m_inv * (Tcell - c.Tliq);
+                double Tunder = c.Tliq - Tcell;
+                Tunder += c.liq_slope * (conc - c.cinit);
+                if (Tcell < 1.0 || Tcell > c.Tliq)
+                {
+                    cl_out = ce_new;
+                    g = 0;
+                    fs_out = 0;
+                }
+                else if (lspfs == 1.0 && (Tunder >= 0.0))
+                {
+                    cl_out = ce_new * c.k_inv;
+                    fs_out = lspfs;
+                }
+                else
+                {
+                    double dce = ce_new - ce_prev;
+                    double dcl = (c_int - conc) * c.D_xt * 2 + dce / (1 - c.km * lspfs);
+                    double dfs = (dcl * (1. - c.km * lspfs) - dce) * c.km_inv / conc;
+                    cl_out = conc + dcl;
+                    fs_out = lspfs + dfs;
+                    fs_out = ref_min(1.0, fs_out);
+                    if (fs_out < 0.)
+                    {
+                        fs_out = 0.;
+                        g = 0;
+                        cl_out = ce_new;
+                    }
+                }
+            }
+            b.cl[nxt][idx] = cl_out;
+            b.fs[nxt][idx] = fs_out;
+            if (g != g_old)
+                b.gr[idx] = g;
+            g_new = g;
+        }
+    }
+    if (FSC)
+    {
+        unsigned long long bits = __ballot(active && g_new > 0);
+        if (threadIdx.x == 0)
+            b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + blockIdx.x] = bits;
+    }
+}
+
+// Solid mask from gr (used when the list is built outside the fused step).
+__global__ void
+k_mask_from_gr(DevBufs b, Consts c)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    bool solid = false;
+    if (x < c.sx)
+        solid = b.gr[k * c.sxy + (long long) j * c.sx + x] > 0;
+    unsigned long long bits = __ballot(solid);
+    if (threadIdx.x == 0)
+        b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + blockIdx.x] = bits;
+}
+
+// ---------------------------------------------------------------------------
+// Interface list: interior cells with gr <= 0 and a neighbour with gr > 0
+// (growth.c:496-513), in raster order.  One thread per 64-bit mask word.
+constexpr int LIST_BS = 256;
+
+__device__ __forceinline__ unsigned long long
+mask_word(const DevBufs &b, const Consts &c, int k, int j, int w)
+{
+    return b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + w];
+}
+
+__device__ __forceinline__ unsigned long long
+member_bits(const DevBufs &b, const Consts &c, long long wi, int *pk, int *pj, int *pw)
+{
+    const long long nrow = wi / c.w64;
+    const int w = (int) (wi - nrow * c.w64);
+    const int j = (int) (nrow % c.ny) + 1;
+    const int k = (int) (nrow / c.ny) + 1;
+    *pk = k;
+    *pj = j;
+    *pw = w;
+    unsigned long long own = mask_word(b, c, k, j, w);
+    unsigned long long left = own << 1;
+    unsigned long long right = own >> 1;
+    if (w > 0)
+        left |= mask_word(b, c, k, j, w - 1) >> 63;
+    if (w + 1 < c.w64)
+        right |= mask_word(b, c, k, j, w + 1) << 63;
+    unsigned long long yz = mask_word(b, c, k, j - 1, w) | mask_word(b, c, k, j + 1, w) |
+                            mask_word(b, c, k - 1, j, w) | mask_word(b, c, k + 1, j, w);
+    // interior x in [1, nx]
+    const int x0 = w * 64;
+    unsigned long long inter = ~0ull;
+    if (x0 == 0)
+        inter &= ~1ull;
+    const int hi = c.nx + 1 - x0;                 // first non-interior bit at or above
+    if (hi < 64)
+        inter &= (hi <= 0) ? 0ull : ((1ull << hi) - 1ull);
+    return ~own & (left | right | yz) & inter;
+}
+
+__global__ void __launch_bounds__(LIST_BS)
+k_list_count(DevBufs b, Consts c, long long nwords)
+{
+    __shared__ int s_w[LIST_BS / 64];
+    if (b.cnt->status != MMCA_OK)
+        return;
+    const long long wi = (long long) blockIdx.x * LIST_BS + threadIdx.x;
+    int v = 0;
+    if (wi < nwords)
+    {
+        int k, j, w;
+        v = __popcll(member_bits(b, c, wi, &k, &j, &w));
+    }
+    int tot;
+    block_exscan_int<LIST_BS / 64>(v, &tot, s_w);
+    if (threadIdx.x == 0)
+        b.blk_count[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the block counts (one workgroup).  When decide_realloc
+// is set, also applies the dc_tmp growth rule of growth.c:644-660.
+constexpr int SCAN_BS = 1024;
+
+__global__ void __launch_bounds__(SCAN_BS)
+k_list_scan(DevBufs b, int nblk, int decide_realloc)
+{
+    __shared__ int s_w[SCAN_BS / 64];
+    Counters *cnt = b.cnt;
+    if (cnt->status != MMCA_OK)
+        return;
+    int carry = 0;
+    for (int base = 0; base < nblk; base += SCAN_BS)
+    {
+        int i = base + threadIdx.x;
+        int v = (i < nblk) ? b.blk_count[i] : 0;
+        int tot;
+        int ex = block_exscan_int<SCAN_BS / 64>(v, &tot, s_w);
+        if (i < nblk)
+            b.blk_count[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0)
+    {
+        cnt->gindex = carry;
+        if (decide_realloc)
+        {
+            if (carry > cnt->dc_tmp_size)
+            {
+                cnt->realloc_old = cnt->dc_tmp_size;
+                cnt->dc_tmp_size = carry + 100;
+            }
+            else
+                cnt->realloc_old = -1;
+        }
+    }
+}
+
+__global__ void
+k_realloc_decide(DevBufs b)
+{
+    Counters *cnt = b.cnt;
+    if (cnt->status != MMCA_OK)
+        return;
+    if (cnt->gindex > cnt->dc_tmp_size)
+    {
+        cnt->realloc_old = cnt->dc_tmp_size;
+        cnt->dc_tmp_size = cnt->gindex + 100;
+    }
+    else
+        cnt->realloc_old = -1;
+}
+
+__device__ __forceinline__ bool
+solid_at(const DevBufs &b, const Consts &c, int x, int j, int k)
+{
+    unsigned long long wd = mask_word(b, c, k, j, x >> 6);
+    return (wd >> (x & 63)) & 1ull;
+}
+
+// capture_octahedra_diffuse, INDEX_SEP branch (growth.c:640-807).
+// do_capture = 0 only writes the list (grow_cell_reduction).  `cb` is the
+// buffer holding fs/cl after fs_change.
+__global__ void __launch_bounds__(LIST_BS)
+k_capture(DevBufs b, Consts c, long long nwords, int cb, int do_capture)
+{
+    __shared__ int s_w[LIST_BS / 64];
+    Counters *cnt = b.cnt;
+    if (cnt->status != MMCA_OK)
+        return;
+    const long long wi = (long long) blockIdx.x * LIST_BS + threadIdx.x;
+    unsigned long long mem = 0;
+    int k = 0, j = 0, w = 0;
+    if (wi < nwords)
+        mem = member_bits(b, c, wi, &k, &j, &w);
+    int tot;
+    int ex = block_exscan_int<LIST_BS / 64>(__popcll(mem), &tot, s_w);
+    if (!mem)
+        return;
+    int slot = b.blk_count[blockIdx.x] + ex;      // 0-based; reference slots start at 1
+    const bool realloc = do_capture && cnt->realloc_old >= 0;
+    const long long maxg = c.max_total_grains;
+    double *fsb = b.fs[cb];
+    double *clb = b.cl[cb];
+    const int nnx[6] = { 1, 0, -1, 0, 0, 0 };
+    const int nny[6] = { 0, 1, 0, -1, 0, 0 };
+    const int nnz[6] = { 0, 0, 0, 0, 1, -1 };
+    while (mem)
+    {
+        const int bit = __ffsll((long long) mem) - 1;
+        mem &= mem - 1ull;
+        const int x = w * 64 + bit;
+        ++slot;
+        const long long idx = k * c.sxy + (long long) j * c.sx + x;
+        b.diff_id[slot] = (int) idx;
+        if (!do_capture)
+            continue;
+        double tx, ty, tz;
+        if (realloc)
+            tx = ty = tz = 0.0;
+        else
+        {
+            tx = b.dtx[slot];
+            ty = b.dty[slot];
+            tz = b.dtz[slot];
+        }
+        const long long ndxs[6] = { idx + 1, idx + c.sx, idx - 1, idx - c.sx, idx + c.sxy, idx - c.sxy };
+        const int nxs[6] = { x + 1, x, x - 1, x, x, x };
+        const int nys[6] = { j, j + 1, j, j - 1, j, j };
+        const int nzs[6] = { k, k, k, k, k + 1, k - 1 };
+        int found = -1;
+        int gfound = 0;
+        long long ndx = 0;
+        double g00 = 0, g10 = 0, g20 = 0, g01 = 0, g11 = 0, g21 = 0, g02 = 0, g12 = 0, g22 = 0;
+        double dx = 0, dy = 0, dz = 0, dn = 0, ddx = 0, ddy = 0, ddz = 0;
+        for (int n = 0; n < 6; ++n)
+        {
+            if (!solid_at(b, c, nxs[n], nys[n], nzs[n]))
+                continue;
+            ndx = ndxs[n];
+            const int gid = b.gr[ndx];
+            if (gid > 0 && gid < maxg)
+            {
+                const double *R = b.g_rot + 9 * (long long) gid;
+                g00 = R[0]; g01 = R[1]; g02 = R[2];
+                g10 = R[3]; g11 = R[4]; g12 = R[5];
+                g20 = R[6]; g21 = R[7]; g22 = R[8];
+                ddx = b.dcx[ndx];
+                ddy = b.dcy[ndx];
+                ddz = b.dcz[ndx];
+                // grow_octahedra (growth.c:392-403)
+                double rx0 = 0.0 - ddx;
+                double ry0 = 0.0 - ddy;
+                double rz0 = 0.0 - ddz;
+                double gx0 = g00 * rx0 + g10 * ry0 + g20 * rz0;
+                double gy0 = g01 * rx0 + g11 * ry0 + g21 * rz0;
+                double gz0 = g02 * rx0 + g12 * ry0 + g22 * rz0;
+                double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
+                ds1 = ref_min(ds1, 1.5);
+                dn = ds1 + fsb[ndx] * c.rev_fsgrow;
+                // growth.c:717-735
+                double rx1 = -1.0 * nnx[n] - ddx;
+                double ry1 = -1.0 * nny[n] - ddy;
+                double rz1 = -1.0 * nnz[n] - ddz;
+                dx = g00 * rx1 + g10 * ry1 + g20 * rz1;
+                dy = g01 * rx1 + g11 * ry1 + g21 * rz1;
+                dz = g02 * rx1 + g12 * ry1 + g22 * rz1;
+                double dfs = dn - (fabs(dx) + fabs(dy) + fabs(dz));
+                if (dfs > 0)
+                {
+                    found = n;
+                    gfound = gid;
+                    break;
+                }
+            }
+        }
+        if (found > -1)
+        {
+            double ncx = 0., ncz = 0., ncy = 0.;
+            double ncd = ref_min(dn, 1.0);
+            double dd = dn - ncd;
+            if (dx + dy >= 0 && dx - dy > 0 && dx + dz > 0 && dx - dz >= 0)
+                ncx = dd;
+            else if (dx + dy <= 0 && dx - dy < 0 && dx + dz < 0 && dx - dz <= 0)
+                ncx = -1. * dd;
+            else if (dy + dx > 0 && dy - dx >= 0 && dy + dz >= 0 && dy - dz > 0)
+                ncy = dd;
+            else if (dy + dx < 0 && dy - dx <= 0 && dy + dz <= 0 && dy - dz < 0)
+                ncy = -1. * dd;
+            else if (dz + dx >= 0 && dz - dx > 0 && dz + dy > 0 && dz - dy >= 0)
+                ncz = dd;
+            else
+                ncz = -1. * dd;
+            tx = ddx + nnx[found] + g00 * ncx + g01 * ncy + g02 * ncz;
+            ty = ddy + nny[found] + g10 * ncx + g11 * ncy + g12 * ncz;
+            tz = ddz + nnz[found] + g20 * ncx + g21 * ncy + g22 * ncz;
+            b.gr[idx] = gfound;
+            fsb[idx] = 0.;
+            clb[idx] = b.ce[idx];
+        }
+        if (found > -1 || realloc)
+        {
+            b.dtx[slot] = tx;
+            b.dty[slot] = ty;
+            b.dtz[slot] = tz;
+        }
+        b.dcx[idx] = tx;
+        b.dcy[idx] = ty;
+        b.dcz[idx] = tz;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sum of min(1,fs) (reduce_fs, growth.c:541-566) and of ce over the interior.
+constexpr int RED_BS = 256;
+
+__global__ void __launch_bounds__(RED_BS)
+k_reduce_partial(DevBufs b, Consts c, int cb)
+{
+    __shared__ double s_a[RED_BS], s_b[RED_BS];
+    double a = 0.0, s = 0.0;
+    const long long N = (long long) c.nx * c.ny * c.nz;
+    const double *fs = b.fs[cb];
+    for (long long i = (long long) blockIdx.x * RED_BS + threadIdx.x; i < N;
+         i += (long long) gridDim.x * RED_BS)
+    {
+        long long row = i / c.nx;
+        int x = (int) (i - row * c.nx) + 1;
+        int j = (int) (row % c.ny) + 1;
+        int k = (int) (row / c.ny) + 1;
+        long long idx = k * c.sxy + (long long) j * c.sx + x;
+        a += ref_min(1.0, fs[idx]);
+        s += b.ce[idx];
+    }
+    s_a[threadIdx.x] = a;
+    s_b[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = RED_BS / 2; d > 0; d >>= 1)
+    {
+        if (threadIdx.x < d)
+        {
+            s_a[threadIdx.x] += s_a[threadIdx.x + d];
+            s_b[threadIdx.x] += s_b[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        b.red_part[2 * blockIdx.x] = s_a[0];
+        b.red_part[2 * blockIdx.x + 1] = s_b[0];
+    }
+}
+
+__global__ void __launch_bounds__(RED_BS)
+k_reduce_final(DevBufs b, int nblk, double *out)
+{
+    __shared__ double s_a[RED_BS], s_b[RED_BS];
+    double a = 0.0, s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += RED_BS)
+    {
+        a += b.red_part[2 * i];
+        s += b.red_part[2 * i + 1];
+    }
+    s_a[threadIdx.x] = a;
+    s_b[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = RED_BS / 2; d > 0; d >>= 1)
+    {
+        if (threadIdx.x < d)
+        {
+            s_a[threadIdx.x] += s_a[threadIdx.x + d];
+            s_b[threadIdx.x] += s_b[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        out[0] = s_a[0];
+        out[1] = s_b[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Derived fields for download.
+__global__ void
+k_fill_temp(DevBufs b, Consts c, double *out)
+{
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.P)
+        return;
+    int z = (int) (i / c.sxy);
+    long long rem = i - z * c.sxy;
+    int y = (int) (rem / c.sx);
+    int x = (int) (rem - (long long) y * c.sx);
+    out[i] = temp_at(c, b.cnt->timestep * c.dt, x, y, z);
+}
+
+// grow_octahedra (growth.c:339-407) into the d buffer
+__global__ void
+k_grow_octahedra(DevBufs b, Consts c, int cb)
+{
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.P || b.cnt->status != MMCA_OK)
+        return;
+    int z = (int) (i / c.sxy);
+    long long rem = i - z * c.sxy;
+    int y = (int) (rem / c.sx);
+    int x = (int) (rem - (long long) y * c.sx);
+    if (x < 1 || x > c.nx || y < 1 || y > c.ny || z < 1 || z > c.nz)
+        return;
+    int grid = b.gr[i];
+    if (grid > 0 && grid < c.max_total_grains)
+    {
+        const double *R = b.g_rot + 9 * (long long) grid;
+        double rx0 = 0.0 - b.dcx[i];
+        double ry0 = 0.0 - b.dcy[i];
+        double rz0 = 0.0 - b.dcz[i];
+        double gx0 = R[0] * rx0 + R[3] * ry0 + R[6] * rz0;
+        double gy0 = R[1] * rx0 + R[4] * ry0 + R[7] * rz0;
+        double gz0 = R[2] * rx0 + R[5] * ry0 + R[8] * rz0;
+        double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
+        ds1 = ref_min(ds1, 1.5);
+        b.d[i] = ds1 + b.fs[cb][i] * c.rev_fsgrow;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side).
+static inline dim3 sol_grid(const Consts &c)
+{
+    return dim3((c.sx + SOL_BX - 1) / SOL_BX, (c.ny + SOL_BY - 1) / SOL_BY, c.nz);
+}
+
+hipError_t launch_nucleate(const DevBufs &b, const Consts &c, int mode, int cur, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nucleate, dim3(1), dim3(NUC_BS), 0, s, b, c, mode, cur);
+    return hipGetLastError();
+}
+
+hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
+                         hipStream_t s)
+{
+    dim3 blk(SOL_BX, SOL_BY);
+    if (diff && fsc)
+        hipLaunchKernelGGL((k_solute<true, true>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+    else if (diff)
+        hipLaunchKernelGGL((k_solute<true, false>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+    else
+        hipLaunchKernelGGL((k_solute<false, true>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_from_gr(const DevBufs &b, const Consts &c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mask_from_gr, dim3(c.w64, c.ny + 2, c.nz + 2), dim3(64), 0, s, b, c);
+    return hipGetLastError();
+}
+
+long long list_words(const Consts &c) { return (long long) c.nz * c.ny * c.w64; }
+int list_blocks(const Consts &c) { return (int) ((list_words(c) + LIST_BS - 1) / LIST_BS); }
+
+hipError_t launch_list(const DevBufs &b, const Consts &c, int cb, int capture, int decide_realloc,
+                       hipStream_t s)
+{
+    long long nw = list_words(c);
+    int nblk = list_blocks(c);
+    hipLaunchKernelGGL(k_list_count, dim3(nblk), dim3(LIST_BS), 0, s, b, c, nw);
+    hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(SCAN_BS), 0, s, b, nblk, decide_realloc);
+    if (capture >= 0)
+        hipLaunchKernelGGL(k_capture, dim3(nblk), dim3(LIST_BS), 0, s, b, c, nw, cb, capture);
+    return hipGetLastError();
+}
+
+hipError_t launch_capture_only(const DevBufs &b, const Consts &c, int cb, hipStream_t s)
+{
+    long long nw = list_words(c);
+    int nblk = list_blocks(c);
+    hipLaunchKernelGGL(k_realloc_decide, dim3(1), dim3(1), 0, s, b);
+    hipLaunchKernelGGL(k_capture, dim3(nblk), dim3(LIST_BS), 0, s, b, c, nw, cb, 1);
+    return hipGetLastError();
+}
+
+int reduce_blocks(const Consts &c)
+{
+    long long N = (long long) c.nx * c.ny * c.nz;
+    long long nb = (N + RED_BS * 8 - 1) / (RED_BS * 8);
+    if (nb > 2048)
+        nb = 2048;
+    if (nb < 1)
+        nb = 1;
+    return (int) nb;
+}
+
+hipError_t launch_reduce(const DevBufs &b, const Consts &c, int cb, double *out2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_partial, dim3(b.nblk_red), dim3(RED_BS), 0, s, b, c, cb);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_BS), 0, s, b, b.nblk_red, out2);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_temp(const DevBufs &b, const Consts &c, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_temp, dim3((unsigned) ((c.P + 255) / 256)), dim3(256), 0, s, b, c, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_grow_octahedra(const DevBufs &b, const Consts &c, int cb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_grow_octahedra, dim3((unsigned) ((c.P + 255) / 256)), dim3(256), 0, s, b, c, cb);
+    return hipGetLastError();
+}
+
+}  // namespace mmca
+
+namespace mmca {
+
+__global__ void
+k_fill_f64(double *p, long long n, double v)
+{
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+hipError_t launch_fill_f64(double *p, long long n, double v, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace mmca

@@ -1,0 +1,221 @@
+"""Python binding of the CA step library (libmmca.so, C ABI in include/mmca.h).
+
+`Subblock` mirrors the reference's per-subblock operator API: one method per
+function that `doiteration()` calls (calculate.c:193-352), plus `step()` for
+`loop()`'s timestep++/doiteration pair (loop.c:222-227).  Errors that make the
+reference call error() (debug.h:54-58) raise `MmcaError`.
+
+There is no CPU fallback: without the built library or without a GPU the
+constructor raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmmca.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("%s is not built: run __graft_entry__.build() (make -C mumatscale_amd/csrc)"
+                           % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    i32p = C.POINTER(C.c_int32)
+    L.mmca_abi_version.restype = C.c_int
+    L.mmca_status_string.restype = C.c_char_p
+    L.mmca_status_string.argtypes = [C.c_int]
+    L.mmca_device_count.argtypes = [i32p]
+    L.mmca_create.argtypes = [C.POINTER(abi.Params), C.c_int, C.POINTER(vp)]
+    L.mmca_destroy.argtypes = [vp]
+    L.mmca_init_nucleation.argtypes = [vp, C.POINTER(abi.NucleationCfg)]
+    L.mmca_step.argtypes = [vp, C.c_int, C.c_int, C.POINTER(abi.Report)]
+    L.mmca_set_timestep.argtypes = [vp, C.c_uint64]
+    L.mmca_get_timestep.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.mmca_cell_nucleation.argtypes = [vp, i32p]
+    L.mmca_grow_cell_reduction.argtypes = [vp, i32p]
+    for n in ("mmca_activate_new_grains", "mmca_sb_diffuse_alloy_decentered",
+              "mmca_fs_change_diffuse", "mmca_grow_octahedra",
+              "mmca_capture_octahedra_diffuse", "mmca_doiteration"):
+        getattr(L, n).argtypes = [vp]
+    L.mmca_solid_volume.argtypes = [vp, C.POINTER(C.c_double)]
+    L.mmca_upload_field.argtypes = [vp, C.c_int, vp]
+    L.mmca_download_field.argtypes = [vp, C.c_int, vp]
+    L.mmca_get_grains.argtypes = [vp, C.POINTER(abi.Grain), C.c_int32, i32p]
+    L.mmca_get_interface_list.argtypes = [vp, i32p, C.c_int32, i32p]
+    L.mmca_padded_cells.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.mmca_set_kernel_timing.argtypes = [vp, C.c_int]
+    L.mmca_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    if L.mmca_abi_version() != abi.ABI_VERSION:
+        raise RuntimeError("libmmca ABI version mismatch")
+    _lib = L
+    return L
+
+
+def device_count():
+    n = C.c_int32(0)
+    lib().mmca_device_count(C.byref(n))
+    return n.value
+
+
+_FIELD_DTYPES = {
+    abi.FIELD_CE: (np.float64, 1), abi.FIELD_CL: (np.float64, 1),
+    abi.FIELD_FS: (np.float64, 1), abi.FIELD_GR: (np.int32, 1),
+    abi.FIELD_DC: (np.float64, 3), abi.FIELD_D: (np.float64, 1),
+    abi.FIELD_TEMP: (np.float64, 1), abi.FIELD_THRESHOLD: (np.float32, 1),
+    abi.FIELD_MOLD: (np.int8, 1), abi.FIELD_OCE: (np.float64, 1),
+}
+
+
+def _check(st, where):
+    if st != abi.OK:
+        raise abi.MmcaError(st, where)
+
+
+class Subblock:
+    """One subblock whose state lives on one GPU."""
+
+    def __init__(self, params, nucleation=None, device=0):
+        self.params = params
+        self.shape = (params.nz + 2, params.ny + 2, params.nx + 2)
+        h = C.c_void_p()
+        st = lib().mmca_create(C.byref(params), device, C.byref(h))
+        self._h = h
+        if st != abi.OK:
+            self.close()
+            raise abi.MmcaError(st, "mmca_create")
+        if nucleation is not None:
+            self.init_nucleation(nucleation)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().mmca_destroy(self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- set-up -----------------------------------------------------------
+    def init_nucleation(self, cfg):
+        """init_nucleation_sites + init_sb_nucleation (grain.c:603-713, 931-1010)."""
+        _check(lib().mmca_init_nucleation(self._h, C.byref(cfg)), "mmca_init_nucleation")
+
+    # ---- stepping ---------------------------------------------------------
+    def step(self, n=1, want_sums=False, raise_on_error=True):
+        """n x (timestep++, doiteration()); returns the mmca_report."""
+        rep = abi.Report()
+        st = lib().mmca_step(self._h, n, 1 if want_sums else 0, C.byref(rep))
+        if raise_on_error:
+            _check(st, "mmca_step")
+        return rep
+
+    def doiteration(self):
+        _check(lib().mmca_doiteration(self._h), "mmca_doiteration")
+
+    def cell_nucleation(self):
+        n = C.c_int32(0)
+        _check(lib().mmca_cell_nucleation(self._h, C.byref(n)), "mmca_cell_nucleation")
+        return n.value
+
+    def activate_new_grains(self):
+        _check(lib().mmca_activate_new_grains(self._h), "mmca_activate_new_grains")
+
+    def sb_diffuse_alloy_decentered(self):
+        _check(lib().mmca_sb_diffuse_alloy_decentered(self._h), "mmca_sb_diffuse_alloy_decentered")
+
+    def fs_change_diffuse(self):
+        _check(lib().mmca_fs_change_diffuse(self._h), "mmca_fs_change_diffuse")
+
+    def grow_octahedra(self):
+        _check(lib().mmca_grow_octahedra(self._h), "mmca_grow_octahedra")
+
+    def grow_cell_reduction(self):
+        g = C.c_int32(0)
+        _check(lib().mmca_grow_cell_reduction(self._h, C.byref(g)), "mmca_grow_cell_reduction")
+        return g.value
+
+    def capture_octahedra_diffuse(self):
+        _check(lib().mmca_capture_octahedra_diffuse(self._h), "mmca_capture_octahedra_diffuse")
+
+    def solid_volume(self):
+        v = C.c_double(0)
+        _check(lib().mmca_solid_volume(self._h, C.byref(v)), "mmca_solid_volume")
+        return v.value
+
+    # ---- state ------------------------------------------------------------
+    @property
+    def timestep(self):
+        t = C.c_uint64(0)
+        _check(lib().mmca_get_timestep(self._h, C.byref(t)), "mmca_get_timestep")
+        return t.value
+
+    @timestep.setter
+    def timestep(self, t):
+        _check(lib().mmca_set_timestep(self._h, t), "mmca_set_timestep")
+
+    def download(self, field):
+        dt, comps = _FIELD_DTYPES[field]
+        shp = self.shape + ((3,) if comps == 3 else ())
+        out = np.empty(shp, dtype=dt)
+        _check(lib().mmca_download_field(self._h, field, out.ctypes.data), "mmca_download_field")
+        return out
+
+    def upload(self, field, arr):
+        dt, comps = _FIELD_DTYPES[field]
+        a = np.ascontiguousarray(arr, dtype=dt)
+        shp = self.shape + ((3,) if comps == 3 else ())
+        if a.shape != shp:
+            raise ValueError("field shape %s, expected %s" % (a.shape, shp))
+        _check(lib().mmca_upload_field(self._h, field, a.ctypes.data), "mmca_upload_field")
+
+    @property
+    def num_grains(self):
+        n = C.c_int32(0)
+        _check(lib().mmca_get_grains(self._h, None, 0, C.byref(n)), "mmca_get_grains")
+        return n.value
+
+    def grains(self):
+        n = self.num_grains
+        arr = (abi.Grain * max(1, n))()
+        _check(lib().mmca_get_grains(self._h, arr, n, None), "mmca_get_grains")
+        return [arr[i] for i in range(n)]
+
+    def interface_list(self):
+        g = C.c_int32(0)
+        _check(lib().mmca_get_interface_list(self._h, None, 0, C.byref(g)), "mmca_get_interface_list")
+        out = np.zeros(g.value, dtype=np.int32)
+        if g.value:
+            _check(lib().mmca_get_interface_list(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                 g.value, None), "mmca_get_interface_list")
+        return out
+
+    @property
+    def padded_cells(self):
+        n = C.c_int64(0)
+        _check(lib().mmca_padded_cells(self._h, C.byref(n)), "mmca_padded_cells")
+        return n.value
+
+    def set_kernel_timing(self, enable=True):
+        _check(lib().mmca_set_kernel_timing(self._h, 1 if enable else 0), "mmca_set_kernel_timing")
+
+    def kernel_time_ms(self):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        _check(lib().mmca_kernel_time_ms(self._h, C.byref(ms), C.byref(n)), "mmca_kernel_time_ms")
+        return ms.value, n.value

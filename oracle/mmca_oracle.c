@@ -21,6 +21,7 @@
  *   - fs_change:       per-cell closed forms of each branch
  *   - capture corner:  truth table of the 5-test chain
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -706,12 +707,10 @@ nucleate_grain(orc_state *s, const orc_nuc *nuc, mmca_grain *grain)
 {
     double c0, s0, c1, s1, c2, s2;
     double a0 = nuc->gx, a1 = nuc->gy, a2 = nuc->gz;
-    c0 = cos(a0);
-    s0 = sin(a0);
-    c1 = cos(a1);
-    s1 = sin(a1);
-    c2 = cos(a2);
-    s2 = sin(a2);
+    /* the reference's gcc build turns each cos/sin pair into one sincos() */
+    sincos(a0, &s0, &c0);
+    sincos(a1, &s1, &c1);
+    sincos(a2, &s2, &c2);
     grain->rotang[0] = nuc->gx;
     grain->rotang[1] = nuc->gy;
     grain->rotang[2] = nuc->gz;

@@ -1,0 +1,174 @@
+"""HIP path (through the C ABI) against the oracle, bit for bit.
+
+Every golden case whose fixtures pin the oracle to the reference
+(tests/golden/) is advanced on the GPU and on the oracle from the same
+control file; fields are compared bit for bit at checkpoints: ce, cl, fs,
+gr, the interface list, the grain table and dc (dc where the oracle marks no
+cell as read from the reference's out-of-bounds dc_tmp slot).  Error cases
+must stop at the same step with the same error.
+"""
+import numpy as np
+import pytest
+
+from mumatscale_amd import abi
+from mumatscale_amd.lib import Subblock
+from oracle.oracle import OracleSubblock
+from golden_cases import CASES, grain_table_bytes, load_case
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.int64) if a.dtype == np.float64 else a
+
+
+def assert_same_state(gpu, orc, where, check_d=False):
+    for f, name in ((abi.FIELD_CE, "ce"), (abi.FIELD_CL, "cl"), (abi.FIELD_FS, "fs"),
+                    (abi.FIELD_GR, "gr")):
+        a = gpu.download(f)
+        b = orc.field(f)
+        if not np.array_equal(bits(a), bits(b)):
+            bad = np.argwhere(bits(a) != bits(b))
+            z, y, x = bad[0]
+            pytest.fail("%s: %s differs at %d cells, first (z,y,x)=(%d,%d,%d): gpu %r oracle %r"
+                        % (where, name, len(bad), z, y, x, a[z, y, x], b[z, y, x]))
+    dc_g = gpu.download(abi.FIELD_DC)
+    dc_o = orc.field(abi.FIELD_DC)
+    ok = orc.dc_taint() == 0
+    if not np.array_equal(bits(dc_g)[ok], bits(dc_o)[ok]):
+        pytest.fail("%s: dc differs" % where)
+    assert gpu.num_grains == orc.num_grains, where
+    assert np.array_equal(gpu.interface_list(), orc.diff_id()), where + ": interface list"
+    ng = orc.num_grains
+    assert grain_table_bytes(gpu.grains(), ng) == grain_table_bytes(orc.grains(), ng), where + ": grains"
+    if check_d:
+        # d of cells solid before the last capture (the others hold stale values
+        # in the reference and are never read)
+        gr = orc.field(abi.FIELD_GR)
+        listed = np.zeros(gr.size, bool)
+        listed[orc.diff_id()] = True
+        sel = (gr.ravel() > 0) & ~listed
+        assert np.array_equal(bits(gpu.download(abi.FIELD_D)).ravel()[sel],
+                              bits(orc.field(abi.FIELD_D)).ravel()[sel]), where + ": d"
+
+
+def checkpoints(last):
+    pts = list(range(1, min(last, 20) + 1))
+    pts += list(range(40, last + 1, 40))
+    if pts[-1] != last:
+        pts.append(last)
+    return pts
+
+
+BIT_EXACT_CASES = [c for c in CASES if c != "c3_am_beam"]
+
+
+@pytest.mark.parametrize("case", BIT_EXACT_CASES)
+def test_gpu_matches_oracle(case):
+    cfg, hashes, expected, _ = load_case(case)
+    last = expected["last_step"]
+    print("case", case, "steps", last, flush=True)
+    orc = OracleSubblock(cfg.params(check_negative_ce=True), cfg.nucleation())
+    with Subblock(cfg.params(check_negative_ce=True), cfg.nucleation()) as gpu:
+        thr = gpu.download(abi.FIELD_THRESHOLD)
+        assert np.array_equal(thr, orc.field(abi.FIELD_THRESHOLD))
+        done = 0
+        for t in checkpoints(last):
+            assert orc.step(t - done) == abi.OK
+            rep = gpu.step(t - done)
+            done = t
+            assert rep.timestep == t
+            assert_same_state(gpu, orc, "%s step %d" % (case, t), check_d=(t % 200 == 0))
+        if expected["returncode"] != 0:
+            st_o = orc.step(1)
+            rep = gpu.step(1, raise_on_error=False)
+            assert rep.status == st_o != abi.OK
+            assert rep.failed_step == orc.failed_step == last + 1
+            with pytest.raises(abi.MmcaError):
+                gpu.step(1)
+
+
+def test_gpu_single_operators_match_oracle():
+    """Each operator of doiteration() called on its own (calculate.c:247-343)."""
+    cfg, _, _, _ = load_case("c1_directional")
+    orc = OracleSubblock(cfg.params(), cfg.nucleation())
+    with Subblock(cfg.params(), cfg.nucleation()) as gpu:
+        for t in range(1, 41):
+            where = "step %d" % t
+            orc.timestep = t
+            gpu.timestep = t
+            orc.temp_update()
+            n_o = orc.cell_nucleation()
+            assert gpu.cell_nucleation() == n_o, where
+            orc.activate_new_grains()
+            gpu.activate_new_grains()
+            assert np.array_equal(gpu.download(abi.FIELD_GR), orc.field(abi.FIELD_GR)), where
+            orc.copy_oce()
+            assert orc.diffuse() == abi.OK
+            gpu.sb_diffuse_alloy_decentered()
+            assert np.array_equal(bits(gpu.download(abi.FIELD_CE)), bits(orc.field(abi.FIELD_CE))), where
+            assert np.array_equal(bits(gpu.download(abi.FIELD_OCE)), bits(orc.field(abi.FIELD_OCE))), where
+            orc.fs_change()
+            gpu.fs_change_diffuse()
+            for f in (abi.FIELD_CL, abi.FIELD_FS, abi.FIELD_GR):
+                assert np.array_equal(bits(gpu.download(f)), bits(orc.field(f))), where
+            orc.grow_octahedra()
+            gpu.grow_octahedra()
+            gr = orc.field(abi.FIELD_GR)
+            sel = (gr > 0) & (gr < cfg.max_total_grains)
+            assert np.array_equal(bits(gpu.download(abi.FIELD_D))[sel], bits(orc.field(abi.FIELD_D))[sel]), where
+            assert gpu.grow_cell_reduction() == orc.grow_cell_reduction(), where
+            assert np.array_equal(gpu.interface_list(), orc.diff_id()), where
+            orc.capture()
+            gpu.capture_octahedra_diffuse()
+            assert_same_state(gpu, orc, where)
+
+
+def test_gpu_temperature_field_matches_oracle():
+    """tempUpdate (temperature.c:219-272) for the three temperature models."""
+    for case in ("c0_single_grain_half_dt", "c1_directional", "c3_am_beam"):
+        cfg, _, _, _ = load_case(case)
+        orc = OracleSubblock(cfg.params())
+        with Subblock(cfg.params()) as gpu:
+            for t in (0, 1, 57, 150, 299):
+                orc.timestep = t
+                gpu.timestep = t
+                orc.temp_update()
+                a = gpu.download(abi.FIELD_TEMP)
+                b = orc.field(abi.FIELD_TEMP)
+                if case == "c3_am_beam":
+                    # device exp/sqrt against glibc's: last-bit differences only
+                    assert np.allclose(a, b, rtol=1e-13, atol=0), (case, t, np.max(np.abs(a - b)))
+                else:
+                    assert np.array_equal(bits(a), bits(b)), (case, t)
+
+
+def test_gpu_am_beam_exact_while_beam_off_then_close():
+    """Gaussian-beam case: bit-exact while the beam is off (t <= 1e-4 s,
+    temperature.c:102-103 makes T = T_sub exactly), then the device exp makes
+    last-bit differences; the run is compared on grain count and solid
+    fraction at the end (tolerance 1 %)."""
+    cfg, _, expected, _ = load_case("c3_am_beam")
+    orc = OracleSubblock(cfg.params(), cfg.nucleation())
+    with Subblock(cfg.params(), cfg.nucleation()) as gpu:
+        orc.step(99)
+        gpu.step(99)
+        assert_same_state(gpu, orc, "am step 99")
+        last = expected["last_step"]
+        orc.step(last - 99)
+        rep = gpu.step(last - 99, want_sums=True)
+        n = cfg.params().nx * cfg.params().ny * cfg.params().nz
+        assert abs(rep.num_grains - orc.num_grains) <= 0.01 * orc.num_grains
+        assert abs(rep.solid_sum / n - orc.solid_sum() / n) <= 0.01
+
+
+def test_gpu_solid_volume_and_sums():
+    cfg, _, _, _ = load_case("c1_directional")
+    orc = OracleSubblock(cfg.params(), cfg.nucleation())
+    with Subblock(cfg.params(), cfg.nucleation()) as gpu:
+        orc.step(300)
+        rep = gpu.step(300, want_sums=True)
+        assert rep.solid_sum == pytest.approx(orc.solid_sum(), rel=1e-13)
+        assert rep.solute_sum == pytest.approx(orc.solute_sum(), rel=1e-13)
+        assert gpu.solid_volume() == pytest.approx(orc.solid_sum() * cfg.cell_size ** 3, rel=1e-13)

@@ -212,6 +212,17 @@ int  mmca_get_interface_list(mmca_ctx *ctx, int32_t *out, int32_t max, int32_t *
 int  mmca_get_timestep(mmca_ctx *ctx, uint64_t *timestep);
 int  mmca_padded_cells(mmca_ctx *ctx, int64_t *cells);
 
+/* --- host set-up without a device (for checking the set-up on a CPU) ----- */
+/* Sites that mmca_init_nucleation would create: padded index, threshold and
+ * matching FixedNuc/NucVolume entry (-1: random orientation), raster order. */
+int  mmca_nucleation_sites(const mmca_params *params, const mmca_nucleation_cfg *cfg,
+                           int32_t *idx_out, float *thr_out, int32_t *fixed_out, int64_t max,
+                           int64_t *n_sites);
+/* The first n random orientations drawn after that set-up (grain.c:118-129,
+ * 3 draws each): Euler angles [rad] and row-major rotation matrices. */
+int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg *cfg, int32_t n,
+                            double *angles_out, double *rotmat_out);
+
 /* Average duration [ms] of the dominant kernel over the last mmca_step call,
  * measured with HIP events on the context's stream (0 when not timed). */
 int  mmca_set_kernel_timing(mmca_ctx *ctx, int enable);

@@ -53,10 +53,43 @@ def lib():
     L.mmca_padded_cells.argtypes = [vp, C.POINTER(C.c_int64)]
     L.mmca_set_kernel_timing.argtypes = [vp, C.c_int]
     L.mmca_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.mmca_nucleation_sites.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg),
+                                        i32p, C.POINTER(C.c_float), i32p, C.c_int64,
+                                        C.POINTER(C.c_int64)]
+    L.mmca_orientation_draws.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg),
+                                         C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     if L.mmca_abi_version() != abi.ABI_VERSION:
         raise RuntimeError("libmmca ABI version mismatch")
     _lib = L
     return L
+
+
+def nucleation_sites(params, cfg):
+    """Sites the set-up creates (padded index, threshold, matching fixed entry);
+    runs on the host only."""
+    n = C.c_int64(0)
+    _check(lib().mmca_nucleation_sites(C.byref(params), C.byref(cfg), None, None, None, 0,
+                                       C.byref(n)), "mmca_nucleation_sites")
+    idx = np.zeros(n.value, np.int32)
+    thr = np.zeros(n.value, np.float32)
+    fixed = np.zeros(n.value, np.int32)
+    _check(lib().mmca_nucleation_sites(C.byref(params), C.byref(cfg),
+                                       idx.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       thr.ctypes.data_as(C.POINTER(C.c_float)),
+                                       fixed.ctypes.data_as(C.POINTER(C.c_int32)), n.value,
+                                       C.byref(n)), "mmca_nucleation_sites")
+    return idx, thr, fixed
+
+
+def orientation_draws(params, cfg, n):
+    """First n random orientations after the set-up: (angles[n,3], rotmat[n,3,3])."""
+    ang = np.zeros((n, 3))
+    rot = np.zeros((n, 3, 3))
+    _check(lib().mmca_orientation_draws(C.byref(params), C.byref(cfg), n,
+                                        ang.ctypes.data_as(C.POINTER(C.c_double)),
+                                        rot.ctypes.data_as(C.POINTER(C.c_double))),
+           "mmca_orientation_draws")
+    return ang, rot
 
 
 def device_count():

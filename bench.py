@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=str, default="256x256x512")
+    ap.add_argument("--zm", type=int, default=16,
+                    help="z-marching solute kernel, planes per workgroup (0: cell-parallel)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
@@ -108,7 +110,8 @@ def main():
     from mumatscale_amd.lib import Subblock
     t0 = time.time()
     gpu = Subblock(cfg.params(check_negative_ce=False), cfg.nucleation(), device=0)
-    log("gpu set-up %.1f s, %d sites" % (time.time() - t0, 0))
+    gpu.set_tuning(abi.TUNE_ZM_PLANES, args.zm)
+    log("gpu set-up %.1f s" % (time.time() - t0))
     n = nx * ny * nz
     if args.warmup:
         gpu.step(args.warmup)
@@ -126,7 +129,8 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None, "kernel": "k_solute<true,true>",
+            "traffic": None,
+            "kernel": "k_solute_zm<4>" if args.zm > 0 else "k_solute<true,true>",
             "bytes_per_cell": B_ALG_FP64}
     gpu.close()
     cpu = None
@@ -147,7 +151,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "directional solidification %dx%dx%d, plain FP64" % (nx, ny, nz),
-                   "cells": n, "dt": cfg.dt, "parallelism": "single GPU"},
+                   "cells": n, "dt": cfg.dt, "zm_planes": args.zm, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

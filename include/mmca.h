@@ -223,6 +223,11 @@ int  mmca_nucleation_sites(const mmca_params *params, const mmca_nucleation_cfg 
 int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg *cfg, int32_t n,
                             double *angles_out, double *rotmat_out);
 
+/* --- tuning (no effect on results) --------------------------------------- */
+#define MMCA_TUNE_ZM_PLANES  0   /* >0: z-marching solute kernel, planes per workgroup; 0: cell-parallel */
+int  mmca_set_tuning(mmca_ctx *ctx, int key, int value);
+int  mmca_get_tuning(mmca_ctx *ctx, int key, int *value);
+
 /* Average duration [ms] of the dominant kernel over the last mmca_step call,
  * measured with HIP events on the context's stream (0 when not timed). */
 int  mmca_set_kernel_timing(mmca_ctx *ctx, int enable);

@@ -47,6 +47,8 @@ FIELD_THRESHOLD = 7
 FIELD_MOLD = 8
 FIELD_OCE = 9
 
+TUNE_ZM_PLANES = 0
+
 
 class Params(C.Structure):
     _fields_ = [

@@ -137,7 +137,7 @@ int build_consts(const mmca_params &p, Consts &c)
     c.sx = p.nx + 2;
     c.sxy = (long long) (p.nx + 2) * (p.ny + 2);
     c.P = padded_cells(p);
-    c.w64 = (c.sx + 63) / 64;
+    c.w64 = (c.nx + 63) / 64;          // mask words per row (interior x)
     for (int a = 0; a < 3; a++) {
         c.pad[a] = p.pad_bndy[a] ? 1 : 0;
     }

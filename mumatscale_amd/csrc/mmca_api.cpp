@@ -74,7 +74,8 @@ struct mmca_ctx {
     double *d_red = nullptr;           // 2 doubles
     // kernel timing
     bool timing = false;
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> ev;       // pool, reused
+    size_t ev_used = 0;
     double time_ms = 0.0;
     long long time_n = 0;
 };
@@ -167,17 +168,19 @@ int enqueue_iteration(mmca_ctx *ctx, bool advance)
 {
     const int nxt = ctx->cur ^ 1;
     HIPCHK(launch_nucleate(ctx->b, ctx->c, advance ? 7 : 3, ctx->cur, ctx->stream));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const size_t ev0 = ctx->ev_used;
     if (ctx->timing) {
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, ctx->stream));
+        while (ctx->ev.size() < ev0 + 2) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            ctx->ev.push_back(e);
+        }
+        HIPCHK(hipEventRecord(ctx->ev[ev0], ctx->stream));
     }
     HIPCHK(launch_solute(ctx->b, ctx->c, 1, 1, ctx->cur, nxt, ctx->stream));
     if (ctx->timing) {
-        HIPCHK(hipEventRecord(e1, ctx->stream));
-        ctx->ev.push_back(e0);
-        ctx->ev.push_back(e1);
+        HIPCHK(hipEventRecord(ctx->ev[ev0 + 1], ctx->stream));
+        ctx->ev_used = ev0 + 2;
     }
     HIPCHK(launch_list(ctx->b, ctx->c, nxt, 1, 1, ctx->stream));
     ctx->cur = nxt;
@@ -188,15 +191,13 @@ int collect_timing(mmca_ctx *ctx)
 {
     double tot = 0.0;
     long long n = 0;
-    for (size_t i = 0; i + 1 < ctx->ev.size(); i += 2) {
+    for (size_t i = 0; i + 1 < ctx->ev_used; i += 2) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]));
         tot += ms;
         n++;
-        (void) hipEventDestroy(ctx->ev[i]);
-        (void) hipEventDestroy(ctx->ev[i + 1]);
     }
-    ctx->ev.clear();
+    ctx->ev_used = 0;
     ctx->time_ms = n ? tot / n : 0.0;
     ctx->time_n = n;
     return MMCA_OK;
@@ -328,7 +329,7 @@ int mmca_destroy(mmca_ctx *ctx)
     DevBufs &b = ctx->b;
     void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.fs[0], b.fs[1], b.gr, b.dcx, b.dcy, b.dcz, b.mask,
                      b.diff_id, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
-                     b.site_ls, b.events, b.g_rot, b.g_cell, b.g_step, b.g_src, b.fixed_rot, b.rand_rot,
+                     b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.fixed_rot, b.rand_rot,
                      b.oce, b.d, b.red_part, b.cnt, ctx->d_red };
     for (void *p : ptrs)
         if (p)
@@ -370,6 +371,9 @@ int mmca_init_nucleation(mmca_ctx *ctx, const mmca_nucleation_cfg *cfg)
     TRY(dev_alloc(&b.site_fixed, ns));
     TRY(dev_alloc(&b.site_ls, ns));
     TRY(dev_alloc(&b.events, ns));
+    const size_t nblk_nuc = (ns + 255) / 256;
+    TRY(dev_alloc(&b.ev_blk, nblk_nuc * 256));
+    TRY(dev_alloc(&b.ev_cnt, nblk_nuc));
     TRY(dev_alloc(&b.fixed_rot, fixed_rot.size()));
     if (ns) {
         HIPCHK(hipMemcpy(b.site_idx, ctx->site_idx.data(), ns * sizeof(int), hipMemcpyHostToDevice));
@@ -380,6 +384,9 @@ int mmca_init_nucleation(mmca_ctx *ctx, const mmca_nucleation_cfg *cfg)
     HIPCHK(hipMemcpy(b.fixed_rot, fixed_rot.data(), fixed_rot.size() * sizeof(double),
                      hipMemcpyHostToDevice));
     ctx->nucleation_ready = true;
+    // draw the first orientations now, outside any timed stepping
+    TRY(ensure_random(ctx, std::min<long long>(ctx->rand_capacity,
+                                               std::max<long long>(65536, 256LL * (long long) ns))));
     return MMCA_OK;
 }
 
@@ -402,10 +409,34 @@ int mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report)
         ctx->hc.failed_step = ctx->hc.timestep + 1;
         TRY(push_counters(ctx));
     } else {
-        long long need = (long long) ctx->hc.random_used + (long long) nsteps * ctx->b.nsites;
-        TRY(ensure_random(ctx, need));
-        for (int n = 0; n < nsteps; n++)
-            TRY(enqueue_iteration(ctx, true));
+        // Steps are issued in chunks small enough that the orientations
+        // already drawn cover every grain the chunk can create (at most one
+        // per site and step); the counters are read back between chunks.
+        int done = 0;
+        while (done < nsteps) {
+            long long k = nsteps - done;
+            if (ctx->b.nsites > 0 && ctx->rand_filled < ctx->rand_capacity) {
+                long long spare = (long long) ctx->rand_filled - ctx->hc.random_used;
+                if (spare < (long long) ctx->b.nsites * 16) {
+                    long long want = (long long) ctx->hc.random_used +
+                                     std::max((long long) ctx->b.nsites * 256, 2 * spare + 65536);
+                    TRY(ensure_random(ctx, want));
+                    spare = (long long) ctx->rand_filled - ctx->hc.random_used;
+                }
+                if (ctx->rand_filled < ctx->rand_capacity)
+                    k = std::min(k, spare / ctx->b.nsites);
+            }
+            if (k < 1)
+                return MMCA_ERR_STATE;
+            for (long long n = 0; n < k; n++)
+                TRY(enqueue_iteration(ctx, true));
+            done += (int) k;
+            if (done < nsteps) {
+                TRY(sync_counters(ctx));
+                if (ctx->hc.status != MMCA_OK)
+                    break;
+            }
+        }
         if (want_sums)
             HIPCHK(launch_reduce(ctx->b, ctx->c, ctx->cur, ctx->d_red, ctx->stream));
         TRY(sync_counters(ctx));
@@ -482,6 +513,9 @@ int mmca_sb_diffuse_alloy_decentered(mmca_ctx *ctx)
     if (st != MMCA_OK)
         return st;
     TRY(ensure_oce(ctx));
+    // oce = ce over the padded box (calculate.c:267-280)
+    HIPCHK(hipMemcpyAsync(ctx->b.oce, ctx->b.ce, (size_t) ctx->c.P * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(launch_solute(ctx->b, ctx->c, 1, 0, ctx->cur, ctx->cur, ctx->stream));
     ctx->have_oce = true;
     return check_status(ctx);
@@ -769,6 +803,34 @@ int mmca_padded_cells(mmca_ctx *ctx, int64_t *cells)
         return MMCA_ERR_ARG;
     *cells = ctx->c.P;
     return MMCA_OK;
+}
+
+int mmca_set_tuning(mmca_ctx *ctx, int key, int value)
+{
+    if (!ctx)
+        return MMCA_ERR_ARG;
+    switch (key) {
+    case MMCA_TUNE_ZM_PLANES:
+        if (value < 0 || value > ctx->c.nz * 2 + 1024)
+            return MMCA_ERR_ARG;
+        ctx->c.zm_planes = value;
+        return MMCA_OK;
+    default:
+        return MMCA_ERR_ARG;
+    }
+}
+
+int mmca_get_tuning(mmca_ctx *ctx, int key, int *value)
+{
+    if (!ctx || !value)
+        return MMCA_ERR_ARG;
+    switch (key) {
+    case MMCA_TUNE_ZM_PLANES:
+        *value = ctx->c.zm_planes;
+        return MMCA_OK;
+    default:
+        return MMCA_ERR_ARG;
+    }
 }
 
 int mmca_set_kernel_timing(mmca_ctx *ctx, int enable)

@@ -21,6 +21,7 @@ struct Consts {
     int pad[3];             // FaceCtrl per axis
     int off[3];             // subblock offset in global cells (sb_coords * n)
     int gdimz;              // global z extent (2-D runs use y as the axis)
+    int zm_planes;          // > 0: z-marching solute kernel with this many planes per workgroup
     // diffusion (growth.c:38-40)
     double rs, rl;
     // partitioning (growth.c:181-189)
@@ -72,6 +73,8 @@ struct DevBufs {
     int *site_fixed;                 // index into the fixed orientation table, -1 if random
     int *site_ls;                    // AM lsindex of the site cell
     int *events;                     // site numbers that nucleate this step, in order
+    int *ev_blk;                     // per-workgroup event lists (NUC_BS per workgroup)
+    int *ev_cnt;                     // per-workgroup event counts
     int nsites;
     // grains
     double *g_rot;                   // 9 per grain, rotmat row-major

@@ -1,9 +1,9 @@
 // HIP kernels of the CA step for gfx950 (CDNA4, wave64).
 //
-// One step (doiteration, calculate.c:193-352) on one subblock is five
+// One step (doiteration, calculate.c:193-352) on one subblock is six
 // launches on the context's stream:
-//   k_nucleate      cell_nucleation + activateNewGrains (grain.c:1222-1318,
-//                   841-916) over the sorted site list, one workgroup
+//   k_nuc_detect    cell_nucleation (grain.c:1222-1318) over the sorted site list
+//   k_nuc_activate  activateNewGrains (grain.c:841-916): ids in site order, one workgroup
 //   k_solute        diffusion + partitioning (growth.c:30-170, 177-276),
 //                   fused, and a 1-bit "solid" mask of gr > 0
 //   k_list_count    interface-list membership counts from the mask
@@ -139,12 +139,70 @@ block_exscan_int(int v, int *total, int *s_w)
 }
 
 // ---------------------------------------------------------------------------
-// cell_nucleation + activateNewGrains/createGrains on one rank.
-// mode: 1 = detect, 2 = activate, 4 = advance the timestep first.
+// cell_nucleation (grain.c:1222-1318): one thread per site, sites sorted by
+// padded index; each workgroup compacts its events in site order.
 constexpr int NUC_BS = 256;
 
 __global__ void __launch_bounds__(NUC_BS)
-k_nucleate(DevBufs b, Consts c, int mode, int cur)
+k_nuc_detect(DevBufs b, Consts c, int advance, int cur)
+{
+    __shared__ int s_w[NUC_BS / 64];
+    Counters *cnt = b.cnt;
+    if (cnt->status != MMCA_OK)
+        return;
+    const unsigned long long ts = cnt->timestep + (advance ? 1ull : 0ull);
+    const double sim_time = ts * c.dt;
+    const int s = blockIdx.x * NUC_BS + threadIdx.x;
+    int flag = 0;
+    if (s < b.nsites)
+    {
+        int idx = b.site_idx[s];
+        int z = (int) (idx / c.sxy);
+        int rem = (int) (idx - z * c.sxy);
+        int y = rem / c.sx;
+        int x = rem - y * c.sx;
+        double T = temp_at(c, sim_time, x, y, z);
+        int layerno = 1, initnuc = 1;
+        if (c.am)
+        {
+            // tempUpdateSB_int sets lsindex before nucleation (temperature.c:256-259)
+            if (T > c.Tliq)
+                b.site_ls[s] = 1;
+            if (c.am_layers == 1)
+                layerno = 1;
+            else if (c.am_layers > 1 && b.site_ls[s] == 1)
+                layerno = 1;
+            else
+                layerno = 0;
+            if (sim_time <= 1e-4)
+                initnuc = 1;
+            else if (T >= c.Tsol)
+                initnuc = 1;
+            else
+                initnuc = 0;
+        }
+        if (b.gr[idx] <= 0 && layerno && initnuc)
+        {
+            double Tunder = c.Tliq - T;
+            double m_solute0 = c.ma * T * T + c.mb * T + c.mc;
+            Tunder += m_solute0 * (b.cl[cur][idx] - c.cinit);
+            if (Tunder > (double) b.site_thr[s])
+                flag = 1;
+        }
+    }
+    int tot;
+    int r = block_exscan_flag<NUC_BS / 64>(flag, &tot, s_w);
+    if (flag)
+        b.ev_blk[blockIdx.x * NUC_BS + r] = s;
+    if (threadIdx.x == 0)
+        b.ev_cnt[blockIdx.x] = tot;
+}
+
+// Event ordering and activateNewGrains/createGrains on one rank
+// (grain.c:841-916, 348-425), one workgroup.
+// mode: 1 = order the detected events, 2 = activate, 4 = advance the timestep.
+__global__ void __launch_bounds__(NUC_BS)
+k_nuc_activate(DevBufs b, Consts c, int mode, int nblk)
 {
     __shared__ int s_w[NUC_BS / 64];
     __shared__ int s_nev;
@@ -152,153 +210,181 @@ k_nucleate(DevBufs b, Consts c, int mode, int cur)
     if (cnt->status != MMCA_OK)
         return;
     const int tid = threadIdx.x;
-    unsigned long long ts = cnt->timestep;
-    if (mode & 4)
-        ts += 1;
-    __syncthreads();
-    if ((mode & 4) && tid == 0)
-        cnt->timestep = ts;
-    const double sim_time = ts * c.dt;
-
+    const unsigned long long ts = cnt->timestep + ((mode & 4) ? 1ull : 0ull);
     if (mode & 1)
     {
-        int nev = 0;
-        for (int base = 0; base < b.nsites; base += NUC_BS)
+        // exclusive scan of the per-workgroup counts, then concatenate
+        int carry = 0;
+        for (int base = 0; base < nblk; base += NUC_BS)
         {
-            int s = base + tid;
-            int flag = 0;
-            if (s < b.nsites)
-            {
-                int idx = b.site_idx[s];
-                int z = (int) (idx / c.sxy);
-                int rem = (int) (idx - z * c.sxy);
-                int y = rem / c.sx;
-                int x = rem - y * c.sx;
-                double T = temp_at(c, sim_time, x, y, z);
-                int layerno = 1, initnuc = 1;
-                if (c.am)
-                {
-                    // tempUpdateSB_int sets lsindex before nucleation (temperature.c:256-259)
-                    if (T > c.Tliq)
-                        b.site_ls[s] = 1;
-                    if (c.am_layers == 1)
-                        layerno = 1;
-                    else if (c.am_layers > 1 && b.site_ls[s] == 1)
-                        layerno = 1;
-                    else
-                        layerno = 0;
-                    if (sim_time <= 1e-4)
-                        initnuc = 1;
-                    else if (T >= c.Tsol)
-                        initnuc = 1;
-                    else
-                        initnuc = 0;
-                }
-                if (b.gr[idx] <= 0 && layerno && initnuc)
-                {
-                    double Tunder = c.Tliq - T;
-                    double m_solute0 = c.ma * T * T + c.mb * T + c.mc;
-                    Tunder += m_solute0 * (b.cl[cur][idx] - c.cinit);
-                    if (Tunder > (double) b.site_thr[s])
-                        flag = 1;
-                }
-            }
+            int blk = base + tid;
+            int v = (blk < nblk) ? b.ev_cnt[blk] : 0;
             int tot;
-            int r = block_exscan_flag<NUC_BS / 64>(flag, &tot, s_w);
-            if (flag)
-                b.events[nev + r] = s;
-            nev += tot;
+            int off = carry + block_exscan_int<NUC_BS / 64>(v, &tot, s_w);
+            for (int i = 0; i < v; ++i)
+                b.events[off + i] = b.ev_blk[blk * NUC_BS + i];
+            carry += tot;
         }
         if (tid == 0)
         {
-            s_nev = nev;
-            cnt->n_events = nev;
+            s_nev = carry;
+            cnt->n_events = carry;
         }
-        __syncthreads();
     }
-    else
-    {
-        if (tid == 0)
-            s_nev = cnt->n_events;
-        __syncthreads();
-    }
-
-    if (!(mode & 2))
-        return;
-    const int nev = s_nev;
-    if (nev == 0)
-        return;
-    const int base_id = cnt->num_grains;
-    const int rnd0 = cnt->random_used;
-    const long long maxg = c.max_total_grains;
-    int rused = 0;
-    for (int base = 0; base < nev; base += NUC_BS)
-    {
-        int e = base + tid;
-        int isev = e < nev;
-        int s = isev ? b.events[e] : 0;
-        int fixed = isev ? b.site_fixed[s] : -1;
-        int israndom = isev && fixed < 0;
-        int rtot;
-        int rrank = block_exscan_flag<NUC_BS / 64>(israndom, &rtot, s_w);
-        if (isev)
-        {
-            long long id = (long long) base_id + e;
-            if (id <= maxg)
-            {
-                int idx = b.site_idx[s];
-                b.gr[idx] = (int) id;
-                const double *rot;
-                int src;
-                if (fixed >= 0)
-                {
-                    rot = b.fixed_rot + 9 * (long long) fixed;
-                    src = fixed;
-                }
-                else
-                {
-                    int k = rnd0 + rused + rrank;
-                    rot = b.rand_rot + 9 * (long long) k;
-                    src = -(k + 1);
-                }
-                for (int q = 0; q < 9; ++q)
-                    b.g_rot[9 * id + q] = rot[q];
-                b.g_cell[id] = idx;
-                b.g_step[id] = ts;
-                b.g_src[id] = src;
-            }
-        }
-        rused += rtot;
-    }
+    else if (tid == 0)
+        s_nev = cnt->n_events;
     __syncthreads();
-    if (tid == 0)
+    const int nev = s_nev;
+    if ((mode & 2) && nev > 0)
     {
-        long long total = (long long) base_id + nev;
-        if (total > maxg + 1)
+        const int base_id = cnt->num_grains;
+        const int rnd0 = cnt->random_used;
+        const long long maxg = c.max_total_grains;
+        int rused = 0;
+        for (int base = 0; base < nev; base += NUC_BS)
         {
-            // grain.c:868-888: with every slot taken nothing is activated;
-            // otherwise createGrains runs past MaxTotGrains and aborts
-            // (grain.c:361-366) after creating the grains that fit.
-            if ((long long) base_id != maxg + 1)
+            int e = base + tid;
+            int isev = e < nev;
+            int s = isev ? b.events[e] : 0;
+            int fixed = isev ? b.site_fixed[s] : -1;
+            int israndom = isev && fixed < 0;
+            int rtot;
+            int rrank = block_exscan_flag<NUC_BS / 64>(israndom, &rtot, s_w);
+            if (isev)
             {
-                cnt->status = MMCA_ERR_GRAIN_CAPACITY;
-                cnt->failed_step = ts;
+                long long id = (long long) base_id + e;
+                if (id <= maxg)
+                {
+                    int idx = b.site_idx[s];
+                    b.gr[idx] = (int) id;
+                    const double *rot;
+                    int src;
+                    if (fixed >= 0)
+                    {
+                        rot = b.fixed_rot + 9 * (long long) fixed;
+                        src = fixed;
+                    }
+                    else
+                    {
+                        int k = rnd0 + rused + rrank;
+                        rot = b.rand_rot + 9 * (long long) k;
+                        src = -(k + 1);
+                    }
+                    for (int q = 0; q < 9; ++q)
+                        b.g_rot[9 * id + q] = rot[q];
+                    b.g_cell[id] = idx;
+                    b.g_step[id] = ts;
+                    b.g_src[id] = src;
+                }
+            }
+            rused += rtot;
+        }
+        __syncthreads();
+        if (tid == 0)
+        {
+            long long total = (long long) base_id + nev;
+            if (total > maxg + 1)
+            {
+                // grain.c:868-888: with every slot taken nothing is activated;
+                // otherwise createGrains runs past MaxTotGrains and aborts
+                // (grain.c:361-366) after creating the grains that fit.
+                if ((long long) base_id != maxg + 1)
+                {
+                    cnt->status = MMCA_ERR_GRAIN_CAPACITY;
+                    cnt->failed_step = ts;
+                }
+            }
+            else
+            {
+                cnt->num_grains = (int) total;
+                cnt->random_used = rnd0 + rused;
+                cnt->new_grains += nev;
             }
         }
-        else
-        {
-            cnt->num_grains = (int) total;
-            cnt->random_used = rnd0 + rused;
-            cnt->new_grains += nev;
-        }
     }
+    if ((mode & 4) && tid == 0)
+        cnt->timestep = ts;
 }
 
 // ---------------------------------------------------------------------------
-// Diffusion (growth.c:85-113, order +x,-x,+y,-y,+z,-z) and partitioning
-// (growth.c:221-273).  DIFF && FSC: fused step reading buffers `cur`,
-// writing `nxt`.  DIFF only: ce += nbsum, oce = old ce.  FSC only: in place
-// on `cur` with ce_old = oce.  FSC writes the solid mask (gr > 0).
+// Diffusion (growth.c:85-113, order +x,-x,+y,-y,+z,-z) of one cell, from the
+// neighbours' cl and fs and their mold flags.
+__device__ __forceinline__ double
+diffuse_cell(const Consts &c, double ce_old, double conc, double lspfs, const double *cln,
+             const double *fsn, const int *mo)
+{
+    double nbsum = 0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n)
+    {
+        double fs_av = 0.5 * (lspfs + fsn[n]);
+        double r = (c.rs * fs_av + c.rl * (1 - fs_av)) * (1 - mo[n]);
+        nbsum += r * (cln[n] - conc);
+    }
+    return ce_old + nbsum;
+}
+
+// Partitioning of one cell (fs_change_diffuse, growth.c:221-273).  The
+// temperature is only needed for cells with gr > 0.
+__device__ __forceinline__ void
+partition_cell(const Consts &c, double sim_time, int x, int j, int k, double ce_new,
+               double ce_prev, double conc, double lspfs, int g_old, double *cl_out,
+               double *fs_out, int *g_out)
+{
+    int g = g_old;
+    if (g <= 0)
+    {
+        *cl_out = ce_new;
+        *g_out = 0;
+        *fs_out = 0;
+        return;
+    }
+    const double Tcell = temp_at(c, sim_time, x, j, k);
+    double c_int = c.cinit + c.m_inv * (Tcell - c.Tliq);
+    double Tunder = c.Tliq - Tcell;
+    Tunder += c.liq_slope * (conc - c.cinit);
+    if (Tcell < 1.0 || Tcell > c.Tliq)
+    {
+        *cl_out = ce_new;
+        *g_out = 0;
+        *fs_out = 0;
+        return;
+    }
+    if (lspfs == 1.0 && (Tunder >= 0.0))
+    {
+        *cl_out = ce_new * c.k_inv;
+        *fs_out = lspfs;
+        *g_out = g;
+        return;
+    }
+    double dce = ce_new - ce_prev;
+    double dcl = (c_int - conc) * c.D_xt * 2 + dce / (1 - c.km * lspfs);
+    double dfs = (dcl * (1. - c.km * lspfs) - dce) * c.km_inv / conc;
+    double cl = conc + dcl;
+    double fs = lspfs + dfs;
+    fs = ref_min(1.0, fs);
+    if (fs < 0.)
+    {
+        fs = 0.;
+        g = 0;
+        cl = ce_new;
+    }
+    *cl_out = cl;
+    *fs_out = fs;
+    *g_out = g;
+}
+
+__device__ __forceinline__ void
+flag_negative_ce(Counters *cnt)
+{
+    if (atomicCAS(&cnt->status, MMCA_OK, MMCA_ERR_NEGATIVE_CE) == MMCA_OK)
+        cnt->failed_step = cnt->timestep;
+}
+
+// Cell-parallel kernel.  DIFF && FSC: fused step reading buffers `cur`,
+// writing `nxt`.  DIFF only: ce += nbsum (the caller copies ce to oce
+// beforehand).  FSC only: in place on `cur` with ce_old = oce.  FSC writes
+// the solid mask (gr > 0).
 constexpr int SOL_BX = 64;
 constexpr int SOL_BY = 4;
 
@@ -309,12 +395,12 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
         return;
-    const int x = blockIdx.x * SOL_BX + threadIdx.x;   // padded x
+    const int x = blockIdx.x * SOL_BX + threadIdx.x + 1;   // interior x
     const int j = blockIdx.y * SOL_BY + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (j > c.ny)
         return;                                          // whole wave
-    const bool active = (x >= 1 && x <= c.nx);
+    const bool active = (x <= c.nx);
     int g_new = 0;
     if (active)
     {
@@ -332,69 +418,26 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
             const int mo[6] = { (x == c.nx) ? c.pad[0] : 0, (x == 1) ? c.pad[0] : 0,
                                 (j == c.ny) ? c.pad[1] : 0, (j == 1) ? c.pad[1] : 0,
                                 (k == c.nz) ? c.pad[2] : 0, (k == 1) ? c.pad[2] : 0 };
-            double nbsum = 0;
+            double cln[6], fsn[6];
 #pragma unroll
             for (int n = 0; n < 6; ++n)
             {
-                double fs_av = 0.5 * (lspfs + fsc[nb[n]]);
-                double r = (c.rs * fs_av + c.rl * (1 - fs_av)) * (1 - mo[n]);
-                nbsum += r * (clc[nb[n]] - conc);
+                cln[n] = clc[nb[n]];
+                fsn[n] = fsc[nb[n]];
             }
-            ce_new = ce_old + nbsum;
+            ce_new = diffuse_cell(c, ce_old, conc, lspfs, cln, fsn, mo);
             if (c.check_neg && ce_new < 0)
-            {
-                if (atomicCAS(&cnt->status, MMCA_OK, MMCA_ERR_NEGATIVE_CE) == MMCA_OK)
-                    cnt->failed_step = cnt->timestep;
-            }
+                flag_negative_ce(cnt);
             b.ce[idx] = ce_new;
-            if (!FSC)
-                b.oce[idx] = ce_old;
         }
         if (FSC)
         {
             const double ce_prev = DIFF ? ce_old : b.oce[idx];
             const int g_old = b.gr[idx];
-            int g = g_old;
             double cl_out, fs_out;
-            if (g <= 0)
-            {
-                cl_out = ce_new;
-                g = 0;
-                fs_out = 0;
-            }
-            else
-            {
-                const double Tcell = temp_at(c, cnt->timestep * c.dt, x, j, k);
-                double c_int = c.cinit + c.m_inv * (Tcell - c.Tliq);
-                double Tunder = c.Tliq - Tcell;
-                Tunder += c.liq_slope * (conc - c.cinit);
-                if (Tcell < 1.0 || Tcell > c.Tliq)
-                {
-                    cl_out = ce_new;
-                    g = 0;
-                    fs_out = 0;
-                }
-                else if (lspfs == 1.0 && (Tunder >= 0.0))
-                {
-                    cl_out = ce_new * c.k_inv;
-                    fs_out = lspfs;
-                }
-                else
-                {
-                    double dce = ce_new - ce_prev;
-                    double dcl = (c_int - conc) * c.D_xt * 2 + dce / (1 - c.km * lspfs);
-                    double dfs = (dcl * (1. - c.km * lspfs) - dce) * c.km_inv / conc;
-                    cl_out = conc + dcl;
-                    fs_out = lspfs + dfs;
-                    fs_out = ref_min(1.0, fs_out);
-                    if (fs_out < 0.)
-                    {
-                        fs_out = 0.;
-                        g = 0;
-                        cl_out = ce_new;
-                    }
-                }
-            }
+            int g;
+            partition_cell(c, cnt->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
+                           &cl_out, &fs_out, &g);
             b.cl[nxt][idx] = cl_out;
             b.fs[nxt][idx] = fs_out;
             if (g != g_old)
@@ -410,15 +453,143 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
     }
 }
 
+// z-marching fused kernel: a workgroup of 64 x BY threads owns a 64 x BY
+// column tile and walks `kz` planes upward.  cl/fs of the planes k-1, k, k+1
+// stay in registers, x neighbours come from the neighbouring lanes
+// (wave shuffles), y neighbours from the rows of the other waves in LDS;
+// only tile edges load neighbours from memory.
+template <int BY>
+__global__ void __launch_bounds__(64 * BY)
+k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
+{
+    __shared__ double s_cl[BY][64];
+    __shared__ double s_fs[BY][64];
+    __shared__ int s_stop;
+    Counters *cnt = b.cnt;
+    // this kernel itself may raise the status: read it once per workgroup so
+    // that every wave takes the same branch before the barriers below
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        s_stop = (cnt->status != MMCA_OK);
+    __syncthreads();
+    if (s_stop)
+        return;
+    const int lane = threadIdx.x;
+    const int ty = threadIdx.y;
+    const int x = blockIdx.x * 64 + lane + 1;
+    const int j = blockIdx.y * BY + ty + 1;
+    const int k0 = blockIdx.z * kz + 1;
+    const int k1 = min(c.nz, k0 + kz - 1);
+    const bool rowok = (j <= c.ny);
+    const bool active = rowok && (x <= c.nx);
+    const double *clc = b.cl[cur];
+    const double *fsc = b.fs[cur];
+    double *cln_out = b.cl[nxt];
+    double *fsn_out = b.fs[nxt];
+    const long long col = active ? ((long long) j * c.sx + x) : 0;
+    const double sim_time = cnt->timestep * c.dt;
+    const bool ylo_lds = (ty > 0);
+    const bool yhi_lds = (ty + 1 < BY) && (j + 1 <= c.ny);
+    const bool xlo_mem = (lane == 0);
+    const bool xhi_mem = (lane == 63) || (x == c.nx);
+    const int mo_x1 = (x == c.nx) ? c.pad[0] : 0, mo_x0 = (x == 1) ? c.pad[0] : 0;
+    const int mo_y1 = (j == c.ny) ? c.pad[1] : 0, mo_y0 = (j == 1) ? c.pad[1] : 0;
+    double clm = 0, clv = 0, clp = 0, fsm = 0, fsv = 0, fsp = 0;
+    if (active)
+    {
+        clm = clc[(k0 - 1) * c.sxy + col];
+        fsm = fsc[(k0 - 1) * c.sxy + col];
+        clv = clc[k0 * c.sxy + col];
+        fsv = fsc[k0 * c.sxy + col];
+    }
+    for (int k = k0; k <= k1; ++k)
+    {
+        const long long idx = k * c.sxy + col;
+        double ce_old = 0;
+        int g_old = 0;
+        if (active)
+        {
+            clp = clc[idx + c.sxy];
+            fsp = fsc[idx + c.sxy];
+            ce_old = b.ce[idx];
+            g_old = b.gr[idx];
+        }
+        s_cl[ty][lane] = clv;
+        s_fs[ty][lane] = fsv;
+        double clxm = __shfl_up(clv, 1, 64), fsxm = __shfl_up(fsv, 1, 64);
+        double clxp = __shfl_down(clv, 1, 64), fsxp = __shfl_down(fsv, 1, 64);
+        __syncthreads();
+        int g_new = 0;
+        if (active)
+        {
+            if (xlo_mem)
+            {
+                clxm = clc[idx - 1];
+                fsxm = fsc[idx - 1];
+            }
+            if (xhi_mem)
+            {
+                clxp = clc[idx + 1];
+                fsxp = fsc[idx + 1];
+            }
+            double clym, fsym, clyp, fsyp;
+            if (ylo_lds)
+            {
+                clym = s_cl[ty - 1][lane];
+                fsym = s_fs[ty - 1][lane];
+            }
+            else
+            {
+                clym = clc[idx - c.sx];
+                fsym = fsc[idx - c.sx];
+            }
+            if (yhi_lds)
+            {
+                clyp = s_cl[ty + 1][lane];
+                fsyp = s_fs[ty + 1][lane];
+            }
+            else
+            {
+                clyp = clc[idx + c.sx];
+                fsyp = fsc[idx + c.sx];
+            }
+            const double cln[6] = { clxp, clxm, clyp, clym, clp, clm };
+            const double fsn[6] = { fsxp, fsxm, fsyp, fsym, fsp, fsm };
+            const int mo[6] = { mo_x1, mo_x0, mo_y1, mo_y0, (k == c.nz) ? c.pad[2] : 0,
+                                (k == 1) ? c.pad[2] : 0 };
+            const double ce_new = diffuse_cell(c, ce_old, clv, fsv, cln, fsn, mo);
+            if (c.check_neg && ce_new < 0)
+                flag_negative_ce(cnt);
+            b.ce[idx] = ce_new;
+            double cl_out, fs_out;
+            partition_cell(c, sim_time, x, j, k, ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out,
+                           &g_new);
+            cln_out[idx] = cl_out;
+            fsn_out[idx] = fs_out;
+            if (g_new != g_old)
+                b.gr[idx] = g_new;
+        }
+        unsigned long long bits = __ballot(active && g_new > 0);
+        if (rowok && lane == 0)
+            b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + blockIdx.x] = bits;
+        __syncthreads();
+        clm = clv;
+        clv = clp;
+        fsm = fsv;
+        fsv = fsp;
+    }
+}
+
 // Solid mask from gr (used when the list is built outside the fused step).
+// Mask layout: per padded row (j,k), w64 words; bit b of word w is interior
+// x = 64 w + b + 1.
 __global__ void
 k_mask_from_gr(DevBufs b, Consts c)
 {
-    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int x = blockIdx.x * 64 + threadIdx.x + 1;
     const int j = blockIdx.y;
     const int k = blockIdx.z;
     bool solid = false;
-    if (x < c.sx)
+    if (x <= c.nx)
         solid = b.gr[k * c.sxy + (long long) j * c.sx + x] > 0;
     unsigned long long bits = __ballot(solid);
     if (threadIdx.x == 0)
@@ -455,14 +626,9 @@ member_bits(const DevBufs &b, const Consts &c, long long wi, int *pk, int *pj, i
         right |= mask_word(b, c, k, j, w + 1) << 63;
     unsigned long long yz = mask_word(b, c, k, j - 1, w) | mask_word(b, c, k, j + 1, w) |
                             mask_word(b, c, k - 1, j, w) | mask_word(b, c, k + 1, j, w);
-    // interior x in [1, nx]
-    const int x0 = w * 64;
-    unsigned long long inter = ~0ull;
-    if (x0 == 0)
-        inter &= ~1ull;
-    const int hi = c.nx + 1 - x0;                 // first non-interior bit at or above
-    if (hi < 64)
-        inter &= (hi <= 0) ? 0ull : ((1ull << hi) - 1ull);
+    // bit b of word w is interior x = 64 w + b + 1 <= nx
+    const int hi = c.nx - w * 64;
+    const unsigned long long inter = (hi >= 64) ? ~0ull : ((1ull << hi) - 1ull);
     return ~own & (left | right | yz) & inter;
 }
 
@@ -538,33 +704,63 @@ k_realloc_decide(DevBufs b)
         cnt->realloc_old = -1;
 }
 
+// gr > 0 before capture; the padded faces are never solid on one subblock
 __device__ __forceinline__ bool
 solid_at(const DevBufs &b, const Consts &c, int x, int j, int k)
 {
-    unsigned long long wd = mask_word(b, c, k, j, x >> 6);
-    return (wd >> (x & 63)) & 1ull;
+    if (x < 1 || x > c.nx)
+        return false;
+    unsigned long long wd = mask_word(b, c, k, j, (x - 1) >> 6);
+    return (wd >> ((x - 1) & 63)) & 1ull;
 }
 
 // capture_octahedra_diffuse, INDEX_SEP branch (growth.c:640-807).
 // do_capture = 0 only writes the list (grow_cell_reduction).  `cb` is the
-// buffer holding fs/cl after fs_change.
+// buffer holding fs/cl after fs_change.  The members of a workgroup's 256
+// mask words are spread evenly over its threads: member i of the workgroup
+// is found by a binary search over the per-word prefix counts in LDS.
+__device__ __forceinline__ int
+nth_set_bit(unsigned long long m, int r)
+{
+    int pos = 0;
+#pragma unroll
+    for (int half = 32; half > 0; half >>= 1)
+    {
+        const unsigned long long lo = m & ((1ull << half) - 1ull);
+        const int c = __popcll(lo);
+        if (r >= c)
+        {
+            r -= c;
+            m >>= half;
+            pos += half;
+        }
+        else
+            m = lo;
+    }
+    return pos;
+}
+
 __global__ void __launch_bounds__(LIST_BS)
 k_capture(DevBufs b, Consts c, long long nwords, int cb, int do_capture)
 {
     __shared__ int s_w[LIST_BS / 64];
+    __shared__ unsigned long long s_mem[LIST_BS];
+    __shared__ int s_ex[LIST_BS];
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
         return;
-    const long long wi = (long long) blockIdx.x * LIST_BS + threadIdx.x;
+    const long long wi0 = (long long) blockIdx.x * LIST_BS;
+    const long long wi = wi0 + threadIdx.x;
     unsigned long long mem = 0;
-    int k = 0, j = 0, w = 0;
+    int k0, j0, w0;
     if (wi < nwords)
-        mem = member_bits(b, c, wi, &k, &j, &w);
+        mem = member_bits(b, c, wi, &k0, &j0, &w0);
     int tot;
     int ex = block_exscan_int<LIST_BS / 64>(__popcll(mem), &tot, s_w);
-    if (!mem)
-        return;
-    int slot = b.blk_count[blockIdx.x] + ex;      // 0-based; reference slots start at 1
+    s_mem[threadIdx.x] = mem;
+    s_ex[threadIdx.x] = ex;
+    __syncthreads();
+    const int base = b.blk_count[blockIdx.x];     // members before this workgroup
     const bool realloc = do_capture && cnt->realloc_old >= 0;
     const long long maxg = c.max_total_grains;
     double *fsb = b.fs[cb];
@@ -572,12 +768,26 @@ k_capture(DevBufs b, Consts c, long long nwords, int cb, int do_capture)
     const int nnx[6] = { 1, 0, -1, 0, 0, 0 };
     const int nny[6] = { 0, 1, 0, -1, 0, 0 };
     const int nnz[6] = { 0, 0, 0, 0, 1, -1 };
-    while (mem)
+    for (int i = threadIdx.x; i < tot; i += LIST_BS)
     {
-        const int bit = __ffsll((long long) mem) - 1;
-        mem &= mem - 1ull;
-        const int x = w * 64 + bit;
-        ++slot;
+        // word of member i: the last word whose prefix count is <= i
+        int lo = 0, hi = LIST_BS - 1;
+        while (lo < hi)
+        {
+            int mid = (lo + hi + 1) >> 1;
+            if (s_ex[mid] <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int bit = nth_set_bit(s_mem[lo], i - s_ex[lo]);
+        const long long wq = wi0 + lo;
+        const long long nrow = wq / c.w64;
+        const int w = (int) (wq - nrow * c.w64);
+        const int j = (int) (nrow % c.ny) + 1;
+        const int k = (int) (nrow / c.ny) + 1;
+        const int x = w * 64 + bit + 1;
+        const int slot = base + i + 1;             // reference slots start at 1
         const long long idx = k * c.sxy + (long long) j * c.sx + x;
         b.diff_id[slot] = (int) idx;
         if (!do_capture)
@@ -795,12 +1005,15 @@ k_grow_octahedra(DevBufs b, Consts c, int cb)
 // Launch wrappers (host side).
 static inline dim3 sol_grid(const Consts &c)
 {
-    return dim3((c.sx + SOL_BX - 1) / SOL_BX, (c.ny + SOL_BY - 1) / SOL_BY, c.nz);
+    return dim3((c.nx + SOL_BX - 1) / SOL_BX, (c.ny + SOL_BY - 1) / SOL_BY, c.nz);
 }
 
 hipError_t launch_nucleate(const DevBufs &b, const Consts &c, int mode, int cur, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_nucleate, dim3(1), dim3(NUC_BS), 0, s, b, c, mode, cur);
+    const int nblk = (b.nsites + NUC_BS - 1) / NUC_BS;
+    if ((mode & 1) && nblk > 0)
+        hipLaunchKernelGGL(k_nuc_detect, dim3(nblk), dim3(NUC_BS), 0, s, b, c, (mode & 4) ? 1 : 0, cur);
+    hipLaunchKernelGGL(k_nuc_activate, dim3(1), dim3(NUC_BS), 0, s, b, c, mode, nblk);
     return hipGetLastError();
 }
 
@@ -808,7 +1021,12 @@ hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, i
                          hipStream_t s)
 {
     dim3 blk(SOL_BX, SOL_BY);
-    if (diff && fsc)
+    if (diff && fsc && c.zm_planes > 0)
+    {
+        dim3 g((c.nx + 63) / 64, (c.ny + 3) / 4, (c.nz + c.zm_planes - 1) / c.zm_planes);
+        hipLaunchKernelGGL((k_solute_zm<4>), g, dim3(64, 4), 0, s, b, c, cur, nxt, c.zm_planes);
+    }
+    else if (diff && fsc)
         hipLaunchKernelGGL((k_solute<true, true>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
     else if (diff)
         hipLaunchKernelGGL((k_solute<true, false>), sol_grid(c), blk, 0, s, b, c, cur, nxt);

@@ -53,6 +53,8 @@ def lib():
     L.mmca_padded_cells.argtypes = [vp, C.POINTER(C.c_int64)]
     L.mmca_set_kernel_timing.argtypes = [vp, C.c_int]
     L.mmca_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.mmca_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.mmca_get_tuning.argtypes = [vp, C.c_int, i32p]
     L.mmca_nucleation_sites.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg),
                                         i32p, C.POINTER(C.c_float), i32p, C.c_int64,
                                         C.POINTER(C.c_int64)]
@@ -243,6 +245,14 @@ class Subblock:
         n = C.c_int64(0)
         _check(lib().mmca_padded_cells(self._h, C.byref(n)), "mmca_padded_cells")
         return n.value
+
+    def set_tuning(self, key, value):
+        _check(lib().mmca_set_tuning(self._h, key, value), "mmca_set_tuning")
+
+    def get_tuning(self, key):
+        v = C.c_int32(0)
+        _check(lib().mmca_get_tuning(self._h, key, C.byref(v)), "mmca_get_tuning")
+        return v.value
 
     def set_kernel_timing(self, enable=True):
         _check(lib().mmca_set_kernel_timing(self._h, 1 if enable else 0), "mmca_set_kernel_timing")

@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--size", type=str, default="256x256x512")
     ap.add_argument("--zm", type=int, default=16,
                     help="z-marching solute kernel, planes per workgroup (0: cell-parallel)")
+    ap.add_argument("--zm-rows", type=int, default=8, help="rows per z-marching workgroup (4, 8, 16)")
+    ap.add_argument("--xcd", type=int, default=1, help="1: contiguous tile runs per XCD")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
@@ -111,6 +113,8 @@ def main():
     t0 = time.time()
     gpu = Subblock(cfg.params(check_negative_ce=False), cfg.nucleation(), device=0)
     gpu.set_tuning(abi.TUNE_ZM_PLANES, args.zm)
+    gpu.set_tuning(abi.TUNE_ZM_ROWS, args.zm_rows)
+    gpu.set_tuning(abi.TUNE_XCD_REMAP, args.xcd)
     log("gpu set-up %.1f s" % (time.time() - t0))
     n = nx * ny * nz
     if args.warmup:
@@ -130,7 +134,7 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": None,
-            "kernel": "k_solute_zm<4>" if args.zm > 0 else "k_solute<true,true>",
+            "kernel": ("k_solute_zm<%d>" % args.zm_rows) if args.zm > 0 else "k_solute<true,true>",
             "bytes_per_cell": B_ALG_FP64}
     gpu.close()
     cpu = None
@@ -151,7 +155,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "directional solidification %dx%dx%d, plain FP64" % (nx, ny, nz),
-                   "cells": n, "dt": cfg.dt, "zm_planes": args.zm, "parallelism": "single GPU"},
+                   "cells": n, "dt": cfg.dt, "zm_planes": args.zm, "zm_rows": args.zm_rows,
+                   "xcd_remap": args.xcd, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

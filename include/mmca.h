@@ -225,6 +225,8 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
 
 /* --- tuning (no effect on results) --------------------------------------- */
 #define MMCA_TUNE_ZM_PLANES  0   /* >0: z-marching solute kernel, planes per workgroup; 0: cell-parallel */
+#define MMCA_TUNE_ZM_ROWS    1   /* rows per z-marching workgroup: 4, 8 or 16 */
+#define MMCA_TUNE_XCD_REMAP  2   /* 1: workgroups of one XCD take contiguous tiles */
 int  mmca_set_tuning(mmca_ctx *ctx, int key, int value);
 int  mmca_get_tuning(mmca_ctx *ctx, int key, int *value);
 

@@ -48,6 +48,8 @@ FIELD_MOLD = 8
 FIELD_OCE = 9
 
 TUNE_ZM_PLANES = 0
+TUNE_ZM_ROWS = 1
+TUNE_XCD_REMAP = 2
 
 
 class Params(C.Structure):

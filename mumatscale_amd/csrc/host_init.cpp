@@ -138,6 +138,9 @@ int build_consts(const mmca_params &p, Consts &c)
     c.sxy = (long long) (p.nx + 2) * (p.ny + 2);
     c.P = padded_cells(p);
     c.w64 = (c.nx + 63) / 64;          // mask words per row (interior x)
+    c.zm_planes = 16;                   // z-marching solute kernel by default
+    c.zm_by = 8;
+    c.xcd_remap = 1;
     for (int a = 0; a < 3; a++) {
         c.pad[a] = p.pad_bndy[a] ? 1 : 0;
     }

@@ -815,6 +815,14 @@ int mmca_set_tuning(mmca_ctx *ctx, int key, int value)
             return MMCA_ERR_ARG;
         ctx->c.zm_planes = value;
         return MMCA_OK;
+    case MMCA_TUNE_ZM_ROWS:
+        if (value != 4 && value != 8 && value != 16)
+            return MMCA_ERR_ARG;
+        ctx->c.zm_by = value;
+        return MMCA_OK;
+    case MMCA_TUNE_XCD_REMAP:
+        ctx->c.xcd_remap = value ? 1 : 0;
+        return MMCA_OK;
     default:
         return MMCA_ERR_ARG;
     }
@@ -827,6 +835,12 @@ int mmca_get_tuning(mmca_ctx *ctx, int key, int *value)
     switch (key) {
     case MMCA_TUNE_ZM_PLANES:
         *value = ctx->c.zm_planes;
+        return MMCA_OK;
+    case MMCA_TUNE_ZM_ROWS:
+        *value = ctx->c.zm_by;
+        return MMCA_OK;
+    case MMCA_TUNE_XCD_REMAP:
+        *value = ctx->c.xcd_remap;
         return MMCA_OK;
     default:
         return MMCA_ERR_ARG;

@@ -22,6 +22,8 @@ struct Consts {
     int off[3];             // subblock offset in global cells (sb_coords * n)
     int gdimz;              // global z extent (2-D runs use y as the axis)
     int zm_planes;          // > 0: z-marching solute kernel with this many planes per workgroup
+    int zm_by;              // rows per z-marching workgroup (4, 8, 16)
+    int xcd_remap;          // 1: contiguous tile runs per XCD
     // diffusion (growth.c:38-40)
     double rs, rl;
     // partitioning (growth.c:181-189)

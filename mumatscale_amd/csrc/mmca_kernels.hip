@@ -475,9 +475,26 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
         return;
     const int lane = threadIdx.x;
     const int ty = threadIdx.y;
-    const int x = blockIdx.x * 64 + lane + 1;
-    const int j = blockIdx.y * BY + ty + 1;
-    const int k0 = blockIdx.z * kz + 1;
+    // Tile of this workgroup.  With xcd_remap the workgroups that the
+    // hardware hands to one XCD (linear id mod 8) take a contiguous run of
+    // tiles (x fastest, then y, then z), so that tiles sharing halo rows and
+    // planes run on the same L2.
+    const int ntx = (c.nx + 63) / 64, nty = (c.ny + BY - 1) / BY, ntz = (c.nz + kz - 1) / kz;
+    const long long ntiles = (long long) ntx * nty * ntz;
+    long long tile = blockIdx.x;
+    if (c.xcd_remap)
+    {
+        const long long per = (ntiles + 7) / 8;
+        tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    }
+    if (tile >= ntiles)
+        return;                                   // uniform over the workgroup
+    const int bx = (int) (tile % ntx);
+    const int by = (int) ((tile / ntx) % nty);
+    const int bz = (int) (tile / ((long long) ntx * nty));
+    const int x = bx * 64 + lane + 1;
+    const int j = by * BY + ty + 1;
+    const int k0 = bz * kz + 1;
     const int k1 = min(c.nz, k0 + kz - 1);
     const bool rowok = (j <= c.ny);
     const bool active = rowok && (x <= c.nx);
@@ -570,7 +587,7 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
         }
         unsigned long long bits = __ballot(active && g_new > 0);
         if (rowok && lane == 0)
-            b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + blockIdx.x] = bits;
+            b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
         __syncthreads();
         clm = clv;
         clv = clp;
@@ -1023,8 +1040,19 @@ hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, i
     dim3 blk(SOL_BX, SOL_BY);
     if (diff && fsc && c.zm_planes > 0)
     {
-        dim3 g((c.nx + 63) / 64, (c.ny + 3) / 4, (c.nz + c.zm_planes - 1) / c.zm_planes);
-        hipLaunchKernelGGL((k_solute_zm<4>), g, dim3(64, 4), 0, s, b, c, cur, nxt, c.zm_planes);
+        const int by = (c.zm_by == 8 || c.zm_by == 16) ? c.zm_by : 4;
+        const long long nt = (long long) ((c.nx + 63) / 64) * ((c.ny + by - 1) / by) *
+                             ((c.nz + c.zm_planes - 1) / c.zm_planes);
+        const long long ng = c.xcd_remap ? ((nt + 7) / 8) * 8 : nt;
+        if (by == 16)
+            hipLaunchKernelGGL((k_solute_zm<16>), dim3((unsigned) ng), dim3(64, 16), 0, s, b, c, cur, nxt,
+                               c.zm_planes);
+        else if (by == 8)
+            hipLaunchKernelGGL((k_solute_zm<8>), dim3((unsigned) ng), dim3(64, 8), 0, s, b, c, cur, nxt,
+                               c.zm_planes);
+        else
+            hipLaunchKernelGGL((k_solute_zm<4>), dim3((unsigned) ng), dim3(64, 4), 0, s, b, c, cur, nxt,
+                               c.zm_planes);
     }
     else if (diff && fsc)
         hipLaunchKernelGGL((k_solute<true, true>), sol_grid(c), blk, 0, s, b, c, cur, nxt);

@@ -71,16 +71,23 @@ def checkpoints(last):
 BIT_EXACT_CASES = [c for c in CASES if c != "c3_am_beam"]
 
 
-@pytest.mark.parametrize("zm", [0, 8])
+# solute kernel variants (results must not depend on them): planes per
+# z-marching workgroup (0 = cell-parallel kernel), rows per workgroup, XCD remap
+VARIANTS = [(0, 4, 0), (8, 8, 1), (5, 16, 0), (16, 4, 1)]
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=["cell", "zm8x8", "zm5x16", "zm16x4"])
 @pytest.mark.parametrize("case", BIT_EXACT_CASES)
-def test_gpu_matches_oracle(case, zm):
-    """zm: 0 = cell-parallel solute kernel, 8 = z-marching kernel, 8 planes."""
+def test_gpu_matches_oracle(case, variant):
     cfg, hashes, expected, _ = load_case(case)
     last = expected["last_step"]
-    print("case", case, "steps", last, "zm", zm, flush=True)
+    zm, rows, xcd = variant
+    print("case", case, "steps", last, "variant", variant, flush=True)
     orc = OracleSubblock(cfg.params(check_negative_ce=True), cfg.nucleation())
     with Subblock(cfg.params(check_negative_ce=True), cfg.nucleation()) as gpu:
         gpu.set_tuning(abi.TUNE_ZM_PLANES, zm)
+        gpu.set_tuning(abi.TUNE_ZM_ROWS, rows)
+        gpu.set_tuning(abi.TUNE_XCD_REMAP, xcd)
         assert_bits_equal(gpu.download(abi.FIELD_THRESHOLD), orc.field(abi.FIELD_THRESHOLD), "thresholds")
         done = 0
         for t in checkpoints(last):
@@ -192,7 +199,6 @@ def test_gpu_full_size_directional_block():
     print("full size: oracle set-up", flush=True)
     orc = OracleSubblock(cfg.params(check_negative_ce=False), cfg.nucleation())
     with Subblock(cfg.params(check_negative_ce=False), cfg.nucleation()) as gpu:
-        gpu.set_tuning(abi.TUNE_ZM_PLANES, 16)
         s0 = orc.solute_sum()
         for t in (1, 2, 3):
             orc.step(1)

@@ -26,16 +26,19 @@ namespace mmca {
 // MIN(x,y) of globals.h:19
 __device__ __forceinline__ double ref_min(double x, double y) { return (x < y) ? x : y; }
 
-// direct_temp_calc (temperature.c:24-126); x,y,z are padded indices
-__device__ double
-temp_at(const Consts &c, double sim_time, int x, int y, int z)
+// direct_temp_calc (temperature.c:24-126); x,y,z are padded indices.  AM
+// selects the Gaussian-beam model at compile time, so that kernels of the
+// other models do not carry its registers.
+template <bool AM>
+__device__ __forceinline__ double
+temp_model(const Consts &c, double sim_time, int x, int y, int z)
 {
-    if (!c.am && c.gradient == 0)
+    if (!AM && c.gradient == 0)
         return c.t_init - sim_time * c.cool_rate;
     double rx = c.origin[0] + ((c.off[0] + x) + 0.5) * c.cell_size;
     double ry = c.origin[1] + ((c.off[1] + y) + 0.5) * c.cell_size;
     double rz = c.origin[2] + ((c.off[2] + z) + 0.5) * c.cell_size;
-    if (!c.am)
+    if (!AM)
     {
         double axis = ((c.gdimz == 1) ? ry : rz) - c.cell_size * 0.5;
         double tgrad = c.gradient + c.grad_coef * sim_time;
@@ -81,6 +84,12 @@ temp_at(const Consts &c, double sim_time, int x, int y, int z)
     }
     double Tfront = c.lab * lpow * lalpha / (pi15 * c.lthcon);
     return c.liniT + Tfront * Tinteg;
+}
+
+__device__ double
+temp_at(const Consts &c, double sim_time, int x, int y, int z)
+{
+    return c.am ? temp_model<true>(c, sim_time, x, y, z) : temp_model<false>(c, sim_time, x, y, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -326,6 +335,7 @@ diffuse_cell(const Consts &c, double ce_old, double conc, double lspfs, const do
 
 // Partitioning of one cell (fs_change_diffuse, growth.c:221-273).  The
 // temperature is only needed for cells with gr > 0.
+template <bool AM>
 __device__ __forceinline__ void
 partition_cell(const Consts &c, double sim_time, int x, int j, int k, double ce_new,
                double ce_prev, double conc, double lspfs, int g_old, double *cl_out,
@@ -339,7 +349,7 @@ partition_cell(const Consts &c, double sim_time, int x, int j, int k, double ce_
         *fs_out = 0;
         return;
     }
-    const double Tcell = temp_at(c, sim_time, x, j, k);
+    const double Tcell = temp_model<AM>(c, sim_time, x, j, k);
     double c_int = c.cinit + c.m_inv * (Tcell - c.Tliq);
     double Tunder = c.Tliq - Tcell;
     Tunder += c.liq_slope * (conc - c.cinit);
@@ -388,7 +398,7 @@ flag_negative_ce(Counters *cnt)
 constexpr int SOL_BX = 64;
 constexpr int SOL_BY = 4;
 
-template <bool DIFF, bool FSC>
+template <bool DIFF, bool FSC, bool AM>
 __global__ void __launch_bounds__(SOL_BX *SOL_BY)
 k_solute(DevBufs b, Consts c, int cur, int nxt)
 {
@@ -436,7 +446,7 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
             const int g_old = b.gr[idx];
             double cl_out, fs_out;
             int g;
-            partition_cell(c, cnt->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
+            partition_cell<AM>(c, cnt->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
                            &cl_out, &fs_out, &g);
             b.cl[nxt][idx] = cl_out;
             b.fs[nxt][idx] = fs_out;
@@ -458,7 +468,7 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
 // stay in registers, x neighbours come from the neighbouring lanes
 // (wave shuffles), y neighbours from the rows of the other waves in LDS;
 // only tile edges load neighbours from memory.
-template <int BY>
+template <int BY, bool AM>
 __global__ void __launch_bounds__(64 * BY)
 k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
 {
@@ -511,24 +521,32 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
     const int mo_x1 = (x == c.nx) ? c.pad[0] : 0, mo_x0 = (x == 1) ? c.pad[0] : 0;
     const int mo_y1 = (j == c.ny) ? c.pad[1] : 0, mo_y0 = (j == 1) ? c.pad[1] : 0;
     double clm = 0, clv = 0, clp = 0, fsm = 0, fsv = 0, fsp = 0;
+    double ce_nx = 0;
+    int g_nx = 0;
     if (active)
     {
         clm = clc[(k0 - 1) * c.sxy + col];
         fsm = fsc[(k0 - 1) * c.sxy + col];
         clv = clc[k0 * c.sxy + col];
         fsv = fsc[k0 * c.sxy + col];
+        ce_nx = b.ce[k0 * c.sxy + col];
+        g_nx = b.gr[k0 * c.sxy + col];
     }
     for (int k = k0; k <= k1; ++k)
     {
         const long long idx = k * c.sxy + col;
-        double ce_old = 0;
-        int g_old = 0;
+        const double ce_old = ce_nx;
+        const int g_old = g_nx;
         if (active)
         {
+            // next plane's values are loaded one iteration ahead
             clp = clc[idx + c.sxy];
             fsp = fsc[idx + c.sxy];
-            ce_old = b.ce[idx];
-            g_old = b.gr[idx];
+            if (k < k1)
+            {
+                ce_nx = b.ce[idx + c.sxy];
+                g_nx = b.gr[idx + c.sxy];
+            }
         }
         s_cl[ty][lane] = clv;
         s_fs[ty][lane] = fsv;
@@ -578,7 +596,7 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
                 flag_negative_ce(cnt);
             b.ce[idx] = ce_new;
             double cl_out, fs_out;
-            partition_cell(c, sim_time, x, j, k, ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out,
+            partition_cell<AM>(c, sim_time, x, j, k, ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out,
                            &g_new);
             cln_out[idx] = cl_out;
             fsn_out[idx] = fs_out;
@@ -1034,8 +1052,9 @@ hipError_t launch_nucleate(const DevBufs &b, const Consts &c, int mode, int cur,
     return hipGetLastError();
 }
 
-hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
-                         hipStream_t s)
+template <bool AM>
+static void launch_solute_t(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
+                            hipStream_t s)
 {
     dim3 blk(SOL_BX, SOL_BY);
     if (diff && fsc && c.zm_planes > 0)
@@ -1045,21 +1064,30 @@ hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, i
                              ((c.nz + c.zm_planes - 1) / c.zm_planes);
         const long long ng = c.xcd_remap ? ((nt + 7) / 8) * 8 : nt;
         if (by == 16)
-            hipLaunchKernelGGL((k_solute_zm<16>), dim3((unsigned) ng), dim3(64, 16), 0, s, b, c, cur, nxt,
-                               c.zm_planes);
+            hipLaunchKernelGGL((k_solute_zm<16, AM>), dim3((unsigned) ng), dim3(64, 16), 0, s, b, c, cur,
+                               nxt, c.zm_planes);
         else if (by == 8)
-            hipLaunchKernelGGL((k_solute_zm<8>), dim3((unsigned) ng), dim3(64, 8), 0, s, b, c, cur, nxt,
-                               c.zm_planes);
+            hipLaunchKernelGGL((k_solute_zm<8, AM>), dim3((unsigned) ng), dim3(64, 8), 0, s, b, c, cur,
+                               nxt, c.zm_planes);
         else
-            hipLaunchKernelGGL((k_solute_zm<4>), dim3((unsigned) ng), dim3(64, 4), 0, s, b, c, cur, nxt,
-                               c.zm_planes);
+            hipLaunchKernelGGL((k_solute_zm<4, AM>), dim3((unsigned) ng), dim3(64, 4), 0, s, b, c, cur,
+                               nxt, c.zm_planes);
     }
     else if (diff && fsc)
-        hipLaunchKernelGGL((k_solute<true, true>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<true, true, AM>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
     else if (diff)
-        hipLaunchKernelGGL((k_solute<true, false>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<true, false, AM>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
     else
-        hipLaunchKernelGGL((k_solute<false, true>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<false, true, AM>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+}
+
+hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
+                         hipStream_t s)
+{
+    if (c.am)
+        launch_solute_t<true>(b, c, diff, fsc, cur, nxt, s);
+    else
+        launch_solute_t<false>(b, c, diff, fsc, cur, nxt, s);
     return hipGetLastError();
 }
 

@@ -153,6 +153,8 @@ typedef struct mmca_report {
     uint64_t failed_step;     /* timestep at which status was raised       */
     double   solid_sum;       /* sum of min(1,fs) over the interior (growth.c:541-566), if requested */
     double   solute_sum;      /* sum of ce over the interior, if requested */
+    double   solute_ledger;   /* FP32 storage: solute lost to rounding since creation (compensated
+                                 FP64 sum); sum + ledger is the conserved total.  0 for FP64 */
 } mmca_report;
 
 /* Field identifiers for upload/download (padded host arrays). */

@@ -134,6 +134,7 @@ class Report(C.Structure):
         ("failed_step", C.c_uint64),
         ("solid_sum", C.c_double),
         ("solute_sum", C.c_double),
+        ("solute_ledger", C.c_double),
     ]
 
 

@@ -141,6 +141,9 @@ int build_consts(const mmca_params &p, Consts &c)
     c.zm_planes = 16;                   // z-marching solute kernel by default
     c.zm_by = 8;
     c.xcd_remap = 1;
+    if (p.precision != MMCA_PREC_FP64 && p.precision != MMCA_PREC_FP32DEV)
+        return MMCA_ERR_ARG;
+    c.f32 = (p.precision == MMCA_PREC_FP32DEV) ? 1 : 0;
     for (int a = 0; a < 3; a++) {
         c.pad[a] = p.pad_bndy[a] ? 1 : 0;
     }

@@ -163,6 +163,23 @@ int check_status(mmca_ctx *ctx)
     return ctx->hc.status;
 }
 
+// solute lost to FP32 storage rounding: sum over workgroup slots of the
+// Kahan pairs (sum - compensation)
+int read_ledger(mmca_ctx *ctx, double *out)
+{
+    std::vector<double> l((size_t) (2 * ctx->b.nledger));
+    HIPCHK(hipMemcpy(l.data(), ctx->b.ledger, l.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double s = 0.0, comp = 0.0;
+    for (long long i = 0; i < ctx->b.nledger; i++) {
+        double v = l[(size_t) (2 * i)] - l[(size_t) (2 * i + 1)];
+        double y = v - comp, t = s + y;
+        comp = (t - s) - y;
+        s = t;
+    }
+    *out = s;
+    return MMCA_OK;
+}
+
 // one doiteration() on the device (calculate.c:193-352)
 int enqueue_iteration(mmca_ctx *ctx, bool advance)
 {
@@ -262,9 +279,17 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     DevBufs &b = ctx->b;
     const size_t P = (size_t) c.P;
     const size_t N = (size_t) c.nx * c.ny * c.nz;
-    TRY(dev_alloc(&b.ce, P));
-    TRY(dev_alloc(&b.cl[0], P));
-    TRY(dev_alloc(&b.cl[1], P));
+    if (c.f32) {
+        TRY(dev_alloc(&b.ce32, P));
+        TRY(dev_alloc(&b.cl32[0], P));
+        TRY(dev_alloc(&b.cl32[1], P));
+        b.nledger = (long long) ((c.nx + 63) / 64) * ((c.ny + 3) / 4) * c.nz;
+        TRY(dev_alloc(&b.ledger, (size_t) (2 * b.nledger)));
+    } else {
+        TRY(dev_alloc(&b.ce, P));
+        TRY(dev_alloc(&b.cl[0], P));
+        TRY(dev_alloc(&b.cl[1], P));
+    }
     TRY(dev_alloc(&b.fs[0], P));
     TRY(dev_alloc(&b.fs[1], P));
     TRY(dev_alloc(&b.gr, P));
@@ -293,9 +318,16 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     TRY(dev_alloc(&b.cnt, 1));
     b.nsites = 0;
     // SetupInitialConditions (setup.c:58-65): ce = cl = Cinit, all else 0
-    HIPCHK(launch_fill_f64(b.ce, (long long) P, c.cinit, ctx->stream));
-    HIPCHK(launch_fill_f64(b.cl[0], (long long) P, c.cinit, ctx->stream));
-    HIPCHK(launch_fill_f64(b.cl[1], (long long) P, c.cinit, ctx->stream));
+    if (c.f32) {   // deviations from cinit
+        HIPCHK(hipMemsetAsync(b.ce32, 0, P * sizeof(float), ctx->stream));
+        HIPCHK(hipMemsetAsync(b.cl32[0], 0, P * sizeof(float), ctx->stream));
+        HIPCHK(hipMemsetAsync(b.cl32[1], 0, P * sizeof(float), ctx->stream));
+        HIPCHK(hipMemsetAsync(b.ledger, 0, (size_t) (2 * b.nledger) * sizeof(double), ctx->stream));
+    } else {
+        HIPCHK(launch_fill_f64(b.ce, (long long) P, c.cinit, ctx->stream));
+        HIPCHK(launch_fill_f64(b.cl[0], (long long) P, c.cinit, ctx->stream));
+        HIPCHK(launch_fill_f64(b.cl[1], (long long) P, c.cinit, ctx->stream));
+    }
     HIPCHK(hipMemsetAsync(b.fs[0], 0, P * sizeof(double), ctx->stream));
     HIPCHK(hipMemsetAsync(b.fs[1], 0, P * sizeof(double), ctx->stream));
     HIPCHK(hipMemsetAsync(b.gr, 0, P * sizeof(int), ctx->stream));
@@ -327,7 +359,7 @@ int mmca_destroy(mmca_ctx *ctx)
     if (ctx->stream)
         (void) hipStreamSynchronize(ctx->stream);
     DevBufs &b = ctx->b;
-    void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.fs[0], b.fs[1], b.gr, b.dcx, b.dcy, b.dcz, b.mask,
+    void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.ce32, b.cl32[0], b.cl32[1], b.ledger, b.fs[0], b.fs[1], b.gr, b.dcx, b.dcy, b.dcz, b.mask,
                      b.diff_id, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
                      b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.fixed_rot, b.rand_rot,
                      b.oce, b.d, b.red_part, b.cnt, ctx->d_red };
@@ -456,6 +488,10 @@ int mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report)
             HIPCHK(hipMemcpy(s2, ctx->d_red, sizeof(s2), hipMemcpyDeviceToHost));
             report->solid_sum = s2[0];
             report->solute_sum = s2[1];
+            if (ctx->c.f32) {
+                report->solute_sum += (double) ctx->c.nx * ctx->c.ny * ctx->c.nz * ctx->c.cinit;
+                TRY(read_ledger(ctx, &report->solute_ledger));
+            }
         }
     }
     return ctx->hc.status;
@@ -508,6 +544,8 @@ int mmca_sb_diffuse_alloy_decentered(mmca_ctx *ctx)
 {
     if (!ctx)
         return MMCA_ERR_ARG;
+    if (ctx->c.f32)
+        return MMCA_ERR_STATE;   // single operators run on FP64 storage
     HIPCHK(hipSetDevice(ctx->device));
     int st = courant_check(ctx);
     if (st != MMCA_OK)
@@ -525,6 +563,8 @@ int mmca_fs_change_diffuse(mmca_ctx *ctx)
 {
     if (!ctx)
         return MMCA_ERR_ARG;
+    if (ctx->c.f32)
+        return MMCA_ERR_STATE;   // single operators run on FP64 storage
     if (!ctx->have_oce)
         return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
@@ -605,11 +645,24 @@ int mmca_upload_field(mmca_ctx *ctx, int field, const void *host)
     DevBufs &b = ctx->b;
     switch (field) {
     case MMCA_FIELD_CE:
-        HIPCHK(hipMemcpy(b.ce, host, P * sizeof(double), hipMemcpyHostToDevice));
-        break;
     case MMCA_FIELD_CL:
-        HIPCHK(hipMemcpy(b.cl[0], host, P * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b.cl[1], host, P * sizeof(double), hipMemcpyHostToDevice));
+        if (ctx->c.f32) {
+            const double *h = (const double *) host;
+            std::vector<float> tmp(P);
+            for (size_t i = 0; i < P; i++)
+                tmp[i] = (float) (h[i] - ctx->c.cinit);
+            if (field == MMCA_FIELD_CE) {
+                HIPCHK(hipMemcpy(b.ce32, tmp.data(), P * sizeof(float), hipMemcpyHostToDevice));
+            } else {
+                HIPCHK(hipMemcpy(b.cl32[0], tmp.data(), P * sizeof(float), hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(b.cl32[1], tmp.data(), P * sizeof(float), hipMemcpyHostToDevice));
+            }
+        } else if (field == MMCA_FIELD_CE) {
+            HIPCHK(hipMemcpy(b.ce, host, P * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(hipMemcpy(b.cl[0], host, P * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b.cl[1], host, P * sizeof(double), hipMemcpyHostToDevice));
+        }
         break;
     case MMCA_FIELD_FS:
         HIPCHK(hipMemcpy(b.fs[0], host, P * sizeof(double), hipMemcpyHostToDevice));
@@ -630,6 +683,8 @@ int mmca_upload_field(mmca_ctx *ctx, int field, const void *host)
         break;
     }
     case MMCA_FIELD_OCE:
+        if (ctx->c.f32)
+            return MMCA_ERR_STATE;
         TRY(ensure_oce(ctx));
         HIPCHK(hipMemcpy(b.oce, host, P * sizeof(double), hipMemcpyHostToDevice));
         ctx->have_oce = true;
@@ -651,10 +706,18 @@ int mmca_download_field(mmca_ctx *ctx, int field, void *host)
     DevBufs &b = ctx->b;
     switch (field) {
     case MMCA_FIELD_CE:
-        HIPCHK(hipMemcpy(host, b.ce, P * sizeof(double), hipMemcpyDeviceToHost));
-        break;
     case MMCA_FIELD_CL:
-        HIPCHK(hipMemcpy(host, b.cl[ctx->cur], P * sizeof(double), hipMemcpyDeviceToHost));
+        if (c.f32) {
+            std::vector<float> tmp(P);
+            const float *src = (field == MMCA_FIELD_CE) ? b.ce32 : b.cl32[ctx->cur];
+            HIPCHK(hipMemcpy(tmp.data(), src, P * sizeof(float), hipMemcpyDeviceToHost));
+            double *h = (double *) host;
+            for (size_t i = 0; i < P; i++)
+                h[i] = c.cinit + (double) tmp[i];
+        } else {
+            const double *src = (field == MMCA_FIELD_CE) ? b.ce : b.cl[ctx->cur];
+            HIPCHK(hipMemcpy(host, src, P * sizeof(double), hipMemcpyDeviceToHost));
+        }
         break;
     case MMCA_FIELD_FS:
         HIPCHK(hipMemcpy(host, b.fs[ctx->cur], P * sizeof(double), hipMemcpyDeviceToHost));

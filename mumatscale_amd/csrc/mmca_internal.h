@@ -24,6 +24,7 @@ struct Consts {
     int zm_planes;          // > 0: z-marching solute kernel with this many planes per workgroup
     int zm_by;              // rows per z-marching workgroup (4, 8, 16)
     int xcd_remap;          // 1: contiguous tile runs per XCD
+    int f32;                // 1: ce, cl stored as FP32 deviations from cinit
     // diffusion (growth.c:38-40)
     double rs, rl;
     // partitioning (growth.c:181-189)
@@ -60,6 +61,10 @@ struct Counters {
 struct DevBufs {
     double *ce;
     double *cl[2];
+    float *ce32;                     // FP32 storage: ce - cinit
+    float *cl32[2];                  // FP32 storage: cl - cinit
+    double *ledger;                  // 2 per workgroup slot: Kahan sum of FP32 storage rounding of ce
+    long long nledger;
     double *fs[2];
     int *gr;
     double *dcx, *dcy, *dcz;         // decentered octahedron centres (SoA)

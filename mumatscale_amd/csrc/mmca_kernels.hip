@@ -26,6 +26,79 @@ namespace mmca {
 // MIN(x,y) of globals.h:19
 __device__ __forceinline__ double ref_min(double x, double y) { return (x < y) ? x : y; }
 
+// Solute storage: FP64 values, or FP32 deviations from c0 (F32).  Arithmetic
+// is FP64 in both cases.
+template <bool F32>
+__device__ __forceinline__ double
+ld_ce(const DevBufs &b, const Consts &c, long long i)
+{
+    return F32 ? c.cinit + (double) b.ce32[i] : b.ce[i];
+}
+
+template <bool F32>
+__device__ __forceinline__ double
+ld_cl(const DevBufs &b, const Consts &c, int buf, long long i)
+{
+    return F32 ? c.cinit + (double) b.cl32[buf][i] : b.cl[buf][i];
+}
+
+// store ce; returns the FP64 solute lost to the FP32 rounding (0 for FP64)
+template <bool F32>
+__device__ __forceinline__ double
+st_ce(const DevBufs &b, const Consts &c, long long i, double v)
+{
+    if (F32)
+    {
+        const double dev = v - c.cinit;
+        const float f = (float) dev;
+        b.ce32[i] = f;
+        return dev - (double) f;
+    }
+    b.ce[i] = v;
+    return 0.0;
+}
+
+template <bool F32>
+__device__ __forceinline__ void
+st_cl(const DevBufs &b, const Consts &c, int buf, long long i, double v)
+{
+    if (F32)
+        b.cl32[buf][i] = (float) (v - c.cinit);
+    else
+        b.cl[buf][i] = v;
+}
+
+// Compensated (Kahan) addition of a workgroup's residual into its ledger slot.
+__device__ __forceinline__ void
+ledger_add(const DevBufs &b, long long slot, double v)
+{
+    double s = b.ledger[2 * slot], comp = b.ledger[2 * slot + 1];
+    double y = v - comp;
+    double t = s + y;
+    comp = (t - s) - y;
+    b.ledger[2 * slot] = t;
+    b.ledger[2 * slot + 1] = comp;
+}
+
+// Sum over a workgroup of NT threads (result valid in thread 0).
+template <int NT>
+__device__ __forceinline__ double
+block_sum(double v, double *s_red)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1)
+        v += __shfl_xor(v, d, 64);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0)
+        s_red[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int i = 0; i < NT / 64; ++i)
+            s += s_red[i];
+    return s;
+}
+
 // direct_temp_calc (temperature.c:24-126); x,y,z are padded indices.  AM
 // selects the Gaussian-beam model at compile time, so that kernels of the
 // other models do not carry its registers.
@@ -194,7 +267,7 @@ k_nuc_detect(DevBufs b, Consts c, int advance, int cur)
         {
             double Tunder = c.Tliq - T;
             double m_solute0 = c.ma * T * T + c.mb * T + c.mc;
-            Tunder += m_solute0 * (b.cl[cur][idx] - c.cinit);
+            Tunder += m_solute0 * ((c.f32 ? ld_cl<true>(b, c, cur, idx) : ld_cl<false>(b, c, cur, idx)) - c.cinit);
             if (Tunder > (double) b.site_thr[s])
                 flag = 1;
         }
@@ -398,27 +471,33 @@ flag_negative_ce(Counters *cnt)
 constexpr int SOL_BX = 64;
 constexpr int SOL_BY = 4;
 
-template <bool DIFF, bool FSC, bool AM>
+template <bool DIFF, bool FSC, bool AM, bool F32>
 __global__ void __launch_bounds__(SOL_BX *SOL_BY)
 k_solute(DevBufs b, Consts c, int cur, int nxt)
 {
+    __shared__ double s_red[SOL_BX * SOL_BY / 64];
+    __shared__ int s_stop;
     Counters *cnt = b.cnt;
-    if (cnt->status != MMCA_OK)
+    // this kernel itself may raise the status: read it once per workgroup so
+    // that every wave takes the same branch before the barrier of block_sum
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        s_stop = (cnt->status != MMCA_OK);
+    __syncthreads();
+    if (s_stop)
         return;
     const int x = blockIdx.x * SOL_BX + threadIdx.x + 1;   // interior x
     const int j = blockIdx.y * SOL_BY + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
-    if (j > c.ny)
-        return;                                          // whole wave
-    const bool active = (x <= c.nx);
+    const bool rowok = (j <= c.ny);
+    const bool active = rowok && (x <= c.nx);
     int g_new = 0;
+    double lost = 0.0;                                   // FP32 storage rounding of ce
     if (active)
     {
         const long long idx = k * c.sxy + (long long) j * c.sx + x;
-        const double *clc = b.cl[cur];
         const double *fsc = b.fs[cur];
-        const double ce_old = b.ce[idx];
-        const double conc = clc[idx];
+        const double ce_old = ld_ce<F32>(b, c, idx);
+        const double conc = ld_cl<F32>(b, c, cur, idx);
         const double lspfs = fsc[idx];
         double ce_new = ce_old;
         if (DIFF)
@@ -432,13 +511,13 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
 #pragma unroll
             for (int n = 0; n < 6; ++n)
             {
-                cln[n] = clc[nb[n]];
+                cln[n] = ld_cl<F32>(b, c, cur, nb[n]);
                 fsn[n] = fsc[nb[n]];
             }
             ce_new = diffuse_cell(c, ce_old, conc, lspfs, cln, fsn, mo);
             if (c.check_neg && ce_new < 0)
                 flag_negative_ce(cnt);
-            b.ce[idx] = ce_new;
+            lost = st_ce<F32>(b, c, idx, ce_new);
         }
         if (FSC)
         {
@@ -447,8 +526,8 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
             double cl_out, fs_out;
             int g;
             partition_cell<AM>(c, cnt->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
-                           &cl_out, &fs_out, &g);
-            b.cl[nxt][idx] = cl_out;
+                               &cl_out, &fs_out, &g);
+            st_cl<F32>(b, c, nxt, idx, cl_out);
             b.fs[nxt][idx] = fs_out;
             if (g != g_old)
                 b.gr[idx] = g;
@@ -458,8 +537,15 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
     if (FSC)
     {
         unsigned long long bits = __ballot(active && g_new > 0);
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0 && rowok)
             b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + blockIdx.x] = bits;
+    }
+    if (F32)
+    {
+        const double tot = block_sum<SOL_BX * SOL_BY>(lost, s_red);
+        if (threadIdx.x == 0 && threadIdx.y == 0 && tot != 0.0)
+            ledger_add(b, blockIdx.x + (long long) gridDim.x * (blockIdx.y + (long long) gridDim.y * blockIdx.z),
+                       tot);
     }
 }
 
@@ -468,12 +554,13 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
 // stay in registers, x neighbours come from the neighbouring lanes
 // (wave shuffles), y neighbours from the rows of the other waves in LDS;
 // only tile edges load neighbours from memory.
-template <int BY, bool AM>
+template <int BY, bool AM, bool F32>
 __global__ void __launch_bounds__(64 * BY)
 k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
 {
     __shared__ double s_cl[BY][64];
     __shared__ double s_fs[BY][64];
+    __shared__ double s_red[BY];
     __shared__ int s_stop;
     Counters *cnt = b.cnt;
     // this kernel itself may raise the status: read it once per workgroup so
@@ -508,9 +595,7 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
     const int k1 = min(c.nz, k0 + kz - 1);
     const bool rowok = (j <= c.ny);
     const bool active = rowok && (x <= c.nx);
-    const double *clc = b.cl[cur];
     const double *fsc = b.fs[cur];
-    double *cln_out = b.cl[nxt];
     double *fsn_out = b.fs[nxt];
     const long long col = active ? ((long long) j * c.sx + x) : 0;
     const double sim_time = cnt->timestep * c.dt;
@@ -523,13 +608,14 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
     double clm = 0, clv = 0, clp = 0, fsm = 0, fsv = 0, fsp = 0;
     double ce_nx = 0;
     int g_nx = 0;
+    double lost = 0.0;
     if (active)
     {
-        clm = clc[(k0 - 1) * c.sxy + col];
+        clm = ld_cl<F32>(b, c, cur, (k0 - 1) * c.sxy + col);
         fsm = fsc[(k0 - 1) * c.sxy + col];
-        clv = clc[k0 * c.sxy + col];
+        clv = ld_cl<F32>(b, c, cur, k0 * c.sxy + col);
         fsv = fsc[k0 * c.sxy + col];
-        ce_nx = b.ce[k0 * c.sxy + col];
+        ce_nx = ld_ce<F32>(b, c, k0 * c.sxy + col);
         g_nx = b.gr[k0 * c.sxy + col];
     }
     for (int k = k0; k <= k1; ++k)
@@ -540,11 +626,11 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
         if (active)
         {
             // next plane's values are loaded one iteration ahead
-            clp = clc[idx + c.sxy];
+            clp = ld_cl<F32>(b, c, cur, idx + c.sxy);
             fsp = fsc[idx + c.sxy];
             if (k < k1)
             {
-                ce_nx = b.ce[idx + c.sxy];
+                ce_nx = ld_ce<F32>(b, c, idx + c.sxy);
                 g_nx = b.gr[idx + c.sxy];
             }
         }
@@ -558,12 +644,12 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
         {
             if (xlo_mem)
             {
-                clxm = clc[idx - 1];
+                clxm = ld_cl<F32>(b, c, cur, idx - 1);
                 fsxm = fsc[idx - 1];
             }
             if (xhi_mem)
             {
-                clxp = clc[idx + 1];
+                clxp = ld_cl<F32>(b, c, cur, idx + 1);
                 fsxp = fsc[idx + 1];
             }
             double clym, fsym, clyp, fsyp;
@@ -574,7 +660,7 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
             }
             else
             {
-                clym = clc[idx - c.sx];
+                clym = ld_cl<F32>(b, c, cur, idx - c.sx);
                 fsym = fsc[idx - c.sx];
             }
             if (yhi_lds)
@@ -584,7 +670,7 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
             }
             else
             {
-                clyp = clc[idx + c.sx];
+                clyp = ld_cl<F32>(b, c, cur, idx + c.sx);
                 fsyp = fsc[idx + c.sx];
             }
             const double cln[6] = { clxp, clxm, clyp, clym, clp, clm };
@@ -594,11 +680,11 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
             const double ce_new = diffuse_cell(c, ce_old, clv, fsv, cln, fsn, mo);
             if (c.check_neg && ce_new < 0)
                 flag_negative_ce(cnt);
-            b.ce[idx] = ce_new;
+            lost += st_ce<F32>(b, c, idx, ce_new);
             double cl_out, fs_out;
             partition_cell<AM>(c, sim_time, x, j, k, ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out,
-                           &g_new);
-            cln_out[idx] = cl_out;
+                               &g_new);
+            st_cl<F32>(b, c, nxt, idx, cl_out);
             fsn_out[idx] = fs_out;
             if (g_new != g_old)
                 b.gr[idx] = g_new;
@@ -611,6 +697,12 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
         clv = clp;
         fsm = fsv;
         fsv = fsp;
+    }
+    if (F32)
+    {
+        const double tot = block_sum<64 * BY>(lost, s_red);
+        if (lane == 0 && ty == 0 && tot != 0.0)
+            ledger_add(b, tile, tot);
     }
 }
 
@@ -908,7 +1000,10 @@ k_capture(DevBufs b, Consts c, long long nwords, int cb, int do_capture)
             tz = ddz + nnz[found] + g20 * ncx + g21 * ncy + g22 * ncz;
             b.gr[idx] = gfound;
             fsb[idx] = 0.;
-            clb[idx] = b.ce[idx];
+            if (c.f32)
+                b.cl32[cb][idx] = b.ce32[idx];
+            else
+                clb[idx] = b.ce[idx];
         }
         if (found > -1 || realloc)
         {
@@ -942,7 +1037,7 @@ k_reduce_partial(DevBufs b, Consts c, int cb)
         int k = (int) (row / c.ny) + 1;
         long long idx = k * c.sxy + (long long) j * c.sx + x;
         a += ref_min(1.0, fs[idx]);
-        s += b.ce[idx];
+        s += c.f32 ? (double) b.ce32[idx] : b.ce[idx];     // FP32: deviation from c0
     }
     s_a[threadIdx.x] = a;
     s_b[threadIdx.x] = s;
@@ -1052,7 +1147,7 @@ hipError_t launch_nucleate(const DevBufs &b, const Consts &c, int mode, int cur,
     return hipGetLastError();
 }
 
-template <bool AM>
+template <bool AM, bool F32>
 static void launch_solute_t(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
                             hipStream_t s)
 {
@@ -1064,30 +1159,42 @@ static void launch_solute_t(const DevBufs &b, const Consts &c, int diff, int fsc
                              ((c.nz + c.zm_planes - 1) / c.zm_planes);
         const long long ng = c.xcd_remap ? ((nt + 7) / 8) * 8 : nt;
         if (by == 16)
-            hipLaunchKernelGGL((k_solute_zm<16, AM>), dim3((unsigned) ng), dim3(64, 16), 0, s, b, c, cur,
-                               nxt, c.zm_planes);
+            hipLaunchKernelGGL((k_solute_zm<16, AM, F32>), dim3((unsigned) ng), dim3(64, 16), 0, s, b, c,
+                               cur, nxt, c.zm_planes);
         else if (by == 8)
-            hipLaunchKernelGGL((k_solute_zm<8, AM>), dim3((unsigned) ng), dim3(64, 8), 0, s, b, c, cur,
-                               nxt, c.zm_planes);
+            hipLaunchKernelGGL((k_solute_zm<8, AM, F32>), dim3((unsigned) ng), dim3(64, 8), 0, s, b, c,
+                               cur, nxt, c.zm_planes);
         else
-            hipLaunchKernelGGL((k_solute_zm<4, AM>), dim3((unsigned) ng), dim3(64, 4), 0, s, b, c, cur,
-                               nxt, c.zm_planes);
+            hipLaunchKernelGGL((k_solute_zm<4, AM, F32>), dim3((unsigned) ng), dim3(64, 4), 0, s, b, c,
+                               cur, nxt, c.zm_planes);
     }
     else if (diff && fsc)
-        hipLaunchKernelGGL((k_solute<true, true, AM>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<true, true, AM, F32>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+    else if (F32)
+        return;     // single operators run on FP64 storage only (the caller checks)
     else if (diff)
-        hipLaunchKernelGGL((k_solute<true, false, AM>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<true, false, AM, false>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
     else
-        hipLaunchKernelGGL((k_solute<false, true, AM>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<false, true, AM, false>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
 }
 
 hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
                          hipStream_t s)
 {
     if (c.am)
-        launch_solute_t<true>(b, c, diff, fsc, cur, nxt, s);
+    {
+        if (c.f32)
+            launch_solute_t<true, true>(b, c, diff, fsc, cur, nxt, s);
+        else
+            launch_solute_t<true, false>(b, c, diff, fsc, cur, nxt, s);
+    }
     else
-        launch_solute_t<false>(b, c, diff, fsc, cur, nxt, s);
+    {
+        if (c.f32)
+            launch_solute_t<false, true>(b, c, diff, fsc, cur, nxt, s);
+        else
+            launch_solute_t<false, false>(b, c, diff, fsc, cur, nxt, s);
+    }
     return hipGetLastError();
 }
 

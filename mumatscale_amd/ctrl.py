@@ -217,12 +217,12 @@ class Config:
         return [self.nsub[a] * self.sbdim[a] for a in range(3)]
 
     def params(self, check_negative_ce=True, precision=abi.PREC_FP64, sb_coords=(0, 0, 0),
-               nx=None, ny=None, nz=None):
-        """mmca_params of one subblock (the whole domain when nsub = 1 1 1)."""
+               whole_domain=True):
+        """mmca_params of the whole domain as one block (whole_domain), or of
+        the subblock at sb_coords."""
         p = abi.Params()
-        p.nx = nx if nx is not None else self.sbdim[0]
-        p.ny = ny if ny is not None else self.sbdim[1]
-        p.nz = nz if nz is not None else self.sbdim[2]
+        dims = self.gdim if whole_domain else self.sbdim
+        p.nx, p.ny, p.nz = dims
         p.pad_bndy[:] = self.pad_bndy
         p.sb_coords[:] = list(sb_coords)
         p.gdim[:] = self.gdim
@@ -246,7 +246,10 @@ class Config:
             p.temp_model = abi.TEMP_UNIFORM
         p.am_layers = self.am_layers
         p.am_tracks = self.am_tracks
-        p.gnsbx = self.nsub[0]
+        p.gnsbx = self.nsub[0] if not whole_domain else 1
+        if whole_domain and self.nsub[0] != 1:
+            # the beam track length uses gnsbx * nx (temperature.c:68)
+            p.gnsbx = 1
         p.t_init = self.t_init
         p.cool_rate = self.cool_rate
         p.gradient = self.gradient

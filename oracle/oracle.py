@@ -130,6 +130,23 @@ class OracleSubblock:
     def step(self, n=1):
         return lib().orc_step(self._h, n)
 
+    def step_report(self, n=1, want_sums=False):
+        """step() with an mmca_report, as the HIP Subblock returns."""
+        st = lib().orc_step(self._h, n)
+        rep = abi.Report()
+        rep.timestep = self.timestep
+        rep.num_grains = self.num_grains
+        rep.gindex = self.gindex
+        rep.status = st
+        rep.failed_step = self.failed_step
+        if want_sums:
+            rep.solid_sum = self.solid_sum()
+            rep.solute_sum = self.solute_sum()
+        return rep
+
+    def download(self, which):
+        return self.field(which).copy()
+
     def doiteration(self):
         return lib().orc_doiteration(self._h)
 
@@ -218,3 +235,19 @@ class OracleSubblock:
 
     def capture(self):
         lib().orc_capture(self._h)
+
+
+class OracleStepper:
+    """Adapter giving the oracle the step()/download() interface of the HIP Subblock."""
+
+    def __init__(self, orc):
+        self.orc = orc
+
+    def step(self, n=1, want_sums=False):
+        rep = self.orc.step_report(n, want_sums)
+        if rep.status:
+            raise abi.MmcaError(rep.status, "oracle step")
+        return rep
+
+    def download(self, which):
+        return self.orc.download(which)

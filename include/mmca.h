@@ -229,6 +229,7 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
 #define MMCA_TUNE_ZM_PLANES  0   /* >0: z-marching solute kernel, planes per workgroup; 0: cell-parallel */
 #define MMCA_TUNE_ZM_ROWS    1   /* rows per z-marching workgroup: 4, 8 or 16 */
 #define MMCA_TUNE_XCD_REMAP  2   /* 1: workgroups of one XCD take contiguous tiles */
+#define MMCA_TUNE_GRAPH      3   /* batches: 1 = replay steps from a captured HIP graph */
 int  mmca_set_tuning(mmca_ctx *ctx, int key, int value);
 int  mmca_get_tuning(mmca_ctx *ctx, int key, int *value);
 
@@ -236,6 +237,29 @@ int  mmca_get_tuning(mmca_ctx *ctx, int key, int *value);
  * measured with HIP events on the context's stream (0 when not timed). */
 int  mmca_set_kernel_timing(mmca_ctx *ctx, int enable);
 int  mmca_kernel_time_ms(mmca_ctx *ctx, double *avg_ms, int64_t *launches);
+
+/* --- batches: independent subblocks advanced together -------------------- */
+/*
+ * No reference counterpart: the reference advances one subblock per MPI
+ * rank.  A batch holds `ndomains` subblocks of any sizes (same precision and
+ * temperature-model class), each created as by mmca_create +
+ * mmca_init_nucleation (nucleation may be NULL), and advances them with one
+ * set of six launches per step.  Each domain keeps its own timestep, grains,
+ * RNG stream and errors; a domain that fails stops, the others go on.
+ */
+typedef struct mmca_batch mmca_batch;
+int  mmca_batch_create(const mmca_params *params, const mmca_nucleation_cfg *nucleation, int ndomains,
+                       int device, mmca_batch **out);
+int  mmca_batch_destroy(mmca_batch *batch);
+int  mmca_batch_size(mmca_batch *batch, int *ndomains);
+/* the context of domain i (for downloads and single operators; do not destroy) */
+int  mmca_batch_domain(mmca_batch *batch, int i, mmca_ctx **ctx);
+/* n steps of every domain; reports[ndomains] may be NULL.  Returns the first
+ * non-OK domain status (the others have stepped). */
+int  mmca_batch_step(mmca_batch *batch, int nsteps, int want_sums, mmca_report *reports);
+int  mmca_batch_set_tuning(mmca_batch *batch, int key, int value);
+int  mmca_batch_set_kernel_timing(mmca_batch *batch, int enable);
+int  mmca_batch_kernel_time_ms(mmca_batch *batch, double *avg_ms, int64_t *launches);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,7 @@ FIELD_OCE = 9
 TUNE_ZM_PLANES = 0
 TUNE_ZM_ROWS = 1
 TUNE_XCD_REMAP = 2
+TUNE_GRAPH = 3
 
 
 class Params(C.Structure):

@@ -17,13 +17,15 @@
 #include "mmca_internal.h"
 
 namespace mmca {
-hipError_t launch_nucleate(const DevBufs &b, const Consts &c, int mode, int cur, hipStream_t s);
-hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
+hipError_t launch_nucleate(const View &v, int mode, int cur, hipStream_t s);
+hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, int fsc, int cur, int nxt,
                          hipStream_t s);
 hipError_t launch_mask_from_gr(const DevBufs &b, const Consts &c, hipStream_t s);
-hipError_t launch_list(const DevBufs &b, const Consts &c, int cb, int capture, int decide_realloc,
-                       hipStream_t s);
-hipError_t launch_capture_only(const DevBufs &b, const Consts &c, int cb, hipStream_t s);
+hipError_t launch_list(const View &v, int cb, int capture, int decide_realloc, hipStream_t s);
+hipError_t launch_capture_only(const View &v, const DevBufs &b, int cb, hipStream_t s);
+int nuc_blocks(int nsites);
+long long cell_blocks(const Consts &c);
+long long zm_tiles(const Consts &c, int by);
 hipError_t launch_reduce(const DevBufs &b, const Consts &c, int cb, double *out2, hipStream_t s);
 hipError_t launch_fill_temp(const DevBufs &b, const Consts &c, double *out, hipStream_t s);
 hipError_t launch_grow_octahedra(const DevBufs &b, const Consts &c, int cb, hipStream_t s);
@@ -72,6 +74,11 @@ struct mmca_ctx {
     int rand_filled = 0;
     int rand_capacity = 0;
     double *d_red = nullptr;           // 2 doubles
+    // device view of this context as a one-domain launch
+    DomainDesc *d_desc = nullptr;
+    long long *d_off = nullptr;
+    View view;
+    bool view_dirty = true;
     // kernel timing
     bool timing = false;
     std::vector<hipEvent_t> ev;       // pool, reused
@@ -143,8 +150,10 @@ int ensure_random(mmca_ctx *ctx, long long upto)
 
 int ensure_oce(mmca_ctx *ctx)
 {
-    if (!ctx->b.oce)
+    if (!ctx->b.oce) {
         TRY(dev_alloc(&ctx->b.oce, (size_t) ctx->c.P));
+        ctx->view_dirty = true;
+    }
     return MMCA_OK;
 }
 
@@ -152,6 +161,7 @@ int ensure_d(mmca_ctx *ctx)
 {
     if (!ctx->b.d) {
         TRY(dev_alloc(&ctx->b.d, (size_t) ctx->c.P));
+        ctx->view_dirty = true;
         HIPCHK(hipMemsetAsync(ctx->b.d, 0, (size_t) ctx->c.P * sizeof(double), ctx->stream));
     }
     return MMCA_OK;
@@ -161,6 +171,67 @@ int check_status(mmca_ctx *ctx)
 {
     TRY(sync_counters(ctx));
     return ctx->hc.status;
+}
+
+int zm_rows(const Consts &c)
+{
+    return c.zm_planes > 0 ? c.zm_by : 0;
+}
+
+// Upload the descriptors of n domains and the per-kernel workgroup prefix
+// tables; fills `v`.  `desc`/`off` are (re)allocated to hold n domains.
+int build_view(mmca_ctx *const *ctxs, int n, int xcd_remap, int zm_by, DomainDesc **desc, long long **off,
+               int *cap, View *v, hipStream_t s)
+{
+    if (*cap < n) {
+        if (*desc)
+            (void) hipFree(*desc);
+        if (*off)
+            (void) hipFree(*off);
+        *desc = nullptr;
+        *off = nullptr;
+        TRY(dev_alloc(desc, (size_t) n));
+        TRY(dev_alloc(off, (size_t) 4 * (n + 1)));
+        *cap = n;
+    }
+    std::vector<DomainDesc> h(n);
+    std::vector<long long> t((size_t) 4 * (n + 1), 0);
+    long long *on = &t[0], *oz = &t[n + 1], *oc = &t[2 * (n + 1)], *ol = &t[3 * (n + 1)];
+    for (int d = 0; d < n; d++) {
+        h[d].b = ctxs[d]->b;
+        h[d].c = ctxs[d]->c;
+        on[d + 1] = on[d] + nuc_blocks(ctxs[d]->b.nsites);
+        oz[d + 1] = oz[d] + (zm_by > 0 ? zm_tiles(ctxs[d]->c, zm_by) : 0);
+        oc[d + 1] = oc[d] + cell_blocks(ctxs[d]->c);
+        ol[d + 1] = ol[d] + list_blocks(ctxs[d]->c);
+    }
+    HIPCHK(hipMemcpyAsync(*desc, h.data(), sizeof(DomainDesc) * (size_t) n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(*off, t.data(), sizeof(long long) * t.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    v->dom = *desc;
+    v->ndom = n;
+    v->xcd_remap = xcd_remap;
+    v->off_nuc = *off;
+    v->off_zm = *off + (n + 1);
+    v->off_cell = *off + 2 * (n + 1);
+    v->off_list = *off + 3 * (n + 1);
+    v->tot_nuc = on[n];
+    v->tot_zm = oz[n];
+    v->tot_cell = oc[n];
+    v->tot_list = ol[n];
+    return MMCA_OK;
+}
+
+int ensure_view(mmca_ctx *ctx)
+{
+    if (!ctx->view_dirty)
+        return MMCA_OK;
+    int cap = ctx->d_desc ? 1 : 0;
+    mmca_ctx *one[1] = { ctx };
+    TRY(build_view(one, 1, ctx->c.xcd_remap, zm_rows(ctx->c), &ctx->d_desc, &ctx->d_off, &cap, &ctx->view,
+                   ctx->stream));
+    ctx->view_dirty = false;
+    return MMCA_OK;
 }
 
 // solute lost to FP32 storage rounding: sum over workgroup slots of the
@@ -184,7 +255,8 @@ int read_ledger(mmca_ctx *ctx, double *out)
 int enqueue_iteration(mmca_ctx *ctx, bool advance)
 {
     const int nxt = ctx->cur ^ 1;
-    HIPCHK(launch_nucleate(ctx->b, ctx->c, advance ? 7 : 3, ctx->cur, ctx->stream));
+    TRY(ensure_view(ctx));
+    HIPCHK(launch_nucleate(ctx->view, advance ? 7 : 3, ctx->cur, ctx->stream));
     const size_t ev0 = ctx->ev_used;
     if (ctx->timing) {
         while (ctx->ev.size() < ev0 + 2) {
@@ -194,12 +266,12 @@ int enqueue_iteration(mmca_ctx *ctx, bool advance)
         }
         HIPCHK(hipEventRecord(ctx->ev[ev0], ctx->stream));
     }
-    HIPCHK(launch_solute(ctx->b, ctx->c, 1, 1, ctx->cur, nxt, ctx->stream));
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, ctx->c.f32, zm_rows(ctx->c), 1, 1, ctx->cur, nxt, ctx->stream));
     if (ctx->timing) {
         HIPCHK(hipEventRecord(ctx->ev[ev0 + 1], ctx->stream));
         ctx->ev_used = ev0 + 2;
     }
-    HIPCHK(launch_list(ctx->b, ctx->c, nxt, 1, 1, ctx->stream));
+    HIPCHK(launch_list(ctx->view, nxt, 1, 1, ctx->stream));
     ctx->cur = nxt;
     return MMCA_OK;
 }
@@ -362,7 +434,7 @@ int mmca_destroy(mmca_ctx *ctx)
     void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.ce32, b.cl32[0], b.cl32[1], b.ledger, b.fs[0], b.fs[1], b.gr, b.dcx, b.dcy, b.dcz, b.mask,
                      b.diff_id, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
                      b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.fixed_rot, b.rand_rot,
-                     b.oce, b.d, b.red_part, b.cnt, ctx->d_red };
+                     b.oce, b.d, b.red_part, b.cnt, ctx->d_red, ctx->d_desc, ctx->d_off };
     for (void *p : ptrs)
         if (p)
             (void) hipFree(p);
@@ -416,6 +488,7 @@ int mmca_init_nucleation(mmca_ctx *ctx, const mmca_nucleation_cfg *cfg)
     HIPCHK(hipMemcpy(b.fixed_rot, fixed_rot.data(), fixed_rot.size() * sizeof(double),
                      hipMemcpyHostToDevice));
     ctx->nucleation_ready = true;
+    ctx->view_dirty = true;
     // draw the first orientations now, outside any timed stepping
     TRY(ensure_random(ctx, std::min<long long>(ctx->rand_capacity,
                                                std::max<long long>(65536, 256LL * (long long) ns))));
@@ -522,7 +595,8 @@ int mmca_cell_nucleation(mmca_ctx *ctx, int32_t *n_new)
     if (!ctx)
         return MMCA_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    HIPCHK(launch_nucleate(ctx->b, ctx->c, 1, ctx->cur, ctx->stream));
+    TRY(ensure_view(ctx));
+    HIPCHK(launch_nucleate(ctx->view, 1, ctx->cur, ctx->stream));
     TRY(sync_counters(ctx));
     if (n_new)
         *n_new = ctx->hc.n_events;
@@ -536,7 +610,8 @@ int mmca_activate_new_grains(mmca_ctx *ctx)
     HIPCHK(hipSetDevice(ctx->device));
     TRY(sync_counters(ctx));
     TRY(ensure_random(ctx, (long long) ctx->hc.random_used + ctx->hc.n_events));
-    HIPCHK(launch_nucleate(ctx->b, ctx->c, 2, ctx->cur, ctx->stream));
+    TRY(ensure_view(ctx));
+    HIPCHK(launch_nucleate(ctx->view, 2, ctx->cur, ctx->stream));
     return check_status(ctx);
 }
 
@@ -554,7 +629,8 @@ int mmca_sb_diffuse_alloy_decentered(mmca_ctx *ctx)
     // oce = ce over the padded box (calculate.c:267-280)
     HIPCHK(hipMemcpyAsync(ctx->b.oce, ctx->b.ce, (size_t) ctx->c.P * sizeof(double),
                           hipMemcpyDeviceToDevice, ctx->stream));
-    HIPCHK(launch_solute(ctx->b, ctx->c, 1, 0, ctx->cur, ctx->cur, ctx->stream));
+    TRY(ensure_view(ctx));
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 1, 0, ctx->cur, ctx->cur, ctx->stream));
     ctx->have_oce = true;
     return check_status(ctx);
 }
@@ -568,7 +644,8 @@ int mmca_fs_change_diffuse(mmca_ctx *ctx)
     if (!ctx->have_oce)
         return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
-    HIPCHK(launch_solute(ctx->b, ctx->c, 0, 1, ctx->cur, ctx->cur, ctx->stream));
+    TRY(ensure_view(ctx));
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 0, 1, ctx->cur, ctx->cur, ctx->stream));
     return check_status(ctx);
 }
 
@@ -588,7 +665,8 @@ int mmca_grow_cell_reduction(mmca_ctx *ctx, int32_t *gindex)
         return MMCA_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(launch_mask_from_gr(ctx->b, ctx->c, ctx->stream));
-    HIPCHK(launch_list(ctx->b, ctx->c, ctx->cur, 0, 0, ctx->stream));
+    TRY(ensure_view(ctx));
+    HIPCHK(launch_list(ctx->view, ctx->cur, 0, 0, ctx->stream));
     TRY(sync_counters(ctx));
     if (gindex)
         *gindex = ctx->hc.gindex;
@@ -600,7 +678,8 @@ int mmca_capture_octahedra_diffuse(mmca_ctx *ctx)
     if (!ctx)
         return MMCA_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    HIPCHK(launch_capture_only(ctx->b, ctx->c, ctx->cur, ctx->stream));
+    TRY(ensure_view(ctx));
+    HIPCHK(launch_capture_only(ctx->view, ctx->b, ctx->cur, ctx->stream));
     return check_status(ctx);
 }
 
@@ -877,14 +956,17 @@ int mmca_set_tuning(mmca_ctx *ctx, int key, int value)
         if (value < 0 || value > ctx->c.nz * 2 + 1024)
             return MMCA_ERR_ARG;
         ctx->c.zm_planes = value;
+        ctx->view_dirty = true;
         return MMCA_OK;
     case MMCA_TUNE_ZM_ROWS:
         if (value != 4 && value != 8 && value != 16)
             return MMCA_ERR_ARG;
         ctx->c.zm_by = value;
+        ctx->view_dirty = true;
         return MMCA_OK;
     case MMCA_TUNE_XCD_REMAP:
         ctx->c.xcd_remap = value ? 1 : 0;
+        ctx->view_dirty = true;
         return MMCA_OK;
     default:
         return MMCA_ERR_ARG;
@@ -927,6 +1009,326 @@ int mmca_kernel_time_ms(mmca_ctx *ctx, double *avg_ms, int64_t *launches)
     if (launches)
         *launches = ctx->time_n;
     return MMCA_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches: independent subblocks (domains) advanced together, each launch of
+// a step covering every domain.
+struct mmca_batch {
+    std::vector<mmca_ctx *> dom;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DomainDesc *d_desc = nullptr;
+    long long *d_off = nullptr;
+    int cap = 0;
+    View view;
+    bool view_dirty = true;
+    int cur = 0;
+    int zm_planes = 16, zm_by = 8, xcd_remap = 1;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    double time_ms = 0.0;
+    long long time_n = 0;
+    bool use_graph = false;
+    hipGraphExec_t graph[2] = { nullptr, nullptr };
+};
+
+namespace {
+
+int batch_view(mmca_batch *bt)
+{
+    if (!bt->view_dirty)
+        return MMCA_OK;
+    for (mmca_ctx *c : bt->dom) {
+        c->c.zm_planes = bt->zm_planes;
+        c->c.zm_by = bt->zm_by;
+        c->c.xcd_remap = bt->xcd_remap;
+        c->view_dirty = true;
+    }
+    TRY(build_view(bt->dom.data(), (int) bt->dom.size(), bt->xcd_remap, bt->zm_planes > 0 ? bt->zm_by : 0,
+                   &bt->d_desc, &bt->d_off, &bt->cap, &bt->view, bt->stream));
+    for (int p = 0; p < 2; p++)
+        if (bt->graph[p]) {
+            (void) hipGraphExecDestroy(bt->graph[p]);
+            bt->graph[p] = nullptr;
+        }
+    bt->view_dirty = false;
+    return MMCA_OK;
+}
+
+int batch_sync(mmca_batch *bt)
+{
+    for (mmca_ctx *c : bt->dom)
+        HIPCHK(hipMemcpyAsync(&c->hc, c->b.cnt, sizeof(Counters), hipMemcpyDeviceToHost, bt->stream));
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    return MMCA_OK;
+}
+
+// one doiteration() of every domain: six launches whatever the batch size
+int batch_enqueue(mmca_batch *bt, int cur)
+{
+    const Consts &c0 = bt->dom[0]->c;
+    const int nxt = cur ^ 1;
+    HIPCHK(launch_nucleate(bt->view, 7, cur, bt->stream));
+    const size_t e0 = bt->ev_used;
+    if (bt->timing) {
+        while (bt->ev.size() < e0 + 2) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            bt->ev.push_back(e);
+        }
+        HIPCHK(hipEventRecord(bt->ev[e0], bt->stream));
+    }
+    HIPCHK(launch_solute(bt->view, c0.am, c0.f32, bt->zm_planes > 0 ? bt->zm_by : 0, 1, 1, cur, nxt,
+                         bt->stream));
+    if (bt->timing) {
+        HIPCHK(hipEventRecord(bt->ev[e0 + 1], bt->stream));
+        bt->ev_used = e0 + 2;
+    }
+    HIPCHK(launch_list(bt->view, nxt, 1, 1, bt->stream));
+    return MMCA_OK;
+}
+
+// a pair of steps starting from buffer `cur`, captured once into a graph
+int batch_graph_pair(mmca_batch *bt, int cur)
+{
+    if (!bt->graph[cur]) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+        int st = batch_enqueue(bt, cur);
+        if (st == MMCA_OK)
+            st = batch_enqueue(bt, cur ^ 1);
+        hipError_t e = hipStreamEndCapture(bt->stream, &g);
+        if (st != MMCA_OK)
+            return st;
+        HIPCHK(e);
+        HIPCHK(hipGraphInstantiate(&bt->graph[cur], g, nullptr, nullptr, 0));
+        (void) hipGraphDestroy(g);
+    }
+    HIPCHK(hipGraphLaunch(bt->graph[cur], bt->stream));
+    return MMCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mmca_batch_create(const mmca_params *params, const mmca_nucleation_cfg *nucleation, int ndomains,
+                      int device, mmca_batch **out)
+{
+    if (!params || !out || ndomains < 1)
+        return MMCA_ERR_ARG;
+    *out = nullptr;
+    for (int d = 1; d < ndomains; d++)
+        if (params[d].precision != params[0].precision ||
+            (params[d].temp_model == MMCA_TEMP_AM_BEAM) != (params[0].temp_model == MMCA_TEMP_AM_BEAM))
+            return MMCA_ERR_ARG;   // one kernel instantiation serves the whole batch
+    mmca_batch *bt = new mmca_batch();
+    bt->device = device;
+    *out = bt;
+    for (int d = 0; d < ndomains; d++) {
+        mmca_ctx *c = nullptr;
+        int st = mmca_create(&params[d], device, &c);
+        if (c)
+            bt->dom.push_back(c);
+        if (st != MMCA_OK)
+            return st;
+        if (nucleation)
+            TRY(mmca_init_nucleation(c, &nucleation[d]));
+    }
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreate(&bt->stream));
+    for (mmca_ctx *c : bt->dom)
+        HIPCHK(hipStreamSynchronize(c->stream));
+    return MMCA_OK;
+}
+
+int mmca_batch_destroy(mmca_batch *bt)
+{
+    if (!bt)
+        return MMCA_OK;
+    (void) hipSetDevice(bt->device);
+    if (bt->stream)
+        (void) hipStreamSynchronize(bt->stream);
+    for (int p = 0; p < 2; p++)
+        if (bt->graph[p])
+            (void) hipGraphExecDestroy(bt->graph[p]);
+    for (auto e : bt->ev)
+        (void) hipEventDestroy(e);
+    if (bt->d_desc)
+        (void) hipFree(bt->d_desc);
+    if (bt->d_off)
+        (void) hipFree(bt->d_off);
+    for (mmca_ctx *c : bt->dom)
+        mmca_destroy(c);
+    if (bt->stream)
+        (void) hipStreamDestroy(bt->stream);
+    delete bt;
+    return MMCA_OK;
+}
+
+int mmca_batch_size(mmca_batch *bt, int *ndomains)
+{
+    if (!bt || !ndomains)
+        return MMCA_ERR_ARG;
+    *ndomains = (int) bt->dom.size();
+    return MMCA_OK;
+}
+
+int mmca_batch_domain(mmca_batch *bt, int i, mmca_ctx **ctx)
+{
+    if (!bt || !ctx || i < 0 || i >= (int) bt->dom.size())
+        return MMCA_ERR_ARG;
+    *ctx = bt->dom[i];
+    return MMCA_OK;
+}
+
+int mmca_batch_set_tuning(mmca_batch *bt, int key, int value)
+{
+    if (!bt)
+        return MMCA_ERR_ARG;
+    switch (key) {
+    case MMCA_TUNE_ZM_PLANES:
+        if (value < 0 || value > 1 << 20)
+            return MMCA_ERR_ARG;
+        bt->zm_planes = value;
+        break;
+    case MMCA_TUNE_ZM_ROWS:
+        if (value != 4 && value != 8 && value != 16)
+            return MMCA_ERR_ARG;
+        bt->zm_by = value;
+        break;
+    case MMCA_TUNE_XCD_REMAP:
+        bt->xcd_remap = value ? 1 : 0;
+        break;
+    case MMCA_TUNE_GRAPH:
+        bt->use_graph = value != 0;
+        return MMCA_OK;
+    default:
+        return MMCA_ERR_ARG;
+    }
+    bt->view_dirty = true;
+    return MMCA_OK;
+}
+
+int mmca_batch_set_kernel_timing(mmca_batch *bt, int enable)
+{
+    if (!bt)
+        return MMCA_ERR_ARG;
+    bt->timing = enable != 0;
+    return MMCA_OK;
+}
+
+int mmca_batch_kernel_time_ms(mmca_batch *bt, double *avg_ms, int64_t *launches)
+{
+    if (!bt)
+        return MMCA_ERR_ARG;
+    if (avg_ms)
+        *avg_ms = bt->time_ms;
+    if (launches)
+        *launches = bt->time_n;
+    return MMCA_OK;
+}
+
+int mmca_batch_step(mmca_batch *bt, int nsteps, int want_sums, mmca_report *reports)
+{
+    if (!bt || nsteps < 0)
+        return MMCA_ERR_ARG;
+    HIPCHK(hipSetDevice(bt->device));
+    TRY(batch_view(bt));
+    TRY(batch_sync(bt));
+    for (mmca_ctx *c : bt->dom) {
+        c->hc.new_grains = 0;
+        HIPCHK(hipMemcpyAsync(&c->b.cnt->new_grains, &c->hc.new_grains, sizeof(int), hipMemcpyHostToDevice,
+                              bt->stream));
+        int st = courant_check(c);
+        if (st != MMCA_OK && nsteps > 0 && c->hc.status == MMCA_OK) {
+            c->hc.status = st;
+            c->hc.failed_step = c->hc.timestep + 1;
+            HIPCHK(hipMemcpyAsync(c->b.cnt, &c->hc, sizeof(Counters), hipMemcpyHostToDevice, bt->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    int done = 0;
+    while (done < nsteps) {
+        // chunk bound: drawn orientations cover every grain the chunk can create
+        long long k = nsteps - done;
+        for (mmca_ctx *c : bt->dom) {
+            if (c->b.nsites == 0 || c->rand_filled >= c->rand_capacity || c->hc.status != MMCA_OK)
+                continue;
+            long long spare = (long long) c->rand_filled - c->hc.random_used;
+            if (spare < (long long) c->b.nsites * 16) {
+                long long want = (long long) c->hc.random_used +
+                                 std::max((long long) c->b.nsites * 256, 2 * spare + 65536);
+                TRY(ensure_random(c, want));
+                spare = (long long) c->rand_filled - c->hc.random_used;
+            }
+            if (c->rand_filled < c->rand_capacity)
+                k = std::min(k, spare / c->b.nsites);
+        }
+        if (k < 1)
+            return MMCA_ERR_STATE;
+        long long n = 0;
+        if (bt->use_graph && !bt->timing)
+            for (; n + 2 <= k; n += 2)
+                TRY(batch_graph_pair(bt, bt->cur));   // two steps: cur returns to itself
+        for (; n < k; n++) {
+            TRY(batch_enqueue(bt, bt->cur));
+            bt->cur ^= 1;
+        }
+        done += (int) k;
+        if (done < nsteps)
+            TRY(batch_sync(bt));
+    }
+    for (mmca_ctx *c : bt->dom) {
+        c->cur = bt->cur;
+        if (want_sums)
+            HIPCHK(launch_reduce(c->b, c->c, c->cur, c->d_red, bt->stream));
+    }
+    TRY(batch_sync(bt));
+    if (bt->timing) {
+        double tot = 0.0;
+        long long n = 0;
+        for (size_t i = 0; i + 1 < bt->ev_used; i += 2) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, bt->ev[i], bt->ev[i + 1]));
+            tot += ms;
+            n++;
+        }
+        bt->ev_used = 0;
+        bt->time_ms = n ? tot / n : 0.0;
+        bt->time_n = n;
+    }
+    int worst = MMCA_OK;
+    for (size_t d = 0; d < bt->dom.size(); d++) {
+        mmca_ctx *c = bt->dom[d];
+        if (c->hc.status != MMCA_OK && worst == MMCA_OK)
+            worst = c->hc.status;
+        if (!reports)
+            continue;
+        mmca_report *r = &reports[d];
+        memset(r, 0, sizeof(*r));
+        r->timestep = c->hc.timestep;
+        r->num_grains = c->hc.num_grains;
+        r->new_grains = c->hc.new_grains;
+        r->gindex = c->hc.gindex;
+        r->status = c->hc.status;
+        r->failed_step = c->hc.failed_step;
+        if (want_sums && c->hc.status == MMCA_OK) {
+            double s2[2];
+            HIPCHK(hipMemcpy(s2, c->d_red, sizeof(s2), hipMemcpyDeviceToHost));
+            r->solid_sum = s2[0];
+            r->solute_sum = s2[1];
+            if (c->c.f32) {
+                r->solute_sum += (double) c->c.nx * c->c.ny * c->c.nz * c->c.cinit;
+                TRY(read_ledger(c, &r->solute_ledger));
+            }
+        }
+    }
+    return worst;
 }
 
 }  // extern "C"

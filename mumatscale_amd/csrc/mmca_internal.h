@@ -100,4 +100,24 @@ struct DevBufs {
     Counters *cnt;
 };
 
+// One domain (subblock) as the hot kernels see it.
+struct DomainDesc {
+    DevBufs b;
+    Consts c;
+};
+
+// The domains advanced by one launch.  Workgroup w of a kernel belongs to
+// domain d when off[d] <= w < off[d+1] (prefix sums of each domain's
+// workgroup count for that kernel).
+struct View {
+    const DomainDesc *dom;       // device array, ndom entries
+    int ndom;
+    int xcd_remap;               // contiguous tile runs per XCD for the solute kernel
+    const long long *off_nuc;    // k_nuc_detect workgroups
+    const long long *off_zm;     // z-marching solute tiles
+    const long long *off_cell;   // cell-parallel solute workgroups
+    const long long *off_list;   // list count / capture workgroups
+    long long tot_nuc, tot_zm, tot_cell, tot_list;
+};
+
 }  // namespace mmca

@@ -220,21 +220,42 @@ block_exscan_int(int v, int *total, int *s_w)
     return off + incl - v;
 }
 
+// Domain of workgroup w: the last d with off[d] <= w (domains without work
+// share their offset with the next domain).
+__device__ __forceinline__ int
+find_dom(const long long *off, int ndom, long long w)
+{
+    int lo = 0, hi = ndom - 1;
+    while (lo < hi)
+    {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= w)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
 // cell_nucleation (grain.c:1222-1318): one thread per site, sites sorted by
 // padded index; each workgroup compacts its events in site order.
 constexpr int NUC_BS = 256;
 
 __global__ void __launch_bounds__(NUC_BS)
-k_nuc_detect(DevBufs b, Consts c, int advance, int cur)
+k_nuc_detect(View V, int advance, int cur)
 {
     __shared__ int s_w[NUC_BS / 64];
+    const int d = find_dom(V.off_nuc, V.ndom, blockIdx.x);
+    const long long blk = blockIdx.x - V.off_nuc[d];
+    const DevBufs &b = V.dom[d].b;
+    const Consts &c = V.dom[d].c;
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
         return;
     const unsigned long long ts = cnt->timestep + (advance ? 1ull : 0ull);
     const double sim_time = ts * c.dt;
-    const int s = blockIdx.x * NUC_BS + threadIdx.x;
+    const int s = (int) blk * NUC_BS + threadIdx.x;
     int flag = 0;
     if (s < b.nsites)
     {
@@ -275,19 +296,22 @@ k_nuc_detect(DevBufs b, Consts c, int advance, int cur)
     int tot;
     int r = block_exscan_flag<NUC_BS / 64>(flag, &tot, s_w);
     if (flag)
-        b.ev_blk[blockIdx.x * NUC_BS + r] = s;
+        b.ev_blk[blk * NUC_BS + r] = s;
     if (threadIdx.x == 0)
-        b.ev_cnt[blockIdx.x] = tot;
+        b.ev_cnt[blk] = tot;
 }
 
 // Event ordering and activateNewGrains/createGrains on one rank
 // (grain.c:841-916, 348-425), one workgroup.
 // mode: 1 = order the detected events, 2 = activate, 4 = advance the timestep.
 __global__ void __launch_bounds__(NUC_BS)
-k_nuc_activate(DevBufs b, Consts c, int mode, int nblk)
+k_nuc_activate(View V, int mode)
 {
     __shared__ int s_w[NUC_BS / 64];
     __shared__ int s_nev;
+    const DevBufs &b = V.dom[blockIdx.x].b;
+    const Consts &c = V.dom[blockIdx.x].c;
+    const int nblk = (b.nsites + NUC_BS - 1) / NUC_BS;
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
         return;
@@ -473,10 +497,18 @@ constexpr int SOL_BY = 4;
 
 template <bool DIFF, bool FSC, bool AM, bool F32>
 __global__ void __launch_bounds__(SOL_BX *SOL_BY)
-k_solute(DevBufs b, Consts c, int cur, int nxt)
+k_solute(View V, int cur, int nxt)
 {
     __shared__ double s_red[SOL_BX * SOL_BY / 64];
     __shared__ int s_stop;
+    const int dm = find_dom(V.off_cell, V.ndom, blockIdx.x);
+    const long long blk = blockIdx.x - V.off_cell[dm];
+    const DevBufs &b = V.dom[dm].b;
+    const Consts &c = V.dom[dm].c;
+    const int gbx = (c.nx + SOL_BX - 1) / SOL_BX, gby = (c.ny + SOL_BY - 1) / SOL_BY;
+    const int bx = (int) (blk % gbx);
+    const int by = (int) ((blk / gbx) % gby);
+    const int bz = (int) (blk / ((long long) gbx * gby));
     Counters *cnt = b.cnt;
     // this kernel itself may raise the status: read it once per workgroup so
     // that every wave takes the same branch before the barrier of block_sum
@@ -485,9 +517,9 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
     __syncthreads();
     if (s_stop)
         return;
-    const int x = blockIdx.x * SOL_BX + threadIdx.x + 1;   // interior x
-    const int j = blockIdx.y * SOL_BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    const int x = bx * SOL_BX + threadIdx.x + 1;   // interior x
+    const int j = by * SOL_BY + threadIdx.y + 1;
+    const int k = bz + 1;
     const bool rowok = (j <= c.ny);
     const bool active = rowok && (x <= c.nx);
     int g_new = 0;
@@ -538,14 +570,13 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
     {
         unsigned long long bits = __ballot(active && g_new > 0);
         if (threadIdx.x == 0 && rowok)
-            b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + blockIdx.x] = bits;
+            b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
     }
     if (F32)
     {
         const double tot = block_sum<SOL_BX * SOL_BY>(lost, s_red);
         if (threadIdx.x == 0 && threadIdx.y == 0 && tot != 0.0)
-            ledger_add(b, blockIdx.x + (long long) gridDim.x * (blockIdx.y + (long long) gridDim.y * blockIdx.z),
-                       tot);
+            ledger_add(b, blk, tot);
     }
 }
 
@@ -556,12 +587,29 @@ k_solute(DevBufs b, Consts c, int cur, int nxt)
 // only tile edges load neighbours from memory.
 template <int BY, bool AM, bool F32>
 __global__ void __launch_bounds__(64 * BY)
-k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
+k_solute_zm(View V, int cur, int nxt)
 {
     __shared__ double s_cl[BY][64];
     __shared__ double s_fs[BY][64];
     __shared__ double s_red[BY];
     __shared__ int s_stop;
+    // Tile of this workgroup.  With xcd_remap the workgroups that the
+    // hardware hands to one XCD (linear id mod 8) take a contiguous run of
+    // tiles (x fastest, then y, then z, then domain), so that tiles sharing
+    // halo rows and planes run on the same L2.
+    long long gtile = blockIdx.x;
+    if (V.xcd_remap)
+    {
+        const long long per = (V.tot_zm + 7) / 8;
+        gtile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    }
+    if (gtile >= V.tot_zm)
+        return;                                   // uniform over the workgroup
+    const int dm = find_dom(V.off_zm, V.ndom, gtile);
+    const long long tile = gtile - V.off_zm[dm];
+    const DevBufs &b = V.dom[dm].b;
+    const Consts &c = V.dom[dm].c;
+    const int kz = c.zm_planes;
     Counters *cnt = b.cnt;
     // this kernel itself may raise the status: read it once per workgroup so
     // that every wave takes the same branch before the barriers below
@@ -572,20 +620,7 @@ k_solute_zm(DevBufs b, Consts c, int cur, int nxt, int kz)
         return;
     const int lane = threadIdx.x;
     const int ty = threadIdx.y;
-    // Tile of this workgroup.  With xcd_remap the workgroups that the
-    // hardware hands to one XCD (linear id mod 8) take a contiguous run of
-    // tiles (x fastest, then y, then z), so that tiles sharing halo rows and
-    // planes run on the same L2.
-    const int ntx = (c.nx + 63) / 64, nty = (c.ny + BY - 1) / BY, ntz = (c.nz + kz - 1) / kz;
-    const long long ntiles = (long long) ntx * nty * ntz;
-    long long tile = blockIdx.x;
-    if (c.xcd_remap)
-    {
-        const long long per = (ntiles + 7) / 8;
-        tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    }
-    if (tile >= ntiles)
-        return;                                   // uniform over the workgroup
+    const int ntx = (c.nx + 63) / 64, nty = (c.ny + BY - 1) / BY;
     const int bx = (int) (tile % ntx);
     const int by = (int) ((tile / ntx) % nty);
     const int bz = (int) (tile / ((long long) ntx * nty));
@@ -728,6 +763,12 @@ k_mask_from_gr(DevBufs b, Consts c)
 // (growth.c:496-513), in raster order.  One thread per 64-bit mask word.
 constexpr int LIST_BS = 256;
 
+__host__ __device__ __forceinline__ long long
+list_words_d(const Consts &c)
+{
+    return (long long) c.nz * c.ny * c.w64;
+}
+
 __device__ __forceinline__ unsigned long long
 mask_word(const DevBufs &b, const Consts &c, int k, int j, int w)
 {
@@ -760,12 +801,17 @@ member_bits(const DevBufs &b, const Consts &c, long long wi, int *pk, int *pj, i
 }
 
 __global__ void __launch_bounds__(LIST_BS)
-k_list_count(DevBufs b, Consts c, long long nwords)
+k_list_count(View V)
 {
     __shared__ int s_w[LIST_BS / 64];
+    const int dm = find_dom(V.off_list, V.ndom, blockIdx.x);
+    const long long blk = blockIdx.x - V.off_list[dm];
+    const DevBufs &b = V.dom[dm].b;
+    const Consts &c = V.dom[dm].c;
+    const long long nwords = list_words_d(c);
     if (b.cnt->status != MMCA_OK)
         return;
-    const long long wi = (long long) blockIdx.x * LIST_BS + threadIdx.x;
+    const long long wi = blk * LIST_BS + threadIdx.x;
     int v = 0;
     if (wi < nwords)
     {
@@ -775,7 +821,7 @@ k_list_count(DevBufs b, Consts c, long long nwords)
     int tot;
     block_exscan_int<LIST_BS / 64>(v, &tot, s_w);
     if (threadIdx.x == 0)
-        b.blk_count[blockIdx.x] = tot;
+        b.blk_count[blk] = tot;
 }
 
 // Exclusive scan of the block counts (one workgroup).  When decide_realloc
@@ -783,9 +829,12 @@ k_list_count(DevBufs b, Consts c, long long nwords)
 constexpr int SCAN_BS = 1024;
 
 __global__ void __launch_bounds__(SCAN_BS)
-k_list_scan(DevBufs b, int nblk, int decide_realloc)
+k_list_scan(View V, int decide_realloc)
 {
     __shared__ int s_w[SCAN_BS / 64];
+    const DevBufs &b = V.dom[blockIdx.x].b;
+    const Consts &c = V.dom[blockIdx.x].c;
+    const int nblk = (int) ((list_words_d(c) + LIST_BS - 1) / LIST_BS);
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
         return;
@@ -868,15 +917,20 @@ nth_set_bit(unsigned long long m, int r)
 }
 
 __global__ void __launch_bounds__(LIST_BS)
-k_capture(DevBufs b, Consts c, long long nwords, int cb, int do_capture)
+k_capture(View V, int cb, int do_capture)
 {
     __shared__ int s_w[LIST_BS / 64];
     __shared__ unsigned long long s_mem[LIST_BS];
     __shared__ int s_ex[LIST_BS];
+    const int dm = find_dom(V.off_list, V.ndom, blockIdx.x);
+    const long long blk = blockIdx.x - V.off_list[dm];
+    const DevBufs &b = V.dom[dm].b;
+    const Consts &c = V.dom[dm].c;
+    const long long nwords = list_words_d(c);
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
         return;
-    const long long wi0 = (long long) blockIdx.x * LIST_BS;
+    const long long wi0 = blk * LIST_BS;
     const long long wi = wi0 + threadIdx.x;
     unsigned long long mem = 0;
     int k0, j0, w0;
@@ -887,7 +941,7 @@ k_capture(DevBufs b, Consts c, long long nwords, int cb, int do_capture)
     s_mem[threadIdx.x] = mem;
     s_ex[threadIdx.x] = ex;
     __syncthreads();
-    const int base = b.blk_count[blockIdx.x];     // members before this workgroup
+    const int base = b.blk_count[blk];            // members before this workgroup
     const bool realloc = do_capture && cnt->realloc_old >= 0;
     const long long maxg = c.max_total_grains;
     double *fsb = b.fs[cb];
@@ -1133,67 +1187,59 @@ k_grow_octahedra(DevBufs b, Consts c, int cb)
 
 // ---------------------------------------------------------------------------
 // Launch wrappers (host side).
-static inline dim3 sol_grid(const Consts &c)
+hipError_t launch_nucleate(const View &v, int mode, int cur, hipStream_t s)
 {
-    return dim3((c.nx + SOL_BX - 1) / SOL_BX, (c.ny + SOL_BY - 1) / SOL_BY, c.nz);
-}
-
-hipError_t launch_nucleate(const DevBufs &b, const Consts &c, int mode, int cur, hipStream_t s)
-{
-    const int nblk = (b.nsites + NUC_BS - 1) / NUC_BS;
-    if ((mode & 1) && nblk > 0)
-        hipLaunchKernelGGL(k_nuc_detect, dim3(nblk), dim3(NUC_BS), 0, s, b, c, (mode & 4) ? 1 : 0, cur);
-    hipLaunchKernelGGL(k_nuc_activate, dim3(1), dim3(NUC_BS), 0, s, b, c, mode, nblk);
+    if ((mode & 1) && v.tot_nuc > 0)
+        hipLaunchKernelGGL(k_nuc_detect, dim3((unsigned) v.tot_nuc), dim3(NUC_BS), 0, s, v,
+                           (mode & 4) ? 1 : 0, cur);
+    hipLaunchKernelGGL(k_nuc_activate, dim3(v.ndom), dim3(NUC_BS), 0, s, v, mode);
     return hipGetLastError();
 }
 
+// zm_by: rows of the z-marching kernel (all domains of a view share it); 0
+// selects the cell-parallel kernel.
 template <bool AM, bool F32>
-static void launch_solute_t(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
-                            hipStream_t s)
+static void launch_solute_t(const View &v, int zm_by, int diff, int fsc, int cur, int nxt, hipStream_t s)
 {
     dim3 blk(SOL_BX, SOL_BY);
-    if (diff && fsc && c.zm_planes > 0)
+    if (diff && fsc && zm_by > 0)
     {
-        const int by = (c.zm_by == 8 || c.zm_by == 16) ? c.zm_by : 4;
-        const long long nt = (long long) ((c.nx + 63) / 64) * ((c.ny + by - 1) / by) *
-                             ((c.nz + c.zm_planes - 1) / c.zm_planes);
-        const long long ng = c.xcd_remap ? ((nt + 7) / 8) * 8 : nt;
-        if (by == 16)
-            hipLaunchKernelGGL((k_solute_zm<16, AM, F32>), dim3((unsigned) ng), dim3(64, 16), 0, s, b, c,
-                               cur, nxt, c.zm_planes);
-        else if (by == 8)
-            hipLaunchKernelGGL((k_solute_zm<8, AM, F32>), dim3((unsigned) ng), dim3(64, 8), 0, s, b, c,
-                               cur, nxt, c.zm_planes);
+        const long long ng = v.xcd_remap ? ((v.tot_zm + 7) / 8) * 8 : v.tot_zm;
+        if (zm_by == 16)
+            hipLaunchKernelGGL((k_solute_zm<16, AM, F32>), dim3((unsigned) ng), dim3(64, 16), 0, s, v, cur, nxt);
+        else if (zm_by == 8)
+            hipLaunchKernelGGL((k_solute_zm<8, AM, F32>), dim3((unsigned) ng), dim3(64, 8), 0, s, v, cur, nxt);
         else
-            hipLaunchKernelGGL((k_solute_zm<4, AM, F32>), dim3((unsigned) ng), dim3(64, 4), 0, s, b, c,
-                               cur, nxt, c.zm_planes);
+            hipLaunchKernelGGL((k_solute_zm<4, AM, F32>), dim3((unsigned) ng), dim3(64, 4), 0, s, v, cur, nxt);
     }
     else if (diff && fsc)
-        hipLaunchKernelGGL((k_solute<true, true, AM, F32>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<true, true, AM, F32>), dim3((unsigned) v.tot_cell), blk, 0, s, v, cur, nxt);
     else if (F32)
         return;     // single operators run on FP64 storage only (the caller checks)
     else if (diff)
-        hipLaunchKernelGGL((k_solute<true, false, AM, false>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<true, false, AM, false>), dim3((unsigned) v.tot_cell), blk, 0, s, v, cur,
+                           nxt);
     else
-        hipLaunchKernelGGL((k_solute<false, true, AM, false>), sol_grid(c), blk, 0, s, b, c, cur, nxt);
+        hipLaunchKernelGGL((k_solute<false, true, AM, false>), dim3((unsigned) v.tot_cell), blk, 0, s, v, cur,
+                           nxt);
 }
 
-hipError_t launch_solute(const DevBufs &b, const Consts &c, int diff, int fsc, int cur, int nxt,
+hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, int fsc, int cur, int nxt,
                          hipStream_t s)
 {
-    if (c.am)
+    if (am)
     {
-        if (c.f32)
-            launch_solute_t<true, true>(b, c, diff, fsc, cur, nxt, s);
+        if (f32)
+            launch_solute_t<true, true>(v, zm_by, diff, fsc, cur, nxt, s);
         else
-            launch_solute_t<true, false>(b, c, diff, fsc, cur, nxt, s);
+            launch_solute_t<true, false>(v, zm_by, diff, fsc, cur, nxt, s);
     }
     else
     {
-        if (c.f32)
-            launch_solute_t<false, true>(b, c, diff, fsc, cur, nxt, s);
+        if (f32)
+            launch_solute_t<false, true>(v, zm_by, diff, fsc, cur, nxt, s);
         else
-            launch_solute_t<false, false>(b, c, diff, fsc, cur, nxt, s);
+            launch_solute_t<false, false>(v, zm_by, diff, fsc, cur, nxt, s);
     }
     return hipGetLastError();
 }
@@ -1204,27 +1250,33 @@ hipError_t launch_mask_from_gr(const DevBufs &b, const Consts &c, hipStream_t s)
     return hipGetLastError();
 }
 
-long long list_words(const Consts &c) { return (long long) c.nz * c.ny * c.w64; }
+long long list_words(const Consts &c) { return list_words_d(c); }
 int list_blocks(const Consts &c) { return (int) ((list_words(c) + LIST_BS - 1) / LIST_BS); }
-
-hipError_t launch_list(const DevBufs &b, const Consts &c, int cb, int capture, int decide_realloc,
-                       hipStream_t s)
+int nuc_blocks(int nsites) { return (nsites + NUC_BS - 1) / NUC_BS; }
+long long cell_blocks(const Consts &c)
 {
-    long long nw = list_words(c);
-    int nblk = list_blocks(c);
-    hipLaunchKernelGGL(k_list_count, dim3(nblk), dim3(LIST_BS), 0, s, b, c, nw);
-    hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(SCAN_BS), 0, s, b, nblk, decide_realloc);
-    if (capture >= 0)
-        hipLaunchKernelGGL(k_capture, dim3(nblk), dim3(LIST_BS), 0, s, b, c, nw, cb, capture);
+    return (long long) ((c.nx + SOL_BX - 1) / SOL_BX) * ((c.ny + SOL_BY - 1) / SOL_BY) * c.nz;
+}
+long long zm_tiles(const Consts &c, int by)
+{
+    return (long long) ((c.nx + 63) / 64) * ((c.ny + by - 1) / by) * ((c.nz + c.zm_planes - 1) / c.zm_planes);
+}
+
+hipError_t launch_list(const View &v, int cb, int capture, int decide_realloc, hipStream_t s)
+{
+    if (v.tot_list > 0)
+        hipLaunchKernelGGL(k_list_count, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v);
+    hipLaunchKernelGGL(k_list_scan, dim3(v.ndom), dim3(SCAN_BS), 0, s, v, decide_realloc);
+    if (capture >= 0 && v.tot_list > 0)
+        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb, capture);
     return hipGetLastError();
 }
 
-hipError_t launch_capture_only(const DevBufs &b, const Consts &c, int cb, hipStream_t s)
+hipError_t launch_capture_only(const View &v, const DevBufs &b, int cb, hipStream_t s)
 {
-    long long nw = list_words(c);
-    int nblk = list_blocks(c);
     hipLaunchKernelGGL(k_realloc_decide, dim3(1), dim3(1), 0, s, b);
-    hipLaunchKernelGGL(k_capture, dim3(nblk), dim3(LIST_BS), 0, s, b, c, nw, cb, 1);
+    if (v.tot_list > 0)
+        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb, 1);
     return hipGetLastError();
 }
 

@@ -55,6 +55,15 @@ def lib():
     L.mmca_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.mmca_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.mmca_get_tuning.argtypes = [vp, C.c_int, i32p]
+    L.mmca_batch_create.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg), C.c_int, C.c_int,
+                                    C.POINTER(vp)]
+    L.mmca_batch_destroy.argtypes = [vp]
+    L.mmca_batch_size.argtypes = [vp, i32p]
+    L.mmca_batch_domain.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.mmca_batch_step.argtypes = [vp, C.c_int, C.c_int, C.POINTER(abi.Report)]
+    L.mmca_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.mmca_batch_set_kernel_timing.argtypes = [vp, C.c_int]
+    L.mmca_batch_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.mmca_nucleation_sites.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg),
                                         i32p, C.POINTER(C.c_float), i32p, C.c_int64,
                                         C.POINTER(C.c_int64)]
@@ -261,4 +270,77 @@ class Subblock:
         ms = C.c_double(0)
         n = C.c_int64(0)
         _check(lib().mmca_kernel_time_ms(self._h, C.byref(ms), C.byref(n)), "mmca_kernel_time_ms")
+        return ms.value, n.value
+
+
+class _Member(Subblock):
+    """A domain owned by a Batch (not destroyed on its own)."""
+
+    def __init__(self, handle, params):
+        self.params = params
+        self.shape = (params.nz + 2, params.ny + 2, params.nx + 2)
+        self._h = handle
+
+    def close(self):
+        self._h = C.c_void_p()
+
+
+class Batch:
+    """Independent subblocks of any sizes advanced together (one set of
+    launches per step for all of them)."""
+
+    def __init__(self, params_list, nucleation_list=None, device=0):
+        n = len(params_list)
+        P = (abi.Params * n)(*params_list)
+        N = None
+        if nucleation_list is not None:
+            N = (abi.NucleationCfg * n)(*nucleation_list)
+            self._keep = [getattr(c, "_keep", None) for c in nucleation_list]
+        h = C.c_void_p()
+        st = lib().mmca_batch_create(P, N, n, device, C.byref(h))
+        self._h = h
+        if st != abi.OK:
+            self.close()
+            raise abi.MmcaError(st, "mmca_batch_create")
+        self.params = list(params_list)
+        self.domains = []
+        for i in range(n):
+            hd = C.c_void_p()
+            _check(lib().mmca_batch_domain(self._h, i, C.byref(hd)), "mmca_batch_domain")
+            self.domains.append(_Member(hd, self.params[i]))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().mmca_batch_destroy(self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, n=1, want_sums=False, raise_on_error=True):
+        reps = (abi.Report * len(self.domains))()
+        st = lib().mmca_batch_step(self._h, n, 1 if want_sums else 0, reps)
+        if raise_on_error:
+            _check(st, "mmca_batch_step")
+        return list(reps)
+
+    def set_tuning(self, key, value):
+        _check(lib().mmca_batch_set_tuning(self._h, key, value), "mmca_batch_set_tuning")
+
+    def set_kernel_timing(self, enable=True):
+        _check(lib().mmca_batch_set_kernel_timing(self._h, 1 if enable else 0), "mmca_batch_set_kernel_timing")
+
+    def kernel_time_ms(self):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        _check(lib().mmca_batch_kernel_time_ms(self._h, C.byref(ms), C.byref(n)), "mmca_batch_kernel_time_ms")
         return ms.value, n.value

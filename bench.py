@@ -68,6 +68,108 @@ def directional_config(nx=256, ny=256, nz=512):
     return c
 
 
+def ensemble_configs(n=96, seed=3):
+    """96 independent domains: sides drawn from {32,48,64,96,128} per axis,
+    cooling rate log-uniform in [1, 1000] K/s, initial undercooling uniform
+    in [1, 5] K, bulk nucleation 1e14 m^-3 (mean 3 K, sigma 0.5 K) with a
+    distinct seed per domain; alloy and dt of the single-grain physics."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([32, 48, 64, 96, 128], size=(n, 3))
+    rates = 10.0 ** rng.uniform(0.0, 3.0, size=n)
+    under = rng.uniform(1.0, 5.0, size=n)
+    out = []
+    for i in range(n):
+        c = directional_config(*(int(v) for v in sides[i]))
+        c.gradient = 0.0
+        c.velocity = 0.0
+        c.cool_rate = float(rates[i])
+        c.t_init = 933.0 - float(under[i])
+        c.surf_density = 0.0
+        c.bulk_density = 1.0e14
+        c.bulk_mean = 3.0
+        c.bulk_sigma = 0.5
+        c.random_seed = 1000 + i
+        c.max_total_grains = 100000
+        out.append(c)
+    return out
+
+
+def run_ensemble(steps, baseline_steps):
+    """Batched FP32-deviation step of the ensemble against the same domains
+    stepped one at a time in plain FP64."""
+    from mumatscale_amd.lib import Batch, Subblock, autotune_batch
+    cfgs = ensemble_configs()
+    cells = sum(c.sbdim[0] * c.sbdim[1] * c.sbdim[2] for c in cfgs)
+    p32 = [c.params(check_negative_ce=False, precision=abi.PREC_FP32DEV) for c in cfgs]
+    nuc = [c.nucleation() for c in cfgs]
+    t0 = time.time()
+    planes, rows, tms = autotune_batch(p32, nuc, warmup=10, trial_steps=20)
+    log("ensemble: autotuner picked planes=%d rows=%d (%.3f ms) in %.1f s" % (planes, rows, tms, time.time() - t0))
+    with Batch(p32, nuc) as bt:
+        bt.set_tuning(abi.TUNE_ZM_PLANES, planes)
+        bt.set_tuning(abi.TUNE_ZM_ROWS, rows)
+        bt.set_tuning(abi.TUNE_GRAPH, 1)
+        bt.step(10)
+        t0 = time.perf_counter()
+        reps = bt.step(steps, want_sums=True)
+        wall = time.perf_counter() - t0
+        drift = max(abs(r.solute_sum + r.solute_ledger - (c.sbdim[0] * c.sbdim[1] * c.sbdim[2]) * c.c_init)
+                    / ((c.sbdim[0] * c.sbdim[1] * c.sbdim[2]) * c.c_init) for r, c in zip(reps, cfgs))
+    batched = cells * steps / wall / 1e6
+    # the same domains one at a time, plain FP64
+    t_one = 0.0
+    for c in cfgs:
+        with Subblock(c.params(check_negative_ce=False), c.nucleation()) as sb:
+            sb.step(2)
+            t0 = time.perf_counter()
+            sb.step(baseline_steps)
+            t_one += time.perf_counter() - t0
+    one = cells * baseline_steps / t_one / 1e6
+    return {"domains": len(cfgs), "cells": cells, "steps": steps, "dtype": "f32-deviation storage, f64 arithmetic",
+            "value": round(batched, 1), "unit": "Mcell-updates/s", "ms_per_step": round(wall / steps * 1e3, 4),
+            "tile": {"zm_planes": planes, "zm_rows": rows}, "hip_graph": True,
+            "one_at_a_time_fp64": round(one, 1), "one_at_a_time_steps": baseline_steps,
+            "speedup_vs_one_at_a_time_fp64": round(batched / one, 2),
+            "max_solute_drift_with_ledger": drift}
+
+
+def run_fp32_variant(cfg, steps, warmup, zm, rows, xcd):
+    """The headline workload with FP32-deviation storage; errors against the
+    FP64 run after the same steps."""
+    import numpy as np
+    from mumatscale_amd.lib import Subblock
+    n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
+    res = {}
+    fields = {}
+    for prec in (abi.PREC_FP32DEV, abi.PREC_FP64):
+        with Subblock(cfg.params(check_negative_ce=False, precision=prec), cfg.nucleation()) as sb:
+            sb.set_tuning(abi.TUNE_ZM_PLANES, zm)
+            sb.set_tuning(abi.TUNE_ZM_ROWS, rows)
+            sb.set_tuning(abi.TUNE_XCD_REMAP, xcd)
+            sb.step(warmup)
+            sb.set_kernel_timing(prec == abi.PREC_FP32DEV)
+            t0 = time.perf_counter()
+            rep = sb.step(steps, want_sums=True)
+            wall = time.perf_counter() - t0
+            k_ms, _ = sb.kernel_time_ms()
+            fields[prec] = (sb.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1], rep)
+            if prec == abi.PREC_FP32DEV:
+                res = {"value": round(n * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+                       "k_solute_ms": round(k_ms, 4),
+                       "roofline_frac": round(n * 40.0 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "bytes_per_cell": 40.0}
+    ce32, r32 = fields[abi.PREC_FP32DEV]
+    ce64, r64 = fields[abi.PREC_FP64]
+    res["error_vs_fp64"] = {
+        "solute_l2_rel": float(np.linalg.norm(ce32 - ce64) / np.linalg.norm(ce64)),
+        "solid_fraction_abs": abs(r32.solid_sum - r64.solid_sum) / n,
+        "grains_fp32": r32.num_grains - 1, "grains_fp64": r64.num_grains - 1,
+        "solute_drift_stored": (r32.solute_sum - n * cfg.c_init) / (n * cfg.c_init),
+        "solute_drift_with_ledger": (r32.solute_sum + r32.solute_ledger - n * cfg.c_init) / (n * cfg.c_init)}
+    return res
+
+
 def cpu_baseline(cfg, target_s):
     """Oracle (scalar C port of the reference path), one thread, bounded sample."""
     from oracle.oracle import OracleSubblock
@@ -101,6 +203,9 @@ def main():
                     help="z-marching solute kernel, planes per workgroup (0: cell-parallel)")
     ap.add_argument("--zm-rows", type=int, default=8, help="rows per z-marching workgroup (4, 8, 16)")
     ap.add_argument("--xcd", type=int, default=1, help="1: contiguous tile runs per XCD")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the FP32-deviation and batched-ensemble measurements")
+    ap.add_argument("--extra-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
@@ -137,6 +242,12 @@ def main():
             "kernel": ("k_solute_zm<%d>" % args.zm_rows) if args.zm > 0 else "k_solute<true,true>",
             "bytes_per_cell": B_ALG_FP64}
     gpu.close()
+    extra = {}
+    if not args.no_extra:
+        extra["fp32dev"] = run_fp32_variant(cfg, args.extra_steps, args.warmup, args.zm, args.zm_rows, args.xcd)
+        log("fp32dev:", extra["fp32dev"])
+        extra["batched_ensemble"] = run_ensemble(3000, 200)
+        log("ensemble:", extra["batched_ensemble"])
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, args.cpu_seconds)
@@ -160,6 +271,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

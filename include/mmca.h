@@ -260,6 +260,13 @@ int  mmca_batch_step(mmca_batch *batch, int nsteps, int want_sums, mmca_report *
 int  mmca_batch_set_tuning(mmca_batch *batch, int key, int value);
 int  mmca_batch_set_kernel_timing(mmca_batch *batch, int enable);
 int  mmca_batch_kernel_time_ms(mmca_batch *batch, double *avg_ms, int64_t *launches);
+/* Autotuner of the solute-kernel shape for a batch: builds a scratch batch
+ * from the same inputs, runs `warmup` steps, then times `trial_steps` steps
+ * of each candidate shape and returns the fastest (results do not depend on
+ * the shape). */
+int  mmca_batch_autotune(const mmca_params *params, const mmca_nucleation_cfg *nucleation, int ndomains,
+                         int device, int warmup, int trial_steps, int *best_planes, int *best_rows,
+                         double *best_ms);
 
 #ifdef __cplusplus
 }

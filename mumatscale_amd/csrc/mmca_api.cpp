@@ -1331,4 +1331,50 @@ int mmca_batch_step(mmca_batch *bt, int nsteps, int want_sums, mmca_report *repo
     return worst;
 }
 
+int mmca_batch_autotune(const mmca_params *params, const mmca_nucleation_cfg *nucleation, int ndomains,
+                        int device, int warmup, int trial_steps, int *best_planes, int *best_rows,
+                        double *best_ms)
+{
+    if (!params || ndomains < 1 || trial_steps < 1 || warmup < 0)
+        return MMCA_ERR_ARG;
+    // solute-kernel shapes tried: (planes per workgroup, rows); planes 0 is
+    // the cell-parallel kernel
+    static const int cand[][2] = { { 0, 4 }, { 8, 4 }, { 16, 4 }, { 8, 8 }, { 16, 8 }, { 32, 8 },
+                                   { 64, 8 }, { 16, 16 }, { 32, 16 }, { 64, 16 } };
+    mmca_batch *bt = nullptr;
+    int st = mmca_batch_create(params, nucleation, ndomains, device, &bt);
+    if (st != MMCA_OK) {
+        mmca_batch_destroy(bt);
+        return st;
+    }
+    int bp = 16, br = 8;
+    double bms = 1e300;
+    if (warmup > 0)
+        st = mmca_batch_step(bt, warmup, 0, nullptr);
+    for (const auto &cd : cand) {
+        if (st != MMCA_OK)
+            break;
+        mmca_batch_set_tuning(bt, MMCA_TUNE_ZM_PLANES, cd[0]);
+        mmca_batch_set_tuning(bt, MMCA_TUNE_ZM_ROWS, cd[1]);
+        mmca_batch_set_kernel_timing(bt, 1);
+        st = mmca_batch_step(bt, trial_steps, 0, nullptr);
+        double ms = 0;
+        mmca_batch_kernel_time_ms(bt, &ms, nullptr);
+        if (st == MMCA_OK && ms > 0 && ms < bms) {
+            bms = ms;
+            bp = cd[0];
+            br = cd[1];
+        }
+    }
+    mmca_batch_destroy(bt);
+    if (best_planes)
+        *best_planes = bp;
+    if (best_rows)
+        *best_rows = br;
+    if (best_ms)
+        *best_ms = bms;
+    // a failing domain (e.g. a negative ce) still leaves valid timings
+    return (bms < 1e300) ? MMCA_OK : st;
+}
+
 }  // extern "C"

@@ -64,6 +64,8 @@ def lib():
     L.mmca_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.mmca_batch_set_kernel_timing.argtypes = [vp, C.c_int]
     L.mmca_batch_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.mmca_batch_autotune.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg), C.c_int, C.c_int,
+                                      C.c_int, C.c_int, i32p, i32p, C.POINTER(C.c_double)]
     L.mmca_nucleation_sites.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg),
                                         i32p, C.POINTER(C.c_float), i32p, C.c_int64,
                                         C.POINTER(C.c_int64)]
@@ -344,3 +346,16 @@ class Batch:
         n = C.c_int64(0)
         _check(lib().mmca_batch_kernel_time_ms(self._h, C.byref(ms), C.byref(n)), "mmca_batch_kernel_time_ms")
         return ms.value, n.value
+
+
+def autotune_batch(params_list, nucleation_list=None, device=0, warmup=20, trial_steps=30):
+    """Fastest solute-kernel shape (planes, rows, kernel ms) for these domains."""
+    n = len(params_list)
+    P = (abi.Params * n)(*params_list)
+    N = (abi.NucleationCfg * n)(*nucleation_list) if nucleation_list is not None else None
+    zp = C.c_int32(0)
+    zr = C.c_int32(0)
+    ms = C.c_double(0)
+    _check(lib().mmca_batch_autotune(P, N, n, device, warmup, trial_steps, C.byref(zp), C.byref(zr),
+                                     C.byref(ms)), "mmca_batch_autotune")
+    return zp.value, zr.value, ms.value

@@ -138,8 +138,8 @@ int build_consts(const mmca_params &p, Consts &c)
     c.sxy = (long long) (p.nx + 2) * (p.ny + 2);
     c.P = padded_cells(p);
     c.w64 = (c.nx + 63) / 64;          // mask words per row (interior x)
-    c.zm_planes = 16;                   // z-marching solute kernel by default
-    c.zm_by = 8;
+    c.zm_planes = 64;                   // z-marching solute kernel by default
+    c.zm_by = 16;
     c.xcd_remap = 1;
     if (p.precision != MMCA_PREC_FP64 && p.precision != MMCA_PREC_FP32DEV)
         return MMCA_ERR_ARG;

@@ -26,6 +26,12 @@ namespace mmca {
 // MIN(x,y) of globals.h:19
 __device__ __forceinline__ double ref_min(double x, double y) { return (x < y) ? x : y; }
 
+// Ping-pong buffer `buf` (0/1) of a field, chosen with a select rather than an
+// indexed load, so that a private copy of DevBufs stays in registers.
+__device__ __forceinline__ double *fs_buf(const DevBufs &b, int buf) { return buf ? b.fs[1] : b.fs[0]; }
+__device__ __forceinline__ double *cl_buf(const DevBufs &b, int buf) { return buf ? b.cl[1] : b.cl[0]; }
+__device__ __forceinline__ float *cl32_buf(const DevBufs &b, int buf) { return buf ? b.cl32[1] : b.cl32[0]; }
+
 // Solute storage: FP64 values, or FP32 deviations from c0 (F32).  Arithmetic
 // is FP64 in both cases.
 template <bool F32>
@@ -39,7 +45,7 @@ template <bool F32>
 __device__ __forceinline__ double
 ld_cl(const DevBufs &b, const Consts &c, int buf, long long i)
 {
-    return F32 ? c.cinit + (double) b.cl32[buf][i] : b.cl[buf][i];
+    return F32 ? c.cinit + (double) cl32_buf(b, buf)[i] : cl_buf(b, buf)[i];
 }
 
 // store ce; returns the FP64 solute lost to the FP32 rounding (0 for FP64)
@@ -63,9 +69,9 @@ __device__ __forceinline__ void
 st_cl(const DevBufs &b, const Consts &c, int buf, long long i, double v)
 {
     if (F32)
-        b.cl32[buf][i] = (float) (v - c.cinit);
+        cl32_buf(b, buf)[i] = (float) (v - c.cinit);
     else
-        b.cl[buf][i] = v;
+        cl_buf(b, buf)[i] = v;
 }
 
 // Compensated (Kahan) addition of a workgroup's residual into its ledger slot.
@@ -248,8 +254,8 @@ k_nuc_detect(View V, int advance, int cur)
     __shared__ int s_w[NUC_BS / 64];
     const int d = find_dom(V.off_nuc, V.ndom, blockIdx.x);
     const long long blk = blockIdx.x - V.off_nuc[d];
-    const DevBufs &b = V.dom[d].b;
-    const Consts &c = V.dom[d].c;
+    const DevBufs b = V.dom[d].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[d].c;
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
         return;
@@ -309,8 +315,8 @@ k_nuc_activate(View V, int mode)
 {
     __shared__ int s_w[NUC_BS / 64];
     __shared__ int s_nev;
-    const DevBufs &b = V.dom[blockIdx.x].b;
-    const Consts &c = V.dom[blockIdx.x].c;
+    const DevBufs b = V.dom[blockIdx.x].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[blockIdx.x].c;
     const int nblk = (b.nsites + NUC_BS - 1) / NUC_BS;
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
@@ -503,8 +509,8 @@ k_solute(View V, int cur, int nxt)
     __shared__ int s_stop;
     const int dm = find_dom(V.off_cell, V.ndom, blockIdx.x);
     const long long blk = blockIdx.x - V.off_cell[dm];
-    const DevBufs &b = V.dom[dm].b;
-    const Consts &c = V.dom[dm].c;
+    const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[dm].c;
     const int gbx = (c.nx + SOL_BX - 1) / SOL_BX, gby = (c.ny + SOL_BY - 1) / SOL_BY;
     const int bx = (int) (blk % gbx);
     const int by = (int) ((blk / gbx) % gby);
@@ -527,7 +533,7 @@ k_solute(View V, int cur, int nxt)
     if (active)
     {
         const long long idx = k * c.sxy + (long long) j * c.sx + x;
-        const double *fsc = b.fs[cur];
+        const double *fsc = fs_buf(b, cur);
         const double ce_old = ld_ce<F32>(b, c, idx);
         const double conc = ld_cl<F32>(b, c, cur, idx);
         const double lspfs = fsc[idx];
@@ -560,7 +566,7 @@ k_solute(View V, int cur, int nxt)
             partition_cell<AM>(c, cnt->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
                                &cl_out, &fs_out, &g);
             st_cl<F32>(b, c, nxt, idx, cl_out);
-            b.fs[nxt][idx] = fs_out;
+            fs_buf(b, nxt)[idx] = fs_out;
             if (g != g_old)
                 b.gr[idx] = g;
             g_new = g;
@@ -607,8 +613,8 @@ k_solute_zm(View V, int cur, int nxt)
         return;                                   // uniform over the workgroup
     const int dm = find_dom(V.off_zm, V.ndom, gtile);
     const long long tile = gtile - V.off_zm[dm];
-    const DevBufs &b = V.dom[dm].b;
-    const Consts &c = V.dom[dm].c;
+    const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[dm].c;
     const int kz = c.zm_planes;
     Counters *cnt = b.cnt;
     // this kernel itself may raise the status: read it once per workgroup so
@@ -630,8 +636,8 @@ k_solute_zm(View V, int cur, int nxt)
     const int k1 = min(c.nz, k0 + kz - 1);
     const bool rowok = (j <= c.ny);
     const bool active = rowok && (x <= c.nx);
-    const double *fsc = b.fs[cur];
-    double *fsn_out = b.fs[nxt];
+    const double *fsc = fs_buf(b, cur);
+    double *fsn_out = fs_buf(b, nxt);
     const long long col = active ? ((long long) j * c.sx + x) : 0;
     const double sim_time = cnt->timestep * c.dt;
     const bool ylo_lds = (ty > 0);
@@ -806,8 +812,8 @@ k_list_count(View V)
     __shared__ int s_w[LIST_BS / 64];
     const int dm = find_dom(V.off_list, V.ndom, blockIdx.x);
     const long long blk = blockIdx.x - V.off_list[dm];
-    const DevBufs &b = V.dom[dm].b;
-    const Consts &c = V.dom[dm].c;
+    const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[dm].c;
     const long long nwords = list_words_d(c);
     if (b.cnt->status != MMCA_OK)
         return;
@@ -832,8 +838,8 @@ __global__ void __launch_bounds__(SCAN_BS)
 k_list_scan(View V, int decide_realloc)
 {
     __shared__ int s_w[SCAN_BS / 64];
-    const DevBufs &b = V.dom[blockIdx.x].b;
-    const Consts &c = V.dom[blockIdx.x].c;
+    const DevBufs b = V.dom[blockIdx.x].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[blockIdx.x].c;
     const int nblk = (int) ((list_words_d(c) + LIST_BS - 1) / LIST_BS);
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
@@ -924,8 +930,8 @@ k_capture(View V, int cb, int do_capture)
     __shared__ int s_ex[LIST_BS];
     const int dm = find_dom(V.off_list, V.ndom, blockIdx.x);
     const long long blk = blockIdx.x - V.off_list[dm];
-    const DevBufs &b = V.dom[dm].b;
-    const Consts &c = V.dom[dm].c;
+    const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[dm].c;
     const long long nwords = list_words_d(c);
     Counters *cnt = b.cnt;
     if (cnt->status != MMCA_OK)
@@ -944,8 +950,8 @@ k_capture(View V, int cb, int do_capture)
     const int base = b.blk_count[blk];            // members before this workgroup
     const bool realloc = do_capture && cnt->realloc_old >= 0;
     const long long maxg = c.max_total_grains;
-    double *fsb = b.fs[cb];
-    double *clb = b.cl[cb];
+    double *fsb = fs_buf(b, cb);
+    double *clb = cl_buf(b, cb);
     const int nnx[6] = { 1, 0, -1, 0, 0, 0 };
     const int nny[6] = { 0, 1, 0, -1, 0, 0 };
     const int nnz[6] = { 0, 0, 0, 0, 1, -1 };
@@ -1055,7 +1061,7 @@ k_capture(View V, int cb, int do_capture)
             b.gr[idx] = gfound;
             fsb[idx] = 0.;
             if (c.f32)
-                b.cl32[cb][idx] = b.ce32[idx];
+                cl32_buf(b, cb)[idx] = b.ce32[idx];
             else
                 clb[idx] = b.ce[idx];
         }
@@ -1081,7 +1087,7 @@ k_reduce_partial(DevBufs b, Consts c, int cb)
     __shared__ double s_a[RED_BS], s_b[RED_BS];
     double a = 0.0, s = 0.0;
     const long long N = (long long) c.nx * c.ny * c.nz;
-    const double *fs = b.fs[cb];
+    const double *fs = fs_buf(b, cb);
     for (long long i = (long long) blockIdx.x * RED_BS + threadIdx.x; i < N;
          i += (long long) gridDim.x * RED_BS)
     {
@@ -1181,7 +1187,7 @@ k_grow_octahedra(DevBufs b, Consts c, int cb)
         double gz0 = R[2] * rx0 + R[5] * ry0 + R[8] * rz0;
         double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
         ds1 = ref_min(ds1, 1.5);
-        b.d[i] = ds1 + b.fs[cb][i] * c.rev_fsgrow;
+        b.d[i] = ds1 + fs_buf(b, cb)[i] * c.rev_fsgrow;
     }
 }
 

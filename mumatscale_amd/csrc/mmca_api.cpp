@@ -1026,7 +1026,7 @@ struct mmca_batch {
     View view;
     bool view_dirty = true;
     int cur = 0;
-    int zm_planes = 16, zm_by = 8, xcd_remap = 1;
+    int zm_planes = 64, zm_by = 16, xcd_remap = 1;
     bool timing = false;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;

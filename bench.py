@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from mumatscale_amd import abi  # noqa: E402
-from mumatscale_amd.ctrl import Config  # noqa: E402
+from mumatscale_amd.workloads import directional_config, ensemble_configs  # noqa: E402,F401
 
 METRIC = "Mcell-updates/s & % HBM roofline; batched/FP32 vs plain FP64 at stated error"
 HBM_PEAK_GBS = 8000.0
@@ -36,63 +36,6 @@ B_ALG_FP64 = 56.0      # {ce, cl, fs, gr} read + written per cell-update
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def directional_config(nx=256, ny=256, nz=512):
-    c = Config()
-    c.nsub = [1, 1, 1]
-    c.sbdim = [nx, ny, nz]
-    c.pad_bndy = [1, 1, 1]
-    c.cell_size = 1.0e-6
-    c.fsgrow = 0.66
-    c.dt = 0.9 * c.cell_size ** 2 / (6 * 3.0e-9)
-    c.random_seed = 1
-    c.liquidus_temp = 933.0
-    c.solidus_temp = 850.0
-    c.c_init = 4.0
-    c.m_solute = [0.0, 0.0, -2.6]
-    c.part_coef = 0.14
-    c.d_sol = 1.0e-12
-    c.d_liq = 3.0e-9
-    c.t_init = 933.0 - 5.0
-    c.gradient = 1.0e4
-    c.velocity = 1.0e-3
-    c.bulk_density = 1.0e13
-    c.bulk_mean = 5.0
-    c.bulk_sigma = 0.5
-    c.surf_density = 1.0e10
-    c.surf_mean = 1.0
-    c.surf_sigma = 0.2
-    c.oriented = 1
-    c.max_total_grains = 1000000
-    return c
-
-
-def ensemble_configs(n=96, seed=3):
-    """96 independent domains: sides drawn from {32,48,64,96,128} per axis,
-    cooling rate log-uniform in [1, 1000] K/s, initial undercooling uniform
-    in [1, 5] K, bulk nucleation 1e14 m^-3 (mean 3 K, sigma 0.5 K) with a
-    distinct seed per domain; alloy and dt of the single-grain physics."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    sides = rng.choice([32, 48, 64, 96, 128], size=(n, 3))
-    rates = 10.0 ** rng.uniform(0.0, 3.0, size=n)
-    under = rng.uniform(1.0, 5.0, size=n)
-    out = []
-    for i in range(n):
-        c = directional_config(*(int(v) for v in sides[i]))
-        c.gradient = 0.0
-        c.velocity = 0.0
-        c.cool_rate = float(rates[i])
-        c.t_init = 933.0 - float(under[i])
-        c.surf_density = 0.0
-        c.bulk_density = 1.0e14
-        c.bulk_mean = 3.0
-        c.bulk_sigma = 0.5
-        c.random_seed = 1000 + i
-        c.max_total_grains = 100000
-        out.append(c)
-    return out
 
 
 def run_ensemble(steps, baseline_steps):
@@ -168,6 +111,75 @@ def run_fp32_variant(cfg, steps, warmup, zm, rows, xcd):
         "solute_drift_stored": (r32.solute_sum - n * cfg.c_init) / (n * cfg.c_init),
         "solute_drift_with_ledger": (r32.solute_sum + r32.solute_ledger - n * cfg.c_init) / (n * cfg.c_init)}
     return res
+
+
+def run_adjoint(n=128, nseeds=16, steps=4000):
+    """Gradients of the final-ce L2 mismatch with respect to D_l, D_s, k and
+    the initial ce field along a recorded trajectory of the seeded-grain
+    workload (n^3, `nseeds` seeds from seed 4, half the explicit dt), against
+    FP64 central finite differences of the frozen-branch replay
+    (h = 1e-6 relative).  The target is a free run with D_l, D_s, k scaled by
+    1.1, 0.8, 1.05."""
+    import numpy as np
+    import torch
+    from mumatscale_amd import adjoint as ADJ
+    from mumatscale_amd.lib import Subblock
+    from mumatscale_amd.workloads import seeded_grains_config
+    cfg = seeded_grains_config(n, nseeds, seed=4)
+    t0 = time.perf_counter()
+    with Subblock(cfg.params(check_negative_ce=False), cfg.nucleation()) as sb:
+        traj = ADJ.record(sb, steps)
+        grains = sb.num_grains - 1
+    t_rec = time.perf_counter() - t0
+    cfg2 = seeded_grains_config(n, nseeds, seed=4)
+    cfg2.d_liq *= 1.1
+    cfg2.d_sol *= 0.8
+    cfg2.part_coef *= 1.05
+    with Subblock(cfg2.params(check_negative_ce=False), cfg2.nucleation()) as sb2:
+        sb2.step(steps)
+        target = torch.from_numpy(sb2.download(abi.FIELD_CE).reshape(-1)).cuda()
+    base = (cfg.d_liq, cfg.d_sol, cfg.part_coef)
+    x = [torch.tensor(v, dtype=torch.float64, requires_grad=True) for v in base]
+    ce0 = traj.ce0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L = ADJ.l2_loss(traj, ADJ.solute_map(traj, *x, ce0), target)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+
+    def loss(p, c0=None):
+        (ce, _, _), _ = ADJ.replay(traj, *p, ce0=c0)
+        return ADJ.l2_loss(traj, ce, target).item()
+
+    names = ("d_liq", "d_sol", "part_coef")
+    grad, fd, rel = {}, {}, {}
+    h = 1e-6
+    for i, v in enumerate(base):
+        a, b = list(base), list(base)
+        a[i] = v * (1 + h)
+        b[i] = v * (1 - h)
+        f = (loss(a) - loss(b)) / (a[i] - b[i])
+        g = x[i].grad.item()
+        grad[names[i]], fd[names[i]], rel[names[i]] = g, f, abs(g - f) / abs(f)
+    rng = np.random.default_rng(11)
+    v = torch.zeros_like(traj.ce0)
+    traj.interior(v)[...] = torch.from_numpy(rng.normal(size=traj.interior(v).shape)).cuda()
+    f = (loss(base, traj.ce0 + h * v) - loss(base, traj.ce0 - h * v)) / (2 * h)
+    g = (ce0.grad * v).sum().item()
+    grad["ce0_random_direction"], fd["ce0_random_direction"] = g, f
+    rel["ce0_random_direction"] = abs(g - f) / abs(f)
+    K = ADJ.checkpoint_interval(steps)
+    P = traj.padded_cells
+    return {"grid": "%d^3" % n, "seeds": nseeds, "grains": grains, "steps": steps, "dt": cfg.dt,
+            "loss": L.item(), "checkpoint_interval": K,
+            "device_bytes": {"branch_record": steps * P, "checkpoints": ((steps + K - 1) // K) * 3 * 8 * P,
+                             "segment_states": K * 3 * 8 * P},
+            "record_s": round(t_rec, 3), "forward_s": round(t_fwd, 3), "backward_s": round(t_bwd, 3),
+            "grad": grad, "central_fd_h1e-6": fd, "rel_err": rel, "max_rel_err": max(rel.values())}
 
 
 def cpu_baseline(cfg, target_s):
@@ -248,6 +260,8 @@ def main():
         log("fp32dev:", extra["fp32dev"])
         extra["batched_ensemble"] = run_ensemble(3000, 200)
         log("ensemble:", extra["batched_ensemble"])
+        extra["adjoint"] = run_adjoint()
+        log("adjoint:", extra["adjoint"])
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, args.cpu_seconds)

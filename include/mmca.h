@@ -211,6 +211,14 @@ int  mmca_upload_field(mmca_ctx *ctx, int field, const void *host);
 int  mmca_download_field(mmca_ctx *ctx, int field, void *host);
 int  mmca_get_grains(mmca_ctx *ctx, mmca_grain *out, int32_t max, int32_t *num_grains);
 int  mmca_get_interface_list(mmca_ctx *ctx, int32_t *out, int32_t max, int32_t *gindex);
+/*
+ * Replace the grain table by records 0..num_grains-1 and set num_grains, as
+ * a restart restores grain_cache and bp->num_grains (addGrainCache,
+ * checkpoint.c:676-745).  Record 0 stands for the liquid; rotation
+ * matrices are used as given.  The next nucleated grain gets id num_grains.
+ * Cells refer to grains through the gr field (mmca_upload_field).
+ */
+int  mmca_set_grains(mmca_ctx *ctx, const mmca_grain *grains, int32_t num_grains);
 int  mmca_get_timestep(mmca_ctx *ctx, uint64_t *timestep);
 int  mmca_padded_cells(mmca_ctx *ctx, int64_t *cells);
 
@@ -267,6 +275,67 @@ int  mmca_batch_kernel_time_ms(mmca_batch *batch, double *avg_ms, int64_t *launc
 int  mmca_batch_autotune(const mmca_params *params, const mmca_nucleation_cfg *nucleation, int ndomains,
                          int device, int warmup, int trial_steps, int *best_planes, int *best_rows,
                          double *best_ms);
+
+/* --- adjoint of diffusion + partitioning on a recorded trajectory --------- */
+/*
+ * No reference counterpart: the reference has no derivatives.  A forward
+ * run records, per step and cell, which branch of fs_change_diffuse
+ * (growth.c:221-273) the cell took.  With those branches (and so every
+ * nucleation and capture decision) frozen, one step maps the FP64 state
+ * (ce, cl, fs) smoothly to the next one: diffusion (growth.c:85-113), then
+ * the recorded branch.  A capture (growth.c:789-791) sets cl = ce and
+ * fs = 0, which the LIQUID and REMELT branches of the captured cell have
+ * already set, so it adds nothing to this map.
+ */
+#define MMCA_BRANCH_LIQUID  0   /* gr <= 0, or T < 1 or T > T_liq: cl = ce, fs = 0 (:228-234) */
+#define MMCA_BRANCH_SOLID   1   /* fs == 1 and undercooled: cl = ce * (1/k)        (:236-240) */
+#define MMCA_BRANCH_GROW    2   /* cl += dcl, fs += dfs                            (:254-263) */
+#define MMCA_BRANCH_FULL    3   /* as GROW with fs clamped to 1                    (:264)     */
+#define MMCA_BRANCH_REMELT  4   /* fs + dfs < 0: fs = 0, gr = 0, cl = ce           (:266-272) */
+
+/*
+ * Record the branch of every cell for the next n_steps steps of mmca_step or
+ * mmca_doiteration into `codes` (device memory of n_steps * padded cells
+ * bytes, step s of the recording at codes + s * padded cells, padded layout;
+ * halo bytes are not written).  Recording uses the cell-parallel solute
+ * kernel and FP64 storage (MMCA_ERR_ARG otherwise); mmca_step refuses to run
+ * past the recorded range.  codes == NULL or n_steps == 0 stops recording.
+ */
+int  mmca_set_record(mmca_ctx *ctx, uint8_t *codes, int64_t n_steps);
+
+/*
+ * One frozen-branch step of the FP64 state.  All pointers are device
+ * memory of padded-cell size; `timestep` is the number of the recorded step
+ * (its temperature time is timestep * dt), `codes` its branch record.  The
+ * constants come from `params` (d_liq, d_sol and part_coef may differ from
+ * the recorded run).  Only interior cells of the outputs are written, and
+ * the outputs may not be the input buffers.  `stream` is a hipStream_t
+ * (NULL: the null stream).
+ */
+int  mmca_replay_step(const mmca_params *params, uint64_t timestep, const uint8_t *codes,
+                      const double *ce, const double *cl, const double *fs,
+                      double *ce_out, double *cl_out, double *fs_out, void *stream);
+
+/* Device workspace, in bytes, that mmca_adjoint_step needs for `params`. */
+int  mmca_adjoint_workspace_bytes(const mmca_params *params, int64_t *bytes);
+
+/*
+ * Adjoint (reverse-mode derivative) of mmca_replay_step.  (ce, cl, fs) is
+ * the state at the start of the step; (g_ce, g_cl, g_fs) the derivative of
+ * a scalar loss with respect to the state after it.  Writes the interior of
+ * (g_ce_in, g_cl_in, g_fs_in), the derivative with respect to the state at
+ * the start, and adds the derivative with respect to (d_liq, d_sol,
+ * part_coef) to g_param[0..2] (device memory).  Each g_*_in may be the
+ * same buffer as the matching g_* (in-place update).  The workspace is
+ * device memory of
+ * mmca_adjoint_workspace_bytes whose padded-cell part the library keeps at
+ * zero on the halo; zero it once before the first call.
+ */
+int  mmca_adjoint_step(const mmca_params *params, uint64_t timestep, const uint8_t *codes,
+                       const double *ce, const double *cl, const double *fs,
+                       const double *g_ce, const double *g_cl, const double *g_fs,
+                       double *g_ce_in, double *g_cl_in, double *g_fs_in, double *g_param,
+                       void *workspace, void *stream);
 
 #ifdef __cplusplus
 }

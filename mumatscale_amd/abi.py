@@ -52,6 +52,13 @@ TUNE_ZM_ROWS = 1
 TUNE_XCD_REMAP = 2
 TUNE_GRAPH = 3
 
+# partitioning branch codes recorded for the adjoint (fs_change_diffuse, growth.c:221-273)
+BRANCH_LIQUID = 0
+BRANCH_SOLID = 1
+BRANCH_GROW = 2
+BRANCH_FULL = 3
+BRANCH_REMELT = 4
+
 
 class Params(C.Structure):
     _fields_ = [

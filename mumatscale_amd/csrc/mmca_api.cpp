@@ -18,7 +18,7 @@
 
 namespace mmca {
 hipError_t launch_nucleate(const View &v, int mode, int cur, hipStream_t s);
-hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, int fsc, int cur, int nxt,
+hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, int fsc, int rec, int cur, int nxt,
                          hipStream_t s);
 hipError_t launch_mask_from_gr(const DevBufs &b, const Consts &c, hipStream_t s);
 hipError_t launch_list(const View &v, int cb, int capture, int decide_realloc, hipStream_t s);
@@ -85,9 +85,14 @@ struct mmca_ctx {
     size_t ev_used = 0;
     double time_ms = 0.0;
     long long time_n = 0;
+    // grain records set by mmca_set_grains (ids below ext_grains.size())
+    std::vector<mmca_grain> ext_grains;
 };
 
 namespace {
+
+// g_src of a grain whose record came from mmca_set_grains
+constexpr int GRAIN_SET = -2147483647 - 1;
 
 template <typename T>
 int dev_alloc(T **p, size_t n)
@@ -266,7 +271,9 @@ int enqueue_iteration(mmca_ctx *ctx, bool advance)
         }
         HIPCHK(hipEventRecord(ctx->ev[ev0], ctx->stream));
     }
-    HIPCHK(launch_solute(ctx->view, ctx->c.am, ctx->c.f32, zm_rows(ctx->c), 1, 1, ctx->cur, nxt, ctx->stream));
+    const int rec = ctx->b.rec != nullptr;
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, ctx->c.f32, rec ? 0 : zm_rows(ctx->c), 1, 1, rec, ctx->cur, nxt,
+                         ctx->stream));
     if (ctx->timing) {
         HIPCHK(hipEventRecord(ctx->ev[ev0 + 1], ctx->stream));
         ctx->ev_used = ev0 + 2;
@@ -503,8 +510,8 @@ int mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report)
     TRY(sync_counters(ctx));
     if (ctx->hc.status != MMCA_OK)
         return ctx->hc.status;
-    const int64_t start_new = ctx->hc.new_grains;
-    (void) start_new;
+    if (ctx->b.rec && ctx->hc.timestep + (uint64_t) nsteps > ctx->b.rec_t0 + (uint64_t) ctx->b.rec_steps)
+        return MMCA_ERR_ARG;     // past the end of the branch record
     ctx->hc.new_grains = 0;
     HIPCHK(hipMemcpyAsync(&ctx->b.cnt->new_grains, &ctx->hc.new_grains, sizeof(int),
                           hipMemcpyHostToDevice, ctx->stream));
@@ -570,6 +577,27 @@ int mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report)
     return ctx->hc.status;
 }
 
+int mmca_set_record(mmca_ctx *ctx, uint8_t *codes, int64_t n_steps)
+{
+    if (!ctx || n_steps < 0)
+        return MMCA_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (codes && n_steps > 0 && ctx->c.f32)
+        return MMCA_ERR_ARG;     // the adjoint works on FP64 storage
+    TRY(sync_counters(ctx));
+    if (codes && n_steps > 0) {
+        ctx->b.rec = codes;
+        ctx->b.rec_t0 = ctx->hc.timestep;
+        ctx->b.rec_steps = n_steps;
+    } else {
+        ctx->b.rec = nullptr;
+        ctx->b.rec_t0 = 0;
+        ctx->b.rec_steps = 0;
+    }
+    ctx->view_dirty = true;
+    return MMCA_OK;
+}
+
 int mmca_set_timestep(mmca_ctx *ctx, uint64_t timestep)
 {
     if (!ctx)
@@ -630,7 +658,7 @@ int mmca_sb_diffuse_alloy_decentered(mmca_ctx *ctx)
     HIPCHK(hipMemcpyAsync(ctx->b.oce, ctx->b.ce, (size_t) ctx->c.P * sizeof(double),
                           hipMemcpyDeviceToDevice, ctx->stream));
     TRY(ensure_view(ctx));
-    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 1, 0, ctx->cur, ctx->cur, ctx->stream));
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 1, 0, 0, ctx->cur, ctx->cur, ctx->stream));
     ctx->have_oce = true;
     return check_status(ctx);
 }
@@ -645,7 +673,7 @@ int mmca_fs_change_diffuse(mmca_ctx *ctx)
         return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
     TRY(ensure_view(ctx));
-    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 0, 1, ctx->cur, ctx->cur, ctx->stream));
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 0, 1, 0, ctx->cur, ctx->cur, ctx->stream));
     return check_status(ctx);
 }
 
@@ -887,6 +915,10 @@ int mmca_get_grains(mmca_ctx *ctx, mmca_grain *out, int32_t max, int32_t *num_gr
     memset(out, 0, sizeof(mmca_grain) * (size_t) n);
     for (int id = 1; id < n; id++) {
         mmca_grain &g = out[id];
+        if (src[id] == GRAIN_SET && id < (int) ctx->ext_grains.size()) {
+            g = ctx->ext_grains[id];   // record given to mmca_set_grains
+            continue;
+        }
         for (int q = 0; q < 9; q++)
             g.rotmat[q / 3][q % 3] = rot[(size_t) id * 9 + q];
         g.nuc_timestep = step[id];
@@ -922,6 +954,32 @@ int mmca_get_grains(mmca_ctx *ctx, mmca_grain *out, int32_t max, int32_t *num_gr
         }
     }
     return MMCA_OK;
+}
+
+int mmca_set_grains(mmca_ctx *ctx, const mmca_grain *grains, int32_t num_grains)
+{
+    if (!ctx || !grains || num_grains < 1 || (int64_t) num_grains > ctx->p.max_total_grains + 1)
+        return MMCA_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    TRY(sync_counters(ctx));
+    const size_t n = (size_t) num_grains;
+    std::vector<double> rot(n * 9);
+    std::vector<int> src(n, GRAIN_SET), cell(n, -1);
+    std::vector<unsigned long long> step(n);
+    for (size_t id = 0; id < n; id++) {
+        for (int q = 0; q < 9; q++)
+            rot[id * 9 + q] = grains[id].rotmat[q / 3][q % 3];
+        step[id] = grains[id].nuc_timestep;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->b.g_rot, rot.data(), rot.size() * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->b.g_src, src.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->b.g_cell, cell.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->b.g_step, step.data(), n * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                          ctx->stream));
+    ctx->ext_grains.assign(grains, grains + n);
+    ctx->hc.num_grains = num_grains;
+    return push_counters(ctx);
 }
 
 int mmca_get_interface_list(mmca_ctx *ctx, int32_t *out, int32_t max, int32_t *gindex)
@@ -1082,7 +1140,7 @@ int batch_enqueue(mmca_batch *bt, int cur)
         }
         HIPCHK(hipEventRecord(bt->ev[e0], bt->stream));
     }
-    HIPCHK(launch_solute(bt->view, c0.am, c0.f32, bt->zm_planes > 0 ? bt->zm_by : 0, 1, 1, cur, nxt,
+    HIPCHK(launch_solute(bt->view, c0.am, c0.f32, bt->zm_planes > 0 ? bt->zm_by : 0, 1, 1, 0, cur, nxt,
                          bt->stream));
     if (bt->timing) {
         HIPCHK(hipEventRecord(bt->ev[e0 + 1], bt->stream));

@@ -1,0 +1,118 @@
+// Device functions shared by the CA step kernels and the adjoint kernels.
+// Arithmetic follows the reference's operation order; everything that
+// includes this file is compiled with -ffp-contract=off.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mmca_internal.h"
+
+namespace mmca {
+
+// MIN(x,y) of globals.h:19
+__device__ __forceinline__ double ref_min(double x, double y) { return (x < y) ? x : y; }
+
+// Sum over a workgroup of NT threads (result valid in thread 0).
+template <int NT>
+__device__ __forceinline__ double
+block_sum(double v, double *s_red)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1)
+        v += __shfl_xor(v, d, 64);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0)
+        s_red[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int i = 0; i < NT / 64; ++i)
+            s += s_red[i];
+    return s;
+}
+
+// direct_temp_calc (temperature.c:24-126); x,y,z are padded indices.  AM
+// selects the Gaussian-beam model at compile time, so that kernels of the
+// other models do not carry its registers.
+template <bool AM>
+__device__ __forceinline__ double
+temp_model(const Consts &c, double sim_time, int x, int y, int z)
+{
+    if (!AM && c.gradient == 0)
+        return c.t_init - sim_time * c.cool_rate;
+    double rx = c.origin[0] + ((c.off[0] + x) + 0.5) * c.cell_size;
+    double ry = c.origin[1] + ((c.off[1] + y) + 0.5) * c.cell_size;
+    double rz = c.origin[2] + ((c.off[2] + z) + 0.5) * c.cell_size;
+    if (!AM)
+    {
+        double axis = ((c.gdimz == 1) ? ry : rz) - c.cell_size * 0.5;
+        double tgrad = c.gradient + c.grad_coef * sim_time;
+        double v = c.velocity + c.velo_coef * sim_time;
+        double vt = 0.5 * (v + c.velocity) * sim_time;
+        return c.t_init + tgrad * (axis - vt);
+    }
+    double lv = c.lv;
+    double invlv = 1. / lv;
+    double ls = c.lsd / (2. * 1.414);
+    double xlen = c.gnsbx * c.nx * c.cell_size;
+    xlen = xlen + c.lsd;
+    double tline = xlen * invlv;
+    double tpause = 1.5e-4 * invlv;
+    tline = tline + tpause;
+    double trackno = 0;
+    if (c.am_tracks > 1)
+        trackno = floor(sim_time / tline);
+    double xnow = c.x0 + (sim_time - trackno * tline) * lv;
+    double ynow = c.y0 + trackno * c.lhs;
+    double znow = c.z0;
+    double lpow = c.lpow;
+    double lx = rx - xnow;
+    double ly = ry - ynow;
+    double lz = rz - znow;
+    if (!(rz <= znow + c.cell_size))
+        return 0.0;
+    const int nt = 400;
+    double Tinteg = 0.0;
+    double pi15 = 5.568328;
+    double lalpha = c.lthcon / (c.rho * c.lcp);
+    double tsteady = 20 * lalpha * invlv * invlv;
+    double ldt = tsteady / nt;
+    if (sim_time < 1e-4)
+        lpow = 0;
+    for (int i = 0; i < nt; i++)
+    {
+        double lt = (0.5 + i) * ldt;
+        double Tlocal = sqrt(lalpha * lt) * (ls * ls + 4. * lalpha * lt);
+        Tinteg += ldt *
+            exp(-((lx + lv * lt) * (lx + lv * lt) + ly * ly) / (ls * ls + 4. * lalpha * lt) -
+                (lz * lz / (4. * lalpha * lt))) / Tlocal;
+    }
+    double Tfront = c.lab * lpow * lalpha / (pi15 * c.lthcon);
+    return c.liniT + Tfront * Tinteg;
+}
+
+__device__ inline double
+temp_at(const Consts &c, double sim_time, int x, int y, int z)
+{
+    return c.am ? temp_model<true>(c, sim_time, x, y, z) : temp_model<false>(c, sim_time, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// Diffusion (growth.c:85-113, order +x,-x,+y,-y,+z,-z) of one cell, from the
+// neighbours' cl and fs and their mold flags.
+__device__ __forceinline__ double
+diffuse_cell(const Consts &c, double ce_old, double conc, double lspfs, const double *cln,
+             const double *fsn, const int *mo)
+{
+    double nbsum = 0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n)
+    {
+        double fs_av = 0.5 * (lspfs + fsn[n]);
+        double r = (c.rs * fs_av + c.rl * (1 - fs_av)) * (1 - mo[n]);
+        nbsum += r * (cln[n] - conc);
+    }
+    return ce_old + nbsum;
+}
+
+}  // namespace mmca

@@ -98,6 +98,11 @@ struct DevBufs {
     double *red_part;                // 2 * nblk_red partial sums
     int nblk_red;
     Counters *cnt;
+    // partitioning-branch record (caller's device memory; null when off):
+    // the step with timestep t goes to rec + (t - rec_t0 - 1) * P
+    unsigned char *rec;
+    unsigned long long rec_t0;
+    long long rec_steps;
 };
 
 // One domain (subblock) as the hot kernels see it.

@@ -1,6 +1,6 @@
 // HIP kernels of the CA step for gfx950 (CDNA4, wave64).
 //
-// One step (doiteration, calculate.c:193-352) on one subblock is six
+// One step (doiteration, calculate.c:193-352) on one subblock is seven
 // launches on the context's stream:
 //   k_nuc_detect    cell_nucleation (grain.c:1222-1318) over the sorted site list
 //   k_nuc_activate  activateNewGrains (grain.c:841-916): ids in site order, one workgroup
@@ -9,7 +9,9 @@
 //   k_list_count    interface-list membership counts from the mask
 //                   (grow_cell_reduction, growth.c:461-538)
 //   k_list_scan     exclusive scan of the counts -> raster-order slots
-//   k_capture       capture_octahedra_diffuse (growth.c:574-807) with the
+//   k_list_write    the list itself (diff_id) in raster order
+//   k_capture       capture_octahedra_diffuse (growth.c:574-807) over the
+//                   list slots, dealt evenly over the whole grid, with the
 //                   octahedron half-diagonal of each candidate neighbour
 //                   computed on the fly (grow_octahedra, growth.c:339-407)
 //
@@ -19,12 +21,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mmca_device.h"
 #include "mmca_internal.h"
 
 namespace mmca {
-
-// MIN(x,y) of globals.h:19
-__device__ __forceinline__ double ref_min(double x, double y) { return (x < y) ? x : y; }
 
 // Ping-pong buffer `buf` (0/1) of a field, chosen with a select rather than an
 // indexed load, so that a private copy of DevBufs stays in registers.
@@ -84,91 +84,6 @@ ledger_add(const DevBufs &b, long long slot, double v)
     comp = (t - s) - y;
     b.ledger[2 * slot] = t;
     b.ledger[2 * slot + 1] = comp;
-}
-
-// Sum over a workgroup of NT threads (result valid in thread 0).
-template <int NT>
-__device__ __forceinline__ double
-block_sum(double v, double *s_red)
-{
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1)
-        v += __shfl_xor(v, d, 64);
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((tid & 63) == 0)
-        s_red[tid >> 6] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (tid == 0)
-        for (int i = 0; i < NT / 64; ++i)
-            s += s_red[i];
-    return s;
-}
-
-// direct_temp_calc (temperature.c:24-126); x,y,z are padded indices.  AM
-// selects the Gaussian-beam model at compile time, so that kernels of the
-// other models do not carry its registers.
-template <bool AM>
-__device__ __forceinline__ double
-temp_model(const Consts &c, double sim_time, int x, int y, int z)
-{
-    if (!AM && c.gradient == 0)
-        return c.t_init - sim_time * c.cool_rate;
-    double rx = c.origin[0] + ((c.off[0] + x) + 0.5) * c.cell_size;
-    double ry = c.origin[1] + ((c.off[1] + y) + 0.5) * c.cell_size;
-    double rz = c.origin[2] + ((c.off[2] + z) + 0.5) * c.cell_size;
-    if (!AM)
-    {
-        double axis = ((c.gdimz == 1) ? ry : rz) - c.cell_size * 0.5;
-        double tgrad = c.gradient + c.grad_coef * sim_time;
-        double v = c.velocity + c.velo_coef * sim_time;
-        double vt = 0.5 * (v + c.velocity) * sim_time;
-        return c.t_init + tgrad * (axis - vt);
-    }
-    double lv = c.lv;
-    double invlv = 1. / lv;
-    double ls = c.lsd / (2. * 1.414);
-    double xlen = c.gnsbx * c.nx * c.cell_size;
-    xlen = xlen + c.lsd;
-    double tline = xlen * invlv;
-    double tpause = 1.5e-4 * invlv;
-    tline = tline + tpause;
-    double trackno = 0;
-    if (c.am_tracks > 1)
-        trackno = floor(sim_time / tline);
-    double xnow = c.x0 + (sim_time - trackno * tline) * lv;
-    double ynow = c.y0 + trackno * c.lhs;
-    double znow = c.z0;
-    double lpow = c.lpow;
-    double lx = rx - xnow;
-    double ly = ry - ynow;
-    double lz = rz - znow;
-    if (!(rz <= znow + c.cell_size))
-        return 0.0;
-    const int nt = 400;
-    double Tinteg = 0.0;
-    double pi15 = 5.568328;
-    double lalpha = c.lthcon / (c.rho * c.lcp);
-    double tsteady = 20 * lalpha * invlv * invlv;
-    double ldt = tsteady / nt;
-    if (sim_time < 1e-4)
-        lpow = 0;
-    for (int i = 0; i < nt; i++)
-    {
-        double lt = (0.5 + i) * ldt;
-        double Tlocal = sqrt(lalpha * lt) * (ls * ls + 4. * lalpha * lt);
-        Tinteg += ldt *
-            exp(-((lx + lv * lt) * (lx + lv * lt) + ly * ly) / (ls * ls + 4. * lalpha * lt) -
-                (lz * lz / (4. * lalpha * lt))) / Tlocal;
-    }
-    double Tfront = c.lab * lpow * lalpha / (pi15 * c.lthcon);
-    return c.liniT + Tfront * Tinteg;
-}
-
-__device__ double
-temp_at(const Consts &c, double sim_time, int x, int y, int z)
-{
-    return c.am ? temp_model<true>(c, sim_time, x, y, z) : temp_model<false>(c, sim_time, x, y, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -418,33 +333,17 @@ k_nuc_activate(View V, int mode)
         cnt->timestep = ts;
 }
 
-// ---------------------------------------------------------------------------
-// Diffusion (growth.c:85-113, order +x,-x,+y,-y,+z,-z) of one cell, from the
-// neighbours' cl and fs and their mold flags.
-__device__ __forceinline__ double
-diffuse_cell(const Consts &c, double ce_old, double conc, double lspfs, const double *cln,
-             const double *fsn, const int *mo)
-{
-    double nbsum = 0;
-#pragma unroll
-    for (int n = 0; n < 6; ++n)
-    {
-        double fs_av = 0.5 * (lspfs + fsn[n]);
-        double r = (c.rs * fs_av + c.rl * (1 - fs_av)) * (1 - mo[n]);
-        nbsum += r * (cln[n] - conc);
-    }
-    return ce_old + nbsum;
-}
-
 // Partitioning of one cell (fs_change_diffuse, growth.c:221-273).  The
-// temperature is only needed for cells with gr > 0.
+// temperature is only needed for cells with gr > 0.  *branch receives the
+// MMCA_BRANCH_* code of the branch taken.
 template <bool AM>
 __device__ __forceinline__ void
 partition_cell(const Consts &c, double sim_time, int x, int j, int k, double ce_new,
                double ce_prev, double conc, double lspfs, int g_old, double *cl_out,
-               double *fs_out, int *g_out)
+               double *fs_out, int *g_out, int *branch)
 {
     int g = g_old;
+    *branch = MMCA_BRANCH_LIQUID;
     if (g <= 0)
     {
         *cl_out = ce_new;
@@ -468,6 +367,7 @@ partition_cell(const Consts &c, double sim_time, int x, int j, int k, double ce_
         *cl_out = ce_new * c.k_inv;
         *fs_out = lspfs;
         *g_out = g;
+        *branch = MMCA_BRANCH_SOLID;
         return;
     }
     double dce = ce_new - ce_prev;
@@ -475,12 +375,14 @@ partition_cell(const Consts &c, double sim_time, int x, int j, int k, double ce_
     double dfs = (dcl * (1. - c.km * lspfs) - dce) * c.km_inv / conc;
     double cl = conc + dcl;
     double fs = lspfs + dfs;
+    *branch = (1.0 < fs) ? MMCA_BRANCH_FULL : MMCA_BRANCH_GROW;
     fs = ref_min(1.0, fs);
     if (fs < 0.)
     {
         fs = 0.;
         g = 0;
         cl = ce_new;
+        *branch = MMCA_BRANCH_REMELT;
     }
     *cl_out = cl;
     *fs_out = fs;
@@ -497,11 +399,12 @@ flag_negative_ce(Counters *cnt)
 // Cell-parallel kernel.  DIFF && FSC: fused step reading buffers `cur`,
 // writing `nxt`.  DIFF only: ce += nbsum (the caller copies ce to oce
 // beforehand).  FSC only: in place on `cur` with ce_old = oce.  FSC writes
-// the solid mask (gr > 0).
+// the solid mask (gr > 0).  REC also writes each cell's partitioning branch
+// into the domain's branch record.
 constexpr int SOL_BX = 64;
 constexpr int SOL_BY = 4;
 
-template <bool DIFF, bool FSC, bool AM, bool F32>
+template <bool DIFF, bool FSC, bool AM, bool F32, bool REC = false>
 __global__ void __launch_bounds__(SOL_BX *SOL_BY)
 k_solute(View V, int cur, int nxt)
 {
@@ -562,9 +465,16 @@ k_solute(View V, int cur, int nxt)
             const double ce_prev = DIFF ? ce_old : b.oce[idx];
             const int g_old = b.gr[idx];
             double cl_out, fs_out;
-            int g;
+            int g, branch;
             partition_cell<AM>(c, cnt->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
-                               &cl_out, &fs_out, &g);
+                               &cl_out, &fs_out, &g, &branch);
+            if (REC)
+            {
+                // branch record of this step (bounds: set by the host, checked here too)
+                const long long rs = (long long) (cnt->timestep - b.rec_t0) - 1;
+                if (rs >= 0 && rs < b.rec_steps)
+                    b.rec[rs * c.P + idx] = (unsigned char) branch;
+            }
             st_cl<F32>(b, c, nxt, idx, cl_out);
             fs_buf(b, nxt)[idx] = fs_out;
             if (g != g_old)
@@ -723,8 +633,9 @@ k_solute_zm(View V, int cur, int nxt)
                 flag_negative_ce(cnt);
             lost += st_ce<F32>(b, c, idx, ce_new);
             double cl_out, fs_out;
+            int branch;
             partition_cell<AM>(c, sim_time, x, j, k, ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out,
-                               &g_new);
+                               &g_new, &branch);
             st_cl<F32>(b, c, nxt, idx, cl_out);
             fsn_out[idx] = fs_out;
             if (g_new != g_old)
@@ -896,11 +807,11 @@ solid_at(const DevBufs &b, const Consts &c, int x, int j, int k)
     return (wd >> ((x - 1) & 63)) & 1ull;
 }
 
-// capture_octahedra_diffuse, INDEX_SEP branch (growth.c:640-807).
-// do_capture = 0 only writes the list (grow_cell_reduction).  `cb` is the
-// buffer holding fs/cl after fs_change.  The members of a workgroup's 256
-// mask words are spread evenly over its threads: member i of the workgroup
-// is found by a binary search over the per-word prefix counts in LDS.
+// Interface list writer (grow_cell_reduction, growth.c:496-513): slot
+// base+i+1 of diff_id gets the padded index of the workgroup's member i, in
+// raster order.  The members of a workgroup's 256 mask words are spread
+// evenly over its threads: member i is found by a binary search over the
+// per-word prefix counts in LDS.
 __device__ __forceinline__ int
 nth_set_bit(unsigned long long m, int r)
 {
@@ -923,7 +834,7 @@ nth_set_bit(unsigned long long m, int r)
 }
 
 __global__ void __launch_bounds__(LIST_BS)
-k_capture(View V, int cb, int do_capture)
+k_list_write(View V)
 {
     __shared__ int s_w[LIST_BS / 64];
     __shared__ unsigned long long s_mem[LIST_BS];
@@ -933,8 +844,7 @@ k_capture(View V, int cb, int do_capture)
     const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
     const Consts c = V.dom[dm].c;
     const long long nwords = list_words_d(c);
-    Counters *cnt = b.cnt;
-    if (cnt->status != MMCA_OK)
+    if (b.cnt->status != MMCA_OK)
         return;
     const long long wi0 = blk * LIST_BS;
     const long long wi = wi0 + threadIdx.x;
@@ -948,13 +858,6 @@ k_capture(View V, int cb, int do_capture)
     s_ex[threadIdx.x] = ex;
     __syncthreads();
     const int base = b.blk_count[blk];            // members before this workgroup
-    const bool realloc = do_capture && cnt->realloc_old >= 0;
-    const long long maxg = c.max_total_grains;
-    double *fsb = fs_buf(b, cb);
-    double *clb = cl_buf(b, cb);
-    const int nnx[6] = { 1, 0, -1, 0, 0, 0 };
-    const int nny[6] = { 0, 1, 0, -1, 0, 0 };
-    const int nnz[6] = { 0, 0, 0, 0, 1, -1 };
     for (int i = threadIdx.x; i < tot; i += LIST_BS)
     {
         // word of member i: the last word whose prefix count is <= i
@@ -974,11 +877,41 @@ k_capture(View V, int cb, int do_capture)
         const int j = (int) (nrow % c.ny) + 1;
         const int k = (int) (nrow / c.ny) + 1;
         const int x = w * 64 + bit + 1;
-        const int slot = base + i + 1;             // reference slots start at 1
-        const long long idx = k * c.sxy + (long long) j * c.sx + x;
-        b.diff_id[slot] = (int) idx;
-        if (!do_capture)
-            continue;
+        b.diff_id[base + i + 1] = (int) (k * c.sxy + (long long) j * c.sx + x);   // slots start at 1
+    }
+}
+
+// capture_octahedra_diffuse, INDEX_SEP branch (growth.c:640-807), over the
+// slots 1..gindex of the list.  Every member is independent: it reads only
+// solid neighbours (through the pre-capture mask) and writes only itself and
+// its slot, so the slots are dealt round-robin over all the domain's threads
+// whatever their place in the grid.  `cb` is the buffer holding fs/cl after
+// fs_change.
+__global__ void __launch_bounds__(LIST_BS)
+k_capture(View V, int cb)
+{
+    const int dm = find_dom(V.off_list, V.ndom, blockIdx.x);
+    const long long blk = blockIdx.x - V.off_list[dm];
+    const long long nthr = (long long) (V.off_list[dm + 1] - V.off_list[dm]) * LIST_BS;
+    const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[dm].c;
+    const Counters *cnt = b.cnt;
+    if (cnt->status != MMCA_OK)
+        return;
+    const int nmem = cnt->gindex;
+    const bool realloc = cnt->realloc_old >= 0;
+    const long long maxg = c.max_total_grains;
+    double *fsb = fs_buf(b, cb);
+    double *clb = cl_buf(b, cb);
+    const unsigned usx = (unsigned) c.sx, usxy = (unsigned) c.sxy;
+    for (long long t = blk * LIST_BS + threadIdx.x; t < nmem; t += nthr)
+    {
+        const int slot = (int) t + 1;
+        const unsigned idx = (unsigned) b.diff_id[slot];
+        const int k = (int) (idx / usxy);
+        const unsigned r = idx - (unsigned) k * usxy;
+        const int j = (int) (r / usx);
+        const int x = (int) (r - (unsigned) j * usx);
         double tx, ty, tz;
         if (realloc)
             tx = ty = tz = 0.0;
@@ -988,54 +921,64 @@ k_capture(View V, int cb, int do_capture)
             ty = b.dty[slot];
             tz = b.dtz[slot];
         }
-        const long long ndxs[6] = { idx + 1, idx + c.sx, idx - 1, idx - c.sx, idx + c.sxy, idx - c.sxy };
-        const int nxs[6] = { x + 1, x, x - 1, x, x, x };
-        const int nys[6] = { j, j + 1, j, j - 1, j, j };
-        const int nzs[6] = { k, k, k, k, k + 1, k - 1 };
+        // neighbour order +x, +y, -x, -y, +z, -z (growth.c:686-700)
+        const int nnx[6] = { 1, 0, -1, 0, 0, 0 };
+        const int nny[6] = { 0, 1, 0, -1, 0, 0 };
+        const int nnz[6] = { 0, 0, 0, 0, 1, -1 };
+        const long long ndxs[6] = { (long long) idx + 1, (long long) idx + c.sx, (long long) idx - 1,
+                                    (long long) idx - c.sx, (long long) idx + c.sxy, (long long) idx - c.sxy };
+        // solid neighbours, then their grain ids: independent loads issued together
+        bool sol[6];
+        sol[0] = solid_at(b, c, x + 1, j, k);
+        sol[1] = solid_at(b, c, x, j + 1, k);
+        sol[2] = solid_at(b, c, x - 1, j, k);
+        sol[3] = solid_at(b, c, x, j - 1, k);
+        sol[4] = solid_at(b, c, x, j, k + 1);
+        sol[5] = solid_at(b, c, x, j, k - 1);
+        int gids[6];
+#pragma unroll
+        for (int n = 0; n < 6; ++n)
+            gids[n] = sol[n] ? b.gr[ndxs[n]] : 0;
         int found = -1;
         int gfound = 0;
-        long long ndx = 0;
         double g00 = 0, g10 = 0, g20 = 0, g01 = 0, g11 = 0, g21 = 0, g02 = 0, g12 = 0, g22 = 0;
         double dx = 0, dy = 0, dz = 0, dn = 0, ddx = 0, ddy = 0, ddz = 0;
         for (int n = 0; n < 6; ++n)
         {
-            if (!solid_at(b, c, nxs[n], nys[n], nzs[n]))
+            const int gid = gids[n];
+            if (!(gid > 0 && gid < maxg))
                 continue;
-            ndx = ndxs[n];
-            const int gid = b.gr[ndx];
-            if (gid > 0 && gid < maxg)
+            const long long ndx = ndxs[n];
+            const double *R = b.g_rot + 9 * (long long) gid;
+            g00 = R[0]; g01 = R[1]; g02 = R[2];
+            g10 = R[3]; g11 = R[4]; g12 = R[5];
+            g20 = R[6]; g21 = R[7]; g22 = R[8];
+            ddx = b.dcx[ndx];
+            ddy = b.dcy[ndx];
+            ddz = b.dcz[ndx];
+            // grow_octahedra (growth.c:392-403)
+            double rx0 = 0.0 - ddx;
+            double ry0 = 0.0 - ddy;
+            double rz0 = 0.0 - ddz;
+            double gx0 = g00 * rx0 + g10 * ry0 + g20 * rz0;
+            double gy0 = g01 * rx0 + g11 * ry0 + g21 * rz0;
+            double gz0 = g02 * rx0 + g12 * ry0 + g22 * rz0;
+            double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
+            ds1 = ref_min(ds1, 1.5);
+            dn = ds1 + fsb[ndx] * c.rev_fsgrow;
+            // growth.c:717-735
+            double rx1 = -1.0 * nnx[n] - ddx;
+            double ry1 = -1.0 * nny[n] - ddy;
+            double rz1 = -1.0 * nnz[n] - ddz;
+            dx = g00 * rx1 + g10 * ry1 + g20 * rz1;
+            dy = g01 * rx1 + g11 * ry1 + g21 * rz1;
+            dz = g02 * rx1 + g12 * ry1 + g22 * rz1;
+            double dfs = dn - (fabs(dx) + fabs(dy) + fabs(dz));
+            if (dfs > 0)
             {
-                const double *R = b.g_rot + 9 * (long long) gid;
-                g00 = R[0]; g01 = R[1]; g02 = R[2];
-                g10 = R[3]; g11 = R[4]; g12 = R[5];
-                g20 = R[6]; g21 = R[7]; g22 = R[8];
-                ddx = b.dcx[ndx];
-                ddy = b.dcy[ndx];
-                ddz = b.dcz[ndx];
-                // grow_octahedra (growth.c:392-403)
-                double rx0 = 0.0 - ddx;
-                double ry0 = 0.0 - ddy;
-                double rz0 = 0.0 - ddz;
-                double gx0 = g00 * rx0 + g10 * ry0 + g20 * rz0;
-                double gy0 = g01 * rx0 + g11 * ry0 + g21 * rz0;
-                double gz0 = g02 * rx0 + g12 * ry0 + g22 * rz0;
-                double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
-                ds1 = ref_min(ds1, 1.5);
-                dn = ds1 + fsb[ndx] * c.rev_fsgrow;
-                // growth.c:717-735
-                double rx1 = -1.0 * nnx[n] - ddx;
-                double ry1 = -1.0 * nny[n] - ddy;
-                double rz1 = -1.0 * nnz[n] - ddz;
-                dx = g00 * rx1 + g10 * ry1 + g20 * rz1;
-                dy = g01 * rx1 + g11 * ry1 + g21 * rz1;
-                dz = g02 * rx1 + g12 * ry1 + g22 * rz1;
-                double dfs = dn - (fabs(dx) + fabs(dy) + fabs(dz));
-                if (dfs > 0)
-                {
-                    found = n;
-                    gfound = gid;
-                    break;
-                }
+                found = n;
+                gfound = gid;
+                break;
             }
         }
         if (found > -1)
@@ -1205,9 +1148,18 @@ hipError_t launch_nucleate(const View &v, int mode, int cur, hipStream_t s)
 // zm_by: rows of the z-marching kernel (all domains of a view share it); 0
 // selects the cell-parallel kernel.
 template <bool AM, bool F32>
-static void launch_solute_t(const View &v, int zm_by, int diff, int fsc, int cur, int nxt, hipStream_t s)
+static void launch_solute_t(const View &v, int zm_by, int diff, int fsc, int rec, int cur, int nxt,
+                            hipStream_t s)
 {
     dim3 blk(SOL_BX, SOL_BY);
+    if (rec)
+    {
+        // branch recording: FP64 storage, fused cell-parallel kernel (the caller checks)
+        if (diff && fsc && !F32)
+            hipLaunchKernelGGL((k_solute<true, true, AM, false, true>), dim3((unsigned) v.tot_cell), blk, 0, s, v,
+                               cur, nxt);
+        return;
+    }
     if (diff && fsc && zm_by > 0)
     {
         const long long ng = v.xcd_remap ? ((v.tot_zm + 7) / 8) * 8 : v.tot_zm;
@@ -1230,22 +1182,22 @@ static void launch_solute_t(const View &v, int zm_by, int diff, int fsc, int cur
                            nxt);
 }
 
-hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, int fsc, int cur, int nxt,
+hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, int fsc, int rec, int cur, int nxt,
                          hipStream_t s)
 {
     if (am)
     {
         if (f32)
-            launch_solute_t<true, true>(v, zm_by, diff, fsc, cur, nxt, s);
+            launch_solute_t<true, true>(v, zm_by, diff, fsc, rec, cur, nxt, s);
         else
-            launch_solute_t<true, false>(v, zm_by, diff, fsc, cur, nxt, s);
+            launch_solute_t<true, false>(v, zm_by, diff, fsc, rec, cur, nxt, s);
     }
     else
     {
         if (f32)
-            launch_solute_t<false, true>(v, zm_by, diff, fsc, cur, nxt, s);
+            launch_solute_t<false, true>(v, zm_by, diff, fsc, rec, cur, nxt, s);
         else
-            launch_solute_t<false, false>(v, zm_by, diff, fsc, cur, nxt, s);
+            launch_solute_t<false, false>(v, zm_by, diff, fsc, rec, cur, nxt, s);
     }
     return hipGetLastError();
 }
@@ -1274,7 +1226,9 @@ hipError_t launch_list(const View &v, int cb, int capture, int decide_realloc, h
         hipLaunchKernelGGL(k_list_count, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v);
     hipLaunchKernelGGL(k_list_scan, dim3(v.ndom), dim3(SCAN_BS), 0, s, v, decide_realloc);
     if (capture >= 0 && v.tot_list > 0)
-        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb, capture);
+        hipLaunchKernelGGL(k_list_write, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v);
+    if (capture > 0 && v.tot_list > 0)
+        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb);
     return hipGetLastError();
 }
 
@@ -1282,7 +1236,7 @@ hipError_t launch_capture_only(const View &v, const DevBufs &b, int cb, hipStrea
 {
     hipLaunchKernelGGL(k_realloc_decide, dim3(1), dim3(1), 0, s, b);
     if (v.tot_list > 0)
-        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb, 1);
+        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb);
     return hipGetLastError();
 }
 

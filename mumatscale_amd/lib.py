@@ -50,6 +50,7 @@ def lib():
     L.mmca_download_field.argtypes = [vp, C.c_int, vp]
     L.mmca_get_grains.argtypes = [vp, C.POINTER(abi.Grain), C.c_int32, i32p]
     L.mmca_get_interface_list.argtypes = [vp, i32p, C.c_int32, i32p]
+    L.mmca_set_grains.argtypes = [vp, C.POINTER(abi.Grain), C.c_int32]
     L.mmca_padded_cells.argtypes = [vp, C.POINTER(C.c_int64)]
     L.mmca_set_kernel_timing.argtypes = [vp, C.c_int]
     L.mmca_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -71,6 +72,12 @@ def lib():
                                         C.POINTER(C.c_int64)]
     L.mmca_orientation_draws.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg),
                                          C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mmca_set_record.argtypes = [vp, vp, C.c_int64]
+    dp = C.c_void_p
+    L.mmca_replay_step.argtypes = [C.POINTER(abi.Params), C.c_uint64, vp, dp, dp, dp, dp, dp, dp, vp]
+    L.mmca_adjoint_workspace_bytes.argtypes = [C.POINTER(abi.Params), C.POINTER(C.c_int64)]
+    L.mmca_adjoint_step.argtypes = [C.POINTER(abi.Params), C.c_uint64, vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
+                                    vp, vp]
     if L.mmca_abi_version() != abi.ABI_VERSION:
         raise RuntimeError("libmmca ABI version mismatch")
     _lib = L
@@ -242,6 +249,11 @@ class Subblock:
         _check(lib().mmca_get_grains(self._h, arr, n, None), "mmca_get_grains")
         return [arr[i] for i in range(n)]
 
+    def set_grains(self, grains):
+        """Replace the grain table (list of abi.Grain, record 0 = liquid)."""
+        arr = (abi.Grain * len(grains))(*grains)
+        _check(lib().mmca_set_grains(self._h, arr, len(grains)), "mmca_set_grains")
+
     def interface_list(self):
         g = C.c_int32(0)
         _check(lib().mmca_get_interface_list(self._h, None, 0, C.byref(g)), "mmca_get_interface_list")
@@ -259,6 +271,12 @@ class Subblock:
 
     def set_tuning(self, key, value):
         _check(lib().mmca_set_tuning(self._h, key, value), "mmca_set_tuning")
+
+    def set_record(self, codes_ptr, n_steps):
+        """Record the partitioning branch of every cell for the next n_steps
+        steps into device memory at codes_ptr (n_steps x padded cells bytes);
+        (None, 0) stops recording."""
+        _check(lib().mmca_set_record(self._h, codes_ptr, n_steps), "mmca_set_record")
 
     def get_tuning(self, key):
         v = C.c_int32(0)

@@ -175,6 +175,9 @@ typedef struct orc_state {
     int oriented;
     int status;
     uint64_t failed_step;
+    /* when set, orc_fs_change writes each interior cell's MMCA_BRANCH_* code
+     * here (padded layout, one step) */
+    uint8_t *rec;
 } orc_state;
 
 #define SBIDX(s, x, y, z) \
@@ -896,11 +899,15 @@ orc_fs_change(orc_state *s)
                     cl[idx] = ce[idx];
                     gr[idx] = 0;
                     fs[idx] = 0;
+                    if (s->rec)
+                        s->rec[idx] = MMCA_BRANCH_LIQUID;
                     continue;
                 }
                 if (fs[idx] == 1.0 && (Tunder >= 0.0))
                 {
                     cl[idx] = ce[idx] * k_inv;
+                    if (s->rec)
+                        s->rec[idx] = MMCA_BRANCH_SOLID;
                     continue;
                 }
                 double dce = ce[idx] - oce[idx];
@@ -908,13 +915,17 @@ orc_fs_change(orc_state *s)
                 double dfs = (dcl * (1. - km * fs[idx]) - dce) * km_inv / cl[idx];
                 cl[idx] += dcl;
                 fs[idx] += dfs;
+                int branch = (1.0 < fs[idx]) ? MMCA_BRANCH_FULL : MMCA_BRANCH_GROW;
                 fs[idx] = O_MIN(1.0, fs[idx]);
                 if (fs[idx] < 0.)
                 {
                     fs[idx] = 0.;
                     gr[idx] = 0;
                     cl[idx] = ce[idx];
+                    branch = MMCA_BRANCH_REMELT;
                 }
+                if (s->rec)
+                    s->rec[idx] = (uint8_t) branch;
             }
 }
 
@@ -1154,6 +1165,7 @@ orc_field(orc_state *s, int which)
     }
 }
 
+void orc_set_record(orc_state *s, uint8_t *rec) { s->rec = rec; }
 int orc_num_grains(const orc_state *s) { return s->num_grains; }
 int orc_gindex(const orc_state *s) { return s->gindex; }
 int orc_status(const orc_state *s) { return s->status; }

@@ -63,6 +63,8 @@ def lib():
         L.orc_timestep.argtypes = [vp]
         L.orc_failed_step.restype = C.c_uint64
         L.orc_failed_step.argtypes = [vp]
+        L.orc_set_record.argtypes = [vp, C.c_void_p]
+        L.orc_set_record.restype = None
         L.orc_set_timestep.argtypes = [vp, C.c_uint64]
         L.orc_set_timestep.restype = None
         L.orc_fnv1a.restype = C.c_uint64
@@ -146,6 +148,19 @@ class OracleSubblock:
 
     def download(self, which):
         return self.field(which).copy()
+
+    def step_recording(self, n):
+        """n steps; returns the partitioning branch of every cell per step,
+        uint8 (n, nz+2, ny+2, nx+2), halo 0, and the status."""
+        rec = np.zeros((n,) + self.shape, np.uint8)
+        st = 0
+        for i in range(n):
+            lib().orc_set_record(self._h, rec[i].ctypes.data)
+            st = lib().orc_step(self._h, 1)
+            if st:
+                break
+        lib().orc_set_record(self._h, None)
+        return rec, st
 
     def doiteration(self):
         return lib().orc_doiteration(self._h)

@@ -1,0 +1,84 @@
+"""Frozen-branch step of diffusion + partitioning in plain torch FP64 (CPU),
+differentiable with torch.autograd: the reference the HIP replay and
+adjoint kernels are checked against.  Test infrastructure only.
+
+The step follows sb_diffuse_alloy_decentered (growth.c:85-113, neighbour
+order +x,-x,+y,-y,+z,-z, zero flux into mold faces) and fs_change_diffuse
+(growth.c:221-273) with the branch of every cell given by a recorded code
+(MMCA_BRANCH_*), in the reference's operation order.  Only the uniform
+cooling temperature model (temperature.c:36-39) is implemented here.
+"""
+import torch
+
+from mumatscale_amd import abi
+
+
+def folded(params, d_liq, d_sol, k):
+    """Folded constants as functions of the differentiable parameters
+    (growth.c:38-40, 181-189)."""
+    dx = params.cell_size
+    dt = params.dt
+    dtx = dt / (dx * dx)
+    km = 1.0 - k
+    return dict(rs=d_sol * dtx, rl=d_liq * dtx, k_inv=1.0 / k, km=km, km_inv=1.0 / km,
+                m_inv=1.0 / params.m_solute[2], D_xt=d_liq / dx / dx * dt)
+
+
+def _mold(params, shape):
+    """(1 - mold) of the neighbour in each direction, over the interior."""
+    nz, ny, nx = shape
+    out = []
+    for axis, side in ((2, 1), (2, -1), (1, 1), (1, -1), (0, 1), (0, -1)):
+        pad = params.pad_bndy[2 - axis]
+        m = torch.ones(shape, dtype=torch.float64)
+        idx = [slice(None)] * 3
+        idx[axis] = -1 if side > 0 else 0
+        m[tuple(idx)] = 1 - pad
+        out.append(m)
+    return out
+
+
+def step(params, cst, code, timestep, ce, cl, fs):
+    """One frozen-branch step; all fields are padded (nz+2, ny+2, nx+2)
+    tensors; returns new padded tensors (halo unchanged)."""
+    nz, ny, nx = ce.shape[0] - 2, ce.shape[1] - 2, ce.shape[2] - 2
+
+    def at(a, dz, dy, dx):
+        return a[1 + dz:nz + 1 + dz, 1 + dy:ny + 1 + dy, 1 + dx:nx + 1 + dx]
+
+    offs = ((0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0))
+    ce_c, cl_c, fs_c = at(ce, 0, 0, 0), at(cl, 0, 0, 0), at(fs, 0, 0, 0)
+    nbsum = 0
+    for o, w in zip(offs, _mold(params, (nz, ny, nx))):
+        fs_av = 0.5 * (fs_c + at(fs, *o))
+        r = (cst["rs"] * fs_av + cst["rl"] * (1 - fs_av)) * w
+        nbsum = nbsum + r * (at(cl, *o) - cl_c)
+    ce_new = ce_c + nbsum
+    sim_time = timestep * params.dt
+    T = params.t_init - sim_time * params.cool_rate
+    c_int = params.c_init + cst["m_inv"] * (T - params.liquidus_temp)
+    dce = ce_new - ce_c
+    dcl = (c_int - cl_c) * cst["D_xt"] * 2 + dce / (1 - cst["km"] * fs_c)
+    dfs = (dcl * (1. - cst["km"] * fs_c) - dce) * cst["km_inv"] / cl_c
+    b = at(code, 0, 0, 0)
+    liquid = (b == abi.BRANCH_LIQUID) | (b == abi.BRANCH_REMELT)
+    solid = b == abi.BRANCH_SOLID
+    full = b == abi.BRANCH_FULL
+    zero = torch.zeros_like(fs_c)
+    cl_new = torch.where(liquid, ce_new, torch.where(solid, ce_new * cst["k_inv"], cl_c + dcl))
+    fs_new = torch.where(liquid, zero, torch.where(solid, fs_c, torch.where(full, zero + 1.0, fs_c + dfs)))
+    out = []
+    for a, v in ((ce, ce_new), (cl, cl_new), (fs, fs_new)):
+        o = a.clone()
+        o[1:-1, 1:-1, 1:-1] = v
+        out.append(o)
+    return tuple(out)
+
+
+def run(params, codes, t0, ce0, cl0, fs0, d_liq, d_sol, k):
+    """Final (ce, cl, fs) after len(codes) frozen-branch steps."""
+    cst = folded(params, d_liq, d_sol, k)
+    ce, cl, fs = ce0, cl0, fs0
+    for s in range(codes.shape[0]):
+        ce, cl, fs = step(params, cst, codes[s], t0 + s + 1, ce, cl, fs)
+    return ce, cl, fs

@@ -1,0 +1,277 @@
+/*
+ * mmca_run — C host driver of the CA step: reads a muMatScale control file,
+ * runs the reference's main loop on one GPU and writes its grain summary.
+ *
+ *   loop()       loop.c:186-272   timestep++, doiteration(), screen line
+ *                                 every ScreenPFreq steps, checkDone
+ *   checkDone    loop.c:148-170   FinishTime (every step) and FsFinish
+ *                                 (fs refreshed on ScreenPFreq steps)
+ *   computeFS    loop.c:173-182   solid_volume / totalNonMoldVolume
+ *   output_grains grain.c:752-833 <BaseFileName>_grains.csv and the
+ *                                 "N grains. Min ... Average" line
+ *
+ * The whole domain (NSubblocks x NCellsPerSB) runs as one block with the
+ * random stream of rank 0, so runs with NSubblocks 1 1 1 reproduce the
+ * reference; with more subblocks the random sites and orientations differ
+ * from a multi-rank reference run.  Steps between two screen or finish
+ * checks are issued as one mmca_step call.
+ *
+ * Usage: mmca_run <ctrl-file> [--device N] [--max-steps N] [--fp32]
+ *                 [--no-ce-check] [--dump-ctrl]
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include "mmca.h"
+#include "mmca_ctrl.h"
+
+static double
+wall_now(void)
+{
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return tv.tv_sec + tv.tv_usec * 1.0e-6;
+}
+
+static int
+check(int st, const char *what)
+{
+    if (st != MMCA_OK)
+        fprintf(stderr, "%s: %s\n", what, mmca_status_string(st));
+    return st;
+}
+
+/* First timestep >= t with timestep * dt >= finish_time (loop.c:154-155). */
+static uint64_t
+finish_step(uint64_t t, double dt, double finish_time)
+{
+    double g = floor(finish_time / dt) - 2.0;
+    uint64_t s = (g > (double) t) ? (uint64_t) g : t;
+    while (s * dt < finish_time)
+        s++;
+    return s;
+}
+
+static void
+dump_ctrl(const mmca_run_cfg *c, const mmca_params *p, const mmca_nucleation_cfg *n)
+{
+    printf("basefilename %s\n", c->basefilename);
+    printf("nsub %d %d %d\n", c->nsub[0], c->nsub[1], c->nsub[2]);
+    printf("finish_time %.17g\nfs_finish %.17g\nscreenpfreq %d\n", c->finish_time, c->fs_finish, c->screenpfreq);
+    printf("nx %d\nny %d\nnz %d\n", p->nx, p->ny, p->nz);
+    printf("pad_bndy %d %d %d\n", p->pad_bndy[0], p->pad_bndy[1], p->pad_bndy[2]);
+    printf("gdim %d %d %d\n", p->gdim[0], p->gdim[1], p->gdim[2]);
+    printf("cell_size %.17g\ndt %.17g\ncourant_limit %.17g\n", p->cell_size, p->dt, p->courant_limit);
+    printf("c_init %.17g\nliquidus_temp %.17g\nsolidus_temp %.17g\n", p->c_init, p->liquidus_temp,
+           p->solidus_temp);
+    printf("m_solute %.17g %.17g %.17g\n", p->m_solute[0], p->m_solute[1], p->m_solute[2]);
+    printf("part_coef %.17g\nd_sol %.17g\nd_liq %.17g\nfsgrow %.17g\n", p->part_coef, p->d_sol, p->d_liq,
+           p->fsgrow);
+    printf("temp_model %d\nam_layers %d\nam_tracks %d\ngnsbx %d\n", p->temp_model, p->am_layers, p->am_tracks,
+           p->gnsbx);
+    printf("t_init %.17g\ncool_rate %.17g\ngradient %.17g\ngrad_coef %.17g\nvelocity %.17g\nvelo_coef %.17g\n",
+           p->t_init, p->cool_rate, p->gradient, p->grad_coef, p->velocity, p->velo_coef);
+    printf("laser %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", p->laser_power, p->laser_speed,
+           p->laser_spot, p->laser_absorp, p->laser_hatch, p->sub_init_temp, p->th_conductivity, p->density,
+           p->specific_heat);
+    printf("laser_start %.17g %.17g %.17g\n", p->laser_start[0], p->laser_start[1], p->laser_start[2]);
+    printf("max_total_grains %lld\n", (long long) p->max_total_grains);
+    printf("random_seed %u\noriented %d\ndo_surface %d\n", n->random_seed, n->oriented, n->do_surface);
+    printf("bulk %.17g %.17g %.17g\nsurf %.17g %.17g %.17g\n", n->bulk_density, n->bulk_mean, n->bulk_sigma,
+           n->surf_density, n->surf_mean, n->surf_sigma);
+    for (int i = 0; i < n->n_fixed; i++)
+        printf("fixed %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", n->fixed[i].x, n->fixed[i].y, n->fixed[i].z,
+               n->fixed[i].ax, n->fixed[i].ay, n->fixed[i].az, n->fixed[i].threshold);
+    for (int i = 0; i < n->n_volumes; i++)
+        printf("volume %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", n->volumes[i].density,
+               n->volumes[i].minx, n->volumes[i].miny, n->volumes[i].minz, n->volumes[i].maxx,
+               n->volumes[i].maxy, n->volumes[i].maxz, n->volumes[i].threshold);
+}
+
+/* output_grains (grain.c:752-833) on the downloaded gr field and grain table */
+static int
+output_grains(mmca_ctx *ctx, const mmca_run_cfg *c, const mmca_params *p)
+{
+    int64_t P = 0;
+    int32_t ng = 0;
+    if (check(mmca_padded_cells(ctx, &P), "mmca_padded_cells") ||
+        check(mmca_get_grains(ctx, NULL, 0, &ng), "mmca_get_grains"))
+        return 1;
+    int32_t *gr = malloc(sizeof(int32_t) * (size_t) P);
+    mmca_grain *g = calloc((size_t) (ng > 0 ? ng : 1), sizeof(mmca_grain));
+    size_t *sizes = calloc((size_t) (ng > 0 ? ng : 1), sizeof(size_t));
+    if (!gr || !g || !sizes)
+    {
+        fprintf(stderr, "out of host memory\n");
+        return 1;
+    }
+    if (check(mmca_download_field(ctx, MMCA_FIELD_GR, gr), "mmca_download_field") ||
+        check(mmca_get_grains(ctx, g, ng, NULL), "mmca_get_grains"))
+        return 1;
+    const int sx = p->nx + 2, sxy = (p->nx + 2) * (p->ny + 2);
+    for (int z = 1; z <= p->nz; z++)          /* calculateGrainSizes (grain.c:715-745) */
+        for (int y = 1; y <= p->ny; y++)
+            for (int x = 1; x <= p->nx; x++)
+            {
+                int id = gr[(size_t) z * sxy + (size_t) y * sx + x];
+                if (id > 0 && id < ng)
+                    sizes[id]++;
+            }
+    char fname[512];
+    snprintf(fname, sizeof(fname), "%s_grains.csv", c->basefilename);
+    FILE *fp = fopen(fname, "w");
+    if (!fp)
+        fprintf(stderr, "Unable to open grain file %s for writing!\n", fname);
+    else
+        fprintf(fp, "Number,Nuc Timestep,Nuc Time,Nuc Threshold,X,Y,Z,CellX,CellY,CellZ,Rot1,Rot2,Rot3,"
+                    "Rotation Matrix,Size mm^3\n");
+    const double cell_volume = pow(c->cell_size * 1000, 3.0);       /* mm^3 */
+    size_t min_size = ng > 1 ? sizes[1] : 0, max_size = min_size, sum = 0, zero = 0;
+    for (int i = 1; i < ng; i++)
+    {
+        const mmca_grain *q = &g[i];
+        if (fp)
+        {
+            /* realcoord2scoord (functions.c:92-107) with a zero origin offset */
+            size_t cx = (size_t) ((q->nuc_x + c->cell_size * 0.1) / c->cell_size);
+            size_t cy = (size_t) ((q->nuc_y + c->cell_size * 0.1) / c->cell_size);
+            size_t cz = (size_t) ((q->nuc_z + c->cell_size * 0.1) / c->cell_size);
+            fprintf(fp, "%d,%lu,%lg,%lg,%lg,%lg,%lg,%lu,%lu,%lu,%lg,%lg,%lg,[[%lg %lg %lg][%lg %lg %lg][%lg %lg %lg]],"
+                        "%lg\n",
+                    i, (unsigned long) q->nuc_timestep, q->nuc_timestep * c->dt, q->nuc_threshold, q->nuc_x,
+                    q->nuc_y, q->nuc_z, cx, cy, cz, q->rotang[0], q->rotang[1], q->rotang[2], q->rotmat[0][0],
+                    q->rotmat[0][1], q->rotmat[0][2], q->rotmat[1][0], q->rotmat[1][1], q->rotmat[1][2],
+                    q->rotmat[2][0], q->rotmat[2][1], q->rotmat[2][2], sizes[i] * cell_volume);
+        }
+        if (sizes[i] < min_size)
+            min_size = sizes[i];
+        if (sizes[i] > max_size)
+            max_size = sizes[i];
+        sum += sizes[i];
+        if (sizes[i] == 0)
+            zero++;
+    }
+    if (fp)
+        fclose(fp);
+    printf("%d grains.  Min: %lg mm^3   Max: %lg mm^3   Average:  %lg mm^3\n", ng - 1, min_size * cell_volume,
+           max_size * cell_volume, ((double) sum / ((double) ((ng - 1) - (int) zero)) * cell_volume));
+    free(gr);
+    free(g);
+    free(sizes);
+    return 0;
+}
+
+int
+main(int argc, char **argv)
+{
+    const char *path = NULL;
+    int device = 0, fp32 = 0, ce_check = 1, dump = 0;
+    long long max_steps = -1;
+    for (int i = 1; i < argc; i++)
+    {
+        if (!strcmp(argv[i], "--device") && i + 1 < argc)
+            device = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "--max-steps") && i + 1 < argc)
+            max_steps = atoll(argv[++i]);
+        else if (!strcmp(argv[i], "--fp32"))
+            fp32 = 1;
+        else if (!strcmp(argv[i], "--no-ce-check"))
+            ce_check = 0;
+        else if (!strcmp(argv[i], "--dump-ctrl"))
+            dump = 1;
+        else if (argv[i][0] != '-' && !path)
+            path = argv[i];
+        else
+        {
+            fprintf(stderr, "usage: %s <ctrl-file> [--device N] [--max-steps N] [--fp32] [--no-ce-check] "
+                            "[--dump-ctrl]\n", argv[0]);
+            return 2;
+        }
+    }
+    if (!path)
+    {
+        fprintf(stderr, "No Control file specified.\n");
+        return 2;
+    }
+    mmca_run_cfg cfg;
+    mmca_ctrl_defaults(&cfg);
+    if (mmca_ctrl_read(path, &cfg))
+        return 1;
+    mmca_params p;
+    mmca_nucleation_cfg nuc;
+    mmca_ctrl_params(&cfg, ce_check, fp32 ? MMCA_PREC_FP32DEV : MMCA_PREC_FP64, &p);
+    mmca_ctrl_nucleation(&cfg, &nuc);
+    if (dump)
+    {
+        dump_ctrl(&cfg, &p, &nuc);
+        mmca_ctrl_free(&cfg);
+        return 0;
+    }
+    const double t_start = wall_now();
+    mmca_ctx *ctx = NULL;
+    if (check(mmca_create(&p, device, &ctx), "mmca_create") ||
+        check(mmca_init_nucleation(ctx, &nuc), "mmca_init_nucleation"))
+        return 1;
+    printf("Initial Subblocks assigned.  Starting calculations.\n");
+    const double total_volume = (p.gdim[0] * p.gdim[1] * p.gdim[2]) * pow(cfg.cell_size, 3);
+    const double cell_vol = pow(cfg.cell_size, 3.0);
+    if (cfg.screenpfreq > 0)
+        printf("%10g  %10g  %8.4lf%% %8d\n", wall_now() - t_start, 0.0, 0.0, 0);
+    uint64_t t = 0;
+    double fs = 0.0;
+    mmca_report rep;
+    memset(&rep, 0, sizeof(rep));
+    const uint64_t fin = cfg.finish_time ? finish_step(1, cfg.dt, cfg.finish_time) : 0;
+    int status = MMCA_OK;
+    for (;;)
+    {
+        uint64_t next = UINT64_MAX;
+        if (cfg.screenpfreq > 0)
+            next = (t / cfg.screenpfreq + 1) * cfg.screenpfreq;
+        if (fin && (fin > t ? fin : t + 1) < next)
+            next = fin > t ? fin : t + 1;
+        if (max_steps >= 0 && (uint64_t) max_steps < next)
+            next = (uint64_t) max_steps > t ? (uint64_t) max_steps : t + 1;
+        if (next == UINT64_MAX)
+            next = t + 1000;
+        const int screen = cfg.screenpfreq > 0 && next % cfg.screenpfreq == 0;
+        status = mmca_step(ctx, (int) (next - t), screen, &rep);
+        if (status != MMCA_OK)
+        {
+            fprintf(stderr, "step %llu: %s\n", (unsigned long long) rep.failed_step, mmca_status_string(status));
+            break;
+        }
+        t = next;
+        if (screen)
+        {
+            fs = rep.solid_sum * cell_vol / total_volume;     /* computeFS */
+            printf("%10g  %10g  %8.4lf%% %8d\n", wall_now() - t_start, t * cfg.dt, 100.0 * fs,
+                   rep.num_grains - 1);
+            fflush(stdout);
+        }
+        int done = 0;
+        if (cfg.finish_time && t * cfg.dt >= cfg.finish_time)
+        {
+            printf("\nAuto-Finish time met.\n");
+            done = 1;
+        }
+        if (fs >= cfg.fs_finish)
+        {
+            printf("\nSolid Fraction termination condition met.\n");
+            done = 1;
+        }
+        if (max_steps >= 0 && t >= (uint64_t) max_steps)
+            done = 1;
+        if (done)
+            break;
+    }
+    int rc = status == MMCA_OK ? output_grains(ctx, &cfg, &p) : 1;
+    mmca_destroy(ctx);
+    mmca_ctrl_free(&cfg);
+    if (rc == 0)
+        printf("\nSimulation complete.  Shutting down.\n");
+    return rc;
+}

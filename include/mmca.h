@@ -252,7 +252,7 @@ int  mmca_kernel_time_ms(mmca_ctx *ctx, double *avg_ms, int64_t *launches);
  * rank.  A batch holds `ndomains` subblocks of any sizes (same precision and
  * temperature-model class), each created as by mmca_create +
  * mmca_init_nucleation (nucleation may be NULL), and advances them with one
- * set of six launches per step.  Each domain keeps its own timestep, grains,
+ * set of seven launches per step.  Each domain keeps its own timestep, grains,
  * RNG stream and errors; a domain that fails stops, the others go on.
  */
 typedef struct mmca_batch mmca_batch;

@@ -1125,7 +1125,7 @@ int batch_sync(mmca_batch *bt)
     return MMCA_OK;
 }
 
-// one doiteration() of every domain: six launches whatever the batch size
+// one doiteration() of every domain: seven launches whatever the batch size
 int batch_enqueue(mmca_batch *bt, int cur)
 {
     const Consts &c0 = bt->dom[0]->c;

@@ -189,6 +189,47 @@ def run_adjoint(n=128, nseeds=16, steps=4000):
             "max_rel_err_best_h": max(min(r.values()) for r in rel.values())}
 
 
+def run_am_track(steps=160):
+    """The AM single track over a 2000-grain Voronoi substrate (1024 x 512 x
+    256, dx = 0.5 um, Gaussian-beam temperature) with FP32-deviation storage,
+    and its error against the FP64 run of the same steps.  Every solid cell
+    evaluates the 400-term beam quadrature each step (temperature.c:88-118),
+    so this workload is bound by FP64 VALU work, not HBM."""
+    import numpy as np
+    from mumatscale_amd.lib import Subblock
+    from mumatscale_amd.workloads import am_track_config, apply_substrate, voronoi_substrate
+    cfg = am_track_config()
+    t0 = time.time()
+    gr, recs = voronoi_substrate(cfg, 2000, 2)
+    log("am track: substrate built in %.1f s" % (time.time() - t0))
+    n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
+    out, fields = {}, {}
+    for prec in (abi.PREC_FP32DEV, abi.PREC_FP64):
+        with Subblock(cfg.params(check_negative_ce=False, precision=prec), cfg.nucleation()) as sb:
+            apply_substrate(sb, gr, recs)
+            sb.set_kernel_timing(True)
+            t0 = time.perf_counter()
+            rep = sb.step(steps, want_sums=True)
+            wall = time.perf_counter() - t0
+            k_ms, _ = sb.kernel_time_ms()
+            ce = sb.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1]
+            fields[prec] = (ce, rep)
+            key = "fp32dev" if prec == abi.PREC_FP32DEV else "fp64"
+            out[key] = {"value": round(n * steps / wall / 1e6, 1), "unit": "Mcell-updates/s",
+                        "ms_per_step": round(wall / steps * 1e3, 3), "k_solute_ms": round(k_ms, 3),
+                        "beam_terms_per_s": round(n * 400 / (k_ms * 1e-3), 1) if k_ms > 0 else None,
+                        "grains": rep.num_grains - 1, "solid_fraction": rep.solid_sum / n}
+    (c32, r32), (c64, r64) = fields[abi.PREC_FP32DEV], fields[abi.PREC_FP64]
+    out["error_vs_fp64"] = {
+        "solute_l2_rel": float(np.linalg.norm(c32 - c64) / np.linalg.norm(c64)),
+        "solid_fraction_abs": abs(r32.solid_sum - r64.solid_sum) / n,
+        "grains_fp32": r32.num_grains - 1, "grains_fp64": r64.num_grains - 1,
+        "solute_drift_with_ledger": (r32.solute_sum + r32.solute_ledger - n * cfg.c_init) / (n * cfg.c_init)}
+    out.update({"cells": n, "steps": steps, "dt": cfg.dt, "substrate_grains": len(recs) - 1,
+                "bound": "FP64 VALU (beam quadrature on every solid cell)"})
+    return out
+
+
 def cpu_baseline(cfg, target_s):
     """Oracle (scalar C port of the reference path), one thread, bounded sample."""
     from oracle.oracle import OracleSubblock
@@ -225,6 +266,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the FP32-deviation and batched-ensemble measurements")
     ap.add_argument("--extra-steps", type=int, default=1000)
+    ap.add_argument("--am", action="store_true",
+                    help="also run the AM single track (FP32 against FP64, 1024x512x256)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
@@ -269,6 +312,9 @@ def main():
         log("ensemble:", extra["batched_ensemble"])
         extra["adjoint"] = run_adjoint()
         log("adjoint:", extra["adjoint"])
+        if args.am:
+            extra["am_track"] = run_am_track()
+            log("am track:", extra["am_track"])
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, args.cpu_seconds)

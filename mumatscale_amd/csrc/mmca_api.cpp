@@ -372,12 +372,11 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     TRY(dev_alloc(&b.fs[0], P));
     TRY(dev_alloc(&b.fs[1], P));
     TRY(dev_alloc(&b.gr, P));
-    TRY(dev_alloc(&b.dcx, P));
-    TRY(dev_alloc(&b.dcy, P));
-    TRY(dev_alloc(&b.dcz, P));
+    TRY(dev_alloc(&b.dc, 3 * P));
     const size_t mask_words = (size_t) (c.ny + 2) * (c.nz + 2) * c.w64;
     TRY(dev_alloc(&b.mask, mask_words));
     TRY(dev_alloc(&b.diff_id, N + 2));
+    TRY(dev_alloc(&b.nbr, N + 2));
     // dc_tmp slots 0 .. N + 101 (dc_tmp_size <= N + 100, slot dc_tmp_size used)
     TRY(dev_alloc(&b.dtx, N + 102));
     TRY(dev_alloc(&b.dty, N + 102));
@@ -410,9 +409,7 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     HIPCHK(hipMemsetAsync(b.fs[0], 0, P * sizeof(double), ctx->stream));
     HIPCHK(hipMemsetAsync(b.fs[1], 0, P * sizeof(double), ctx->stream));
     HIPCHK(hipMemsetAsync(b.gr, 0, P * sizeof(int), ctx->stream));
-    HIPCHK(hipMemsetAsync(b.dcx, 0, P * sizeof(double), ctx->stream));
-    HIPCHK(hipMemsetAsync(b.dcy, 0, P * sizeof(double), ctx->stream));
-    HIPCHK(hipMemsetAsync(b.dcz, 0, P * sizeof(double), ctx->stream));
+    HIPCHK(hipMemsetAsync(b.dc, 0, 3 * P * sizeof(double), ctx->stream));
     HIPCHK(hipMemsetAsync(b.mask, 0, mask_words * sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipMemsetAsync(b.diff_id, 0, (N + 2) * sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(b.dtx, 0, (N + 102) * sizeof(double), ctx->stream));
@@ -438,8 +435,8 @@ int mmca_destroy(mmca_ctx *ctx)
     if (ctx->stream)
         (void) hipStreamSynchronize(ctx->stream);
     DevBufs &b = ctx->b;
-    void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.ce32, b.cl32[0], b.cl32[1], b.ledger, b.fs[0], b.fs[1], b.gr, b.dcx, b.dcy, b.dcz, b.mask,
-                     b.diff_id, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
+    void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.ce32, b.cl32[0], b.cl32[1], b.ledger, b.fs[0], b.fs[1], b.gr, b.dc, b.mask,
+                     b.diff_id, b.nbr, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
                      b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.fixed_rot, b.rand_rot,
                      b.oce, b.d, b.red_part, b.cnt, ctx->d_red, ctx->d_desc, ctx->d_off };
     for (void *p : ptrs)
@@ -778,17 +775,9 @@ int mmca_upload_field(mmca_ctx *ctx, int field, const void *host)
     case MMCA_FIELD_GR:
         HIPCHK(hipMemcpy(b.gr, host, P * sizeof(int), hipMemcpyHostToDevice));
         break;
-    case MMCA_FIELD_DC: {
-        const double *h = (const double *) host;
-        std::vector<double> tmp(P);
-        double *dst[3] = { b.dcx, b.dcy, b.dcz };
-        for (int a = 0; a < 3; a++) {
-            for (size_t i = 0; i < P; i++)
-                tmp[i] = h[3 * i + a];
-            HIPCHK(hipMemcpy(dst[a], tmp.data(), P * sizeof(double), hipMemcpyHostToDevice));
-        }
+    case MMCA_FIELD_DC:
+        HIPCHK(hipMemcpy(b.dc, host, 3 * P * sizeof(double), hipMemcpyHostToDevice));
         break;
-    }
     case MMCA_FIELD_OCE:
         if (ctx->c.f32)
             return MMCA_ERR_STATE;
@@ -832,17 +821,9 @@ int mmca_download_field(mmca_ctx *ctx, int field, void *host)
     case MMCA_FIELD_GR:
         HIPCHK(hipMemcpy(host, b.gr, P * sizeof(int), hipMemcpyDeviceToHost));
         break;
-    case MMCA_FIELD_DC: {
-        std::vector<double> tmp(P);
-        double *h = (double *) host;
-        double *src[3] = { b.dcx, b.dcy, b.dcz };
-        for (int a = 0; a < 3; a++) {
-            HIPCHK(hipMemcpy(tmp.data(), src[a], P * sizeof(double), hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < P; i++)
-                h[3 * i + a] = tmp[i];
-        }
+    case MMCA_FIELD_DC:
+        HIPCHK(hipMemcpy(host, b.dc, 3 * P * sizeof(double), hipMemcpyDeviceToHost));
         break;
-    }
     case MMCA_FIELD_D:
         TRY(ensure_d(ctx));
         HIPCHK(launch_grow_octahedra(ctx->b, c, ctx->cur, ctx->stream));

@@ -67,10 +67,11 @@ struct DevBufs {
     long long nledger;
     double *fs[2];
     int *gr;
-    double *dcx, *dcy, *dcz;         // decentered octahedron centres (SoA)
+    double *dc;                      // decentered octahedron centres, x,y,z per cell (decentered_t)
     unsigned long long *mask;        // 1 bit per padded cell: gr > 0
     // interface list and dc_tmp emulation
     int *diff_id;                    // slots 1..gindex
+    unsigned char *nbr;              // per slot: solid neighbours before capture, bit n = capture direction n
     double *dtx, *dty, *dtz;         // dc_tmp slots 0..capacity
     int *blk_count;                  // per-block member counts -> offsets
     int nblk_list;

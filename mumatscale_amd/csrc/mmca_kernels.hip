@@ -797,16 +797,6 @@ k_realloc_decide(DevBufs b)
         cnt->realloc_old = -1;
 }
 
-// gr > 0 before capture; the padded faces are never solid on one subblock
-__device__ __forceinline__ bool
-solid_at(const DevBufs &b, const Consts &c, int x, int j, int k)
-{
-    if (x < 1 || x > c.nx)
-        return false;
-    unsigned long long wd = mask_word(b, c, k, j, (x - 1) >> 6);
-    return (wd >> ((x - 1) & 63)) & 1ull;
-}
-
 // Interface list writer (grow_cell_reduction, growth.c:496-513): slot
 // base+i+1 of diff_id gets the padded index of the workgroup's member i, in
 // raster order.  The members of a workgroup's 256 mask words are spread
@@ -838,6 +828,7 @@ k_list_write(View V)
 {
     __shared__ int s_w[LIST_BS / 64];
     __shared__ unsigned long long s_mem[LIST_BS];
+    __shared__ unsigned long long s_nb[6][LIST_BS];   // solid bits of the 6 neighbours, per word
     __shared__ int s_ex[LIST_BS];
     const int dm = find_dom(V.off_list, V.ndom, blockIdx.x);
     const long long blk = blockIdx.x - V.off_list[dm];
@@ -849,12 +840,36 @@ k_list_write(View V)
     const long long wi0 = blk * LIST_BS;
     const long long wi = wi0 + threadIdx.x;
     unsigned long long mem = 0;
-    int k0, j0, w0;
+    unsigned long long nb[6] = { 0, 0, 0, 0, 0, 0 };
     if (wi < nwords)
-        mem = member_bits(b, c, wi, &k0, &j0, &w0);
+    {
+        const long long nrow = wi / c.w64;
+        const int w = (int) (wi - nrow * c.w64);
+        const int j = (int) (nrow % c.ny) + 1;
+        const int k = (int) (nrow / c.ny) + 1;
+        const unsigned long long own = mask_word(b, c, k, j, w);
+        // bit b of each word: the neighbour of interior x = 64 w + b + 1 in the
+        // capture order +x, +y, -x, -y, +z, -z (growth.c:604-610)
+        nb[0] = own >> 1;
+        nb[2] = own << 1;
+        if (w + 1 < c.w64)
+            nb[0] |= mask_word(b, c, k, j, w + 1) << 63;
+        if (w > 0)
+            nb[2] |= mask_word(b, c, k, j, w - 1) >> 63;
+        nb[1] = mask_word(b, c, k, j + 1, w);
+        nb[3] = mask_word(b, c, k, j - 1, w);
+        nb[4] = mask_word(b, c, k + 1, j, w);
+        nb[5] = mask_word(b, c, k - 1, j, w);
+        const int hi = c.nx - w * 64;
+        const unsigned long long inter = (hi >= 64) ? ~0ull : ((1ull << hi) - 1ull);
+        mem = ~own & (nb[0] | nb[1] | nb[2] | nb[3] | nb[4] | nb[5]) & inter;
+    }
     int tot;
     int ex = block_exscan_int<LIST_BS / 64>(__popcll(mem), &tot, s_w);
     s_mem[threadIdx.x] = mem;
+#pragma unroll
+    for (int n = 0; n < 6; ++n)
+        s_nb[n][threadIdx.x] = nb[n];
     s_ex[threadIdx.x] = ex;
     __syncthreads();
     const int base = b.blk_count[blk];            // members before this workgroup
@@ -877,7 +892,12 @@ k_list_write(View V)
         const int j = (int) (nrow % c.ny) + 1;
         const int k = (int) (nrow / c.ny) + 1;
         const int x = w * 64 + bit + 1;
+        unsigned int pattern = 0;
+#pragma unroll
+        for (int n = 0; n < 6; ++n)
+            pattern |= (unsigned int) ((s_nb[n][lo] >> bit) & 1ull) << n;
         b.diff_id[base + i + 1] = (int) (k * c.sxy + (long long) j * c.sx + x);   // slots start at 1
+        b.nbr[base + i + 1] = (unsigned char) pattern;
     }
 }
 
@@ -903,15 +923,10 @@ k_capture(View V, int cb)
     const long long maxg = c.max_total_grains;
     double *fsb = fs_buf(b, cb);
     double *clb = cl_buf(b, cb);
-    const unsigned usx = (unsigned) c.sx, usxy = (unsigned) c.sxy;
     for (long long t = blk * LIST_BS + threadIdx.x; t < nmem; t += nthr)
     {
         const int slot = (int) t + 1;
         const unsigned idx = (unsigned) b.diff_id[slot];
-        const int k = (int) (idx / usxy);
-        const unsigned r = idx - (unsigned) k * usxy;
-        const int j = (int) (r / usx);
-        const int x = (int) (r - (unsigned) j * usx);
         double tx, ty, tz;
         if (realloc)
             tx = ty = tz = 0.0;
@@ -927,35 +942,42 @@ k_capture(View V, int cb)
         const int nnz[6] = { 0, 0, 0, 0, 1, -1 };
         const long long ndxs[6] = { (long long) idx + 1, (long long) idx + c.sx, (long long) idx - 1,
                                     (long long) idx - c.sx, (long long) idx + c.sxy, (long long) idx - c.sxy };
-        // solid neighbours, then their grain ids: independent loads issued together
-        bool sol[6];
-        sol[0] = solid_at(b, c, x + 1, j, k);
-        sol[1] = solid_at(b, c, x, j + 1, k);
-        sol[2] = solid_at(b, c, x - 1, j, k);
-        sol[3] = solid_at(b, c, x, j - 1, k);
-        sol[4] = solid_at(b, c, x, j, k + 1);
-        sol[5] = solid_at(b, c, x, j, k - 1);
+        // solid neighbours before capture (recorded by the list writer); their
+        // grain ids, centres and fs are independent loads issued together
+        const unsigned int sol = b.nbr[slot];
         int gids[6];
+        double pdc[6][3], pfs[6];
 #pragma unroll
         for (int n = 0; n < 6; ++n)
-            gids[n] = sol[n] ? b.gr[ndxs[n]] : 0;
+        {
+            gids[n] = 0;
+            pdc[n][0] = pdc[n][1] = pdc[n][2] = pfs[n] = 0.0;
+            if ((sol >> n) & 1u)
+            {
+                gids[n] = b.gr[ndxs[n]];
+                pdc[n][0] = b.dc[3 * ndxs[n]];
+                pdc[n][1] = b.dc[3 * ndxs[n] + 1];
+                pdc[n][2] = b.dc[3 * ndxs[n] + 2];
+                pfs[n] = fsb[ndxs[n]];
+            }
+        }
         int found = -1;
         int gfound = 0;
         double g00 = 0, g10 = 0, g20 = 0, g01 = 0, g11 = 0, g21 = 0, g02 = 0, g12 = 0, g22 = 0;
         double dx = 0, dy = 0, dz = 0, dn = 0, ddx = 0, ddy = 0, ddz = 0;
+#pragma unroll
         for (int n = 0; n < 6; ++n)
         {
             const int gid = gids[n];
-            if (!(gid > 0 && gid < maxg))
+            if (found >= 0 || !(gid > 0 && gid < maxg))
                 continue;
-            const long long ndx = ndxs[n];
             const double *R = b.g_rot + 9 * (long long) gid;
             g00 = R[0]; g01 = R[1]; g02 = R[2];
             g10 = R[3]; g11 = R[4]; g12 = R[5];
             g20 = R[6]; g21 = R[7]; g22 = R[8];
-            ddx = b.dcx[ndx];
-            ddy = b.dcy[ndx];
-            ddz = b.dcz[ndx];
+            ddx = pdc[n][0];
+            ddy = pdc[n][1];
+            ddz = pdc[n][2];
             // grow_octahedra (growth.c:392-403)
             double rx0 = 0.0 - ddx;
             double ry0 = 0.0 - ddy;
@@ -965,7 +987,7 @@ k_capture(View V, int cb)
             double gz0 = g02 * rx0 + g12 * ry0 + g22 * rz0;
             double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
             ds1 = ref_min(ds1, 1.5);
-            dn = ds1 + fsb[ndx] * c.rev_fsgrow;
+            dn = ds1 + pfs[n] * c.rev_fsgrow;
             // growth.c:717-735
             double rx1 = -1.0 * nnx[n] - ddx;
             double ry1 = -1.0 * nny[n] - ddy;
@@ -978,7 +1000,6 @@ k_capture(View V, int cb)
             {
                 found = n;
                 gfound = gid;
-                break;
             }
         }
         if (found > -1)
@@ -1014,9 +1035,9 @@ k_capture(View V, int cb)
             b.dty[slot] = ty;
             b.dtz[slot] = tz;
         }
-        b.dcx[idx] = tx;
-        b.dcy[idx] = ty;
-        b.dcz[idx] = tz;
+        b.dc[3 * (long long) idx] = tx;
+        b.dc[3 * (long long) idx + 1] = ty;
+        b.dc[3 * (long long) idx + 2] = tz;
     }
 }
 
@@ -1122,9 +1143,9 @@ k_grow_octahedra(DevBufs b, Consts c, int cb)
     if (grid > 0 && grid < c.max_total_grains)
     {
         const double *R = b.g_rot + 9 * (long long) grid;
-        double rx0 = 0.0 - b.dcx[i];
-        double ry0 = 0.0 - b.dcy[i];
-        double rz0 = 0.0 - b.dcz[i];
+        double rx0 = 0.0 - b.dc[3 * i];
+        double ry0 = 0.0 - b.dc[3 * i + 1];
+        double rz0 = 0.0 - b.dc[3 * i + 2];
         double gx0 = R[0] * rx0 + R[3] * ry0 + R[6] * rz0;
         double gy0 = R[1] * rx0 + R[4] * ry0 + R[7] * rz0;
         double gz0 = R[2] * rx0 + R[5] * ry0 + R[8] * rz0;

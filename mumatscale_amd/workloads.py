@@ -7,6 +7,7 @@ import math
 
 import numpy as np
 
+from . import abi
 from .ctrl import Config
 
 DT_EXPLICIT = 0.9 * 1.0e-6 ** 2 / (6 * 3.0e-9)     # 0.9 dx^2 / (6 D_l) = 5e-5 s
@@ -127,3 +128,85 @@ def seeded_grains_config(n=128, nseeds=16, seed=4, dt_scale=0.5, ny=None, nz=Non
         c.fixed.append(((i + 0.5) * c.cell_size, (j + 0.5) * c.cell_size, (k + 0.5) * c.cell_size,
                         ax, ay, az, 0.0))
     return c
+
+
+def am_track_config(nx=1024, ny=512, nz=256, dt_scale=0.5):
+    """Additive-manufacturing single track over a polycrystalline substrate:
+    dx = 0.5 um, the reference's Gaussian-beam temperature model
+    (temperature.c:62-123) with power 200 W, absorptivity 0.35, speed
+    0.8 m/s, spot 130 um, conductivity 20 W/m K, density x specific heat
+    4e6 J/m^3 K (diffusivity 5e-6 m^2/s), substrate at 300 K; the beam starts
+    at the -x face, mid-width, on the top surface.  Bulk nucleation as in
+    the directional workload (1e13 m^-3, mean 5 K, sigma 0.5 K, seed 2), no
+    surface nucleation.  dt = dt_scale x 0.9 dx^2 / (6 D_l)."""
+    c = _alloy(Config(), nx, ny, nz)
+    c.cell_size = 0.5e-6
+    c.dt = 0.9 * c.cell_size ** 2 / (6 * c.d_liq) * dt_scale
+    c.random_seed = 2
+    c.thermal_ana = 1
+    c.laser.update(power=200.0, speed=0.8, spot=130e-6, absorp=0.35, hatch=50e-6, sub_init_temp=300.0,
+                   th_conductivity=20.0, density=8000.0, specific_heat=500.0,
+                   start=[0.0, ny * c.cell_size / 2, nz * c.cell_size])
+    c.am_layers = 1
+    c.am_tracks = 1
+    c.t_init = 300.0
+    c.bulk_density = 1.0e13
+    c.bulk_mean = 5.0
+    c.bulk_sigma = 0.5
+    c.surf_density = 0.0
+    c.oriented = 1
+    c.max_total_grains = 1000000
+    return c
+
+
+def voronoi_substrate(cfg, ngrains=2000, seed=2):
+    """A substrate filling the block: the interior cells of each Voronoi
+    region of `ngrains` uniform-random seed points form one grain with a
+    uniform-random orientation (normalised Gaussian quaternion), all from
+    numpy seed `seed`.  Returns (gr, records): the padded int32 gr field and
+    the grain records 0..ngrains (record 0 is the liquid).  The nearest seed
+    is found on the GPU with torch."""
+    import torch
+    nx, ny, nz = (cfg.nsub[a] * cfg.sbdim[a] for a in range(3))
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(size=(ngrains, 3)) * np.array([nx, ny, nz])
+    quats = rng.normal(size=(ngrains, 4))
+    dev = torch.device("cuda")
+    P = torch.from_numpy(pts).to(dev)
+    pn = (P * P).sum(1)
+    gr = np.zeros((nz + 2, ny + 2, nx + 2), np.int32)
+    xs = torch.arange(nx, device=dev, dtype=torch.float64) + 0.5
+    ys = torch.arange(ny, device=dev, dtype=torch.float64) + 0.5
+    Y, X = torch.meshgrid(ys, xs, indexing="ij")
+    rows = max(1, (1 << 15) // nx)                   # 32k cells x ngrains distances per chunk
+    for k in range(nz):
+        plane = torch.empty((ny, nx), dtype=torch.int32, device=dev)
+        for j0 in range(0, ny, rows):
+            xq, yq = X[j0:j0 + rows].reshape(-1), Y[j0:j0 + rows].reshape(-1)
+            Q = torch.stack([xq, yq, torch.full_like(xq, k + 0.5)], 1)
+            d = pn[None, :] - 2.0 * (Q @ P.T)        # |q|^2 is the same for every seed
+            plane[j0:j0 + rows] = (d.argmin(1) + 1).to(torch.int32).reshape(-1, nx)
+        gr[k + 1, 1:-1, 1:-1] = plane.cpu().numpy()
+    recs = [abi.Grain()]
+    for i in range(ngrains):
+        q = quats[i] / np.linalg.norm(quats[i])
+        R = _quat_matrix(q)
+        g = abi.Grain()
+        g.nuc_x, g.nuc_y, g.nuc_z = (float(v) * cfg.cell_size for v in pts[i])
+        g.nuc_timestep = 0
+        g.nuc_threshold = 0.0
+        for a in range(3):
+            for b in range(3):
+                g.rotmat[a][b] = R[a][b]
+        g.rotang[0], g.rotang[1], g.rotang[2] = (math.radians(v) for v in _zxz_degrees(R))
+        recs.append(g)
+    return gr, recs
+
+
+def apply_substrate(sb, gr, records):
+    """Make every interior cell of `sb` solid (fs = 1) in the grain of `gr`."""
+    sb.set_grains(records)
+    sb.upload(abi.FIELD_GR, gr)
+    fs = np.zeros(gr.shape)
+    fs[1:-1, 1:-1, 1:-1] = 1.0
+    sb.upload(abi.FIELD_FS, fs)

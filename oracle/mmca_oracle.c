@@ -1166,6 +1166,17 @@ orc_field(orc_state *s, int which)
 }
 
 void orc_set_record(orc_state *s, uint8_t *rec) { s->rec = rec; }
+/* grain_cache records 0..n-1 and bp->num_grains, as a restart restores them
+ * (addGrainCache, checkpoint.c:676-745) */
+int
+orc_set_grains(orc_state *s, const mmca_grain *g, int n)
+{
+    if (n < 1 || n > s->p.max_total_grains + 1)
+        return MMCA_ERR_ARG;
+    memcpy(s->grains, g, sizeof(mmca_grain) * (size_t) n);
+    s->num_grains = n;
+    return MMCA_OK;
+}
 int orc_num_grains(const orc_state *s) { return s->num_grains; }
 int orc_gindex(const orc_state *s) { return s->gindex; }
 int orc_status(const orc_state *s) { return s->status; }

@@ -63,6 +63,7 @@ def lib():
         L.orc_timestep.argtypes = [vp]
         L.orc_failed_step.restype = C.c_uint64
         L.orc_failed_step.argtypes = [vp]
+        L.orc_set_grains.argtypes = [vp, C.POINTER(abi.Grain), C.c_int]
         L.orc_set_record.argtypes = [vp, C.c_void_p]
         L.orc_set_record.restype = None
         L.orc_set_timestep.argtypes = [vp, C.c_uint64]
@@ -148,6 +149,15 @@ class OracleSubblock:
 
     def download(self, which):
         return self.field(which).copy()
+
+    def set_grains(self, grains):
+        arr = (abi.Grain * len(grains))(*grains)
+        st = lib().orc_set_grains(self._h, arr, len(grains))
+        if st:
+            raise abi.MmcaError(st, "orc_set_grains")
+
+    def upload(self, which, arr):
+        self.field(which)[...] = arr
 
     def step_recording(self, n):
         """n steps; returns the partitioning branch of every cell per step,

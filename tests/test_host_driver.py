@@ -85,7 +85,10 @@ def test_one_grain_through_c_driver(tmp_path):
     assert r.returncode == 0
     lines = [l for l in r.stdout.splitlines() if "Average" in l]
     assert lines == ["1 grains.  Min: 0.000737875 mm^3   Max: 0.000737875 mm^3   Average:  0.000737875 mm^3"]
-    assert "Auto-Finish time met." in r.stdout
+    # FsFinish 0.01 ends the run at step 9000 (t = 9 s), as in the reference
+    assert "Solid Fraction termination condition met." in r.stdout
+    screen = [l.split() for l in r.stdout.splitlines() if "%" in l]
+    assert float(screen[-1][1]) == 9.0
     cfg = read_ctrl(path)
     csv = (tmp_path / ("%s_grains.csv" % cfg.basefilename)).read_text().splitlines()
     assert csv[0].startswith("Number,Nuc Timestep")

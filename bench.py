@@ -38,28 +38,54 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_solute_zm_directional_256x256x512_fp64.md")
+
+
+def measured_traffic(size, zm, rows, xcd):
+    """HBM bytes per launch of the solute kernel from the committed
+    FETCH_SIZE/WRITE_SIZE summary, when this run has the shape it was
+    measured on (None otherwise)."""
+    import re
+    if (size, zm, rows, xcd) != ("256x256x512", 64, 16, 1) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as fh:
+        m = re.search(r"traffic_bytes_per_launch = ([0-9.eE+]+)", fh.read())
+    return float(m.group(1)) if m else None
+
+
 def run_ensemble(steps, baseline_steps):
-    """Batched FP32-deviation step of the ensemble against the same domains
-    stepped one at a time in plain FP64."""
+    """Batched FP32-deviation step of the 96-domain ensemble (graph replay,
+    autotuned tile) against the same domains stepped one at a time in plain
+    FP64 (throughput), and against an FP64 batch of the same steps (grain
+    counts and solute balance per domain).  dt is half the explicit limit of
+    the single-grain physics, where the partitioning update is stable."""
     from mumatscale_amd.lib import Batch, Subblock, autotune_batch
     cfgs = ensemble_configs()
+    for c in cfgs:
+        c.dt *= 0.5
     cells = sum(c.sbdim[0] * c.sbdim[1] * c.sbdim[2] for c in cfgs)
     p32 = [c.params(check_negative_ce=False, precision=abi.PREC_FP32DEV) for c in cfgs]
+    p64 = [c.params(check_negative_ce=False) for c in cfgs]
     nuc = [c.nucleation() for c in cfgs]
     t0 = time.time()
     planes, rows, tms = autotune_batch(p32, nuc, warmup=10, trial_steps=20)
     log("ensemble: autotuner picked planes=%d rows=%d (%.3f ms) in %.1f s" % (planes, rows, tms, time.time() - t0))
-    with Batch(p32, nuc) as bt:
-        bt.set_tuning(abi.TUNE_ZM_PLANES, planes)
-        bt.set_tuning(abi.TUNE_ZM_ROWS, rows)
-        bt.set_tuning(abi.TUNE_GRAPH, 1)
-        bt.step(10)
-        t0 = time.perf_counter()
-        reps = bt.step(steps, want_sums=True)
-        wall = time.perf_counter() - t0
-        drift = max(abs(r.solute_sum + r.solute_ledger - (c.sbdim[0] * c.sbdim[1] * c.sbdim[2]) * c.c_init)
-                    / ((c.sbdim[0] * c.sbdim[1] * c.sbdim[2]) * c.c_init) for r, c in zip(reps, cfgs))
-    batched = cells * steps / wall / 1e6
+    reps = {}
+    walls = {}
+    for prec, params in (("fp32", p32), ("fp64", p64)):
+        with Batch(params, nuc) as bt:
+            bt.set_tuning(abi.TUNE_ZM_PLANES, planes)
+            bt.set_tuning(abi.TUNE_ZM_ROWS, rows)
+            bt.set_tuning(abi.TUNE_GRAPH, 1)
+            t0 = time.perf_counter()
+            reps[prec] = bt.step(steps, want_sums=True)
+            walls[prec] = time.perf_counter() - t0
+    ncell = [c.sbdim[0] * c.sbdim[1] * c.sbdim[2] for c in cfgs]
+    drift = max(abs(r.solute_sum + r.solute_ledger - n * c.c_init) / (n * c.c_init)
+                for r, c, n in zip(reps["fp32"], cfgs, ncell))
+    grains_equal = sum(a.num_grains == b.num_grains for a, b in zip(reps["fp32"], reps["fp64"]))
+    fs_err = max(abs(a.solid_sum - b.solid_sum) / n for a, b, n in zip(reps["fp32"], reps["fp64"], ncell))
+    batched = cells * steps / walls["fp32"] / 1e6
     # the same domains one at a time, plain FP64
     t_one = 0.0
     for c in cfgs:
@@ -69,42 +95,56 @@ def run_ensemble(steps, baseline_steps):
             sb.step(baseline_steps)
             t_one += time.perf_counter() - t0
     one = cells * baseline_steps / t_one / 1e6
-    return {"domains": len(cfgs), "cells": cells, "steps": steps, "dtype": "f32-deviation storage, f64 arithmetic",
-            "value": round(batched, 1), "unit": "Mcell-updates/s", "ms_per_step": round(wall / steps * 1e3, 4),
+    return {"domains": len(cfgs), "cells": cells, "steps": steps, "dt": cfgs[0].dt,
+            "dtype": "f32-deviation storage, f64 arithmetic",
+            "value": round(batched, 1), "unit": "Mcell-updates/s", "ms_per_step": round(walls["fp32"] / steps * 1e3, 4),
             "tile": {"zm_planes": planes, "zm_rows": rows}, "hip_graph": True,
             "one_at_a_time_fp64": round(one, 1), "one_at_a_time_steps": baseline_steps,
             "speedup_vs_one_at_a_time_fp64": round(batched / one, 2),
-            "max_solute_drift_with_ledger": drift}
+            "batched_fp64": round(cells * steps / walls["fp64"] / 1e6, 1),
+            "max_solute_drift_with_ledger": drift,
+            "domains_with_equal_grain_count_vs_fp64": grains_equal,
+            "grains_fp32": sum(r.num_grains - 1 for r in reps["fp32"]),
+            "grains_fp64": sum(r.num_grains - 1 for r in reps["fp64"]),
+            "max_solid_fraction_err_vs_fp64": fs_err}
 
 
 def run_fp32_variant(cfg, steps, warmup, zm, rows, xcd):
-    """The headline workload with FP32-deviation storage; errors against the
-    FP64 run after the same steps."""
+    """The headline workload with FP32-deviation storage: throughput at the
+    headline dt, and its error against FP64 after the same steps at half
+    that dt (at the explicit limit the partitioning update is unstable and
+    two precisions diverge like two chaotic runs)."""
     import numpy as np
     from mumatscale_amd.lib import Subblock
     n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
     res = {}
+    with Subblock(cfg.params(check_negative_ce=False, precision=abi.PREC_FP32DEV), cfg.nucleation()) as sb:
+        sb.set_tuning(abi.TUNE_ZM_PLANES, zm)
+        sb.set_tuning(abi.TUNE_ZM_ROWS, rows)
+        sb.set_tuning(abi.TUNE_XCD_REMAP, xcd)
+        sb.step(warmup)
+        sb.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        sb.step(steps)
+        wall = time.perf_counter() - t0
+        k_ms, _ = sb.kernel_time_ms()
+    res = {"value": round(n * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+           "k_solute_ms": round(k_ms, 4),
+           "roofline_frac": round(n * 40.0 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "bytes_per_cell": 40.0}
+    half = directional_config(*cfg.sbdim)
+    half.dt = cfg.dt * 0.5
     fields = {}
     for prec in (abi.PREC_FP32DEV, abi.PREC_FP64):
-        with Subblock(cfg.params(check_negative_ce=False, precision=prec), cfg.nucleation()) as sb:
+        with Subblock(half.params(check_negative_ce=False, precision=prec), half.nucleation()) as sb:
             sb.set_tuning(abi.TUNE_ZM_PLANES, zm)
             sb.set_tuning(abi.TUNE_ZM_ROWS, rows)
-            sb.set_tuning(abi.TUNE_XCD_REMAP, xcd)
-            sb.step(warmup)
-            sb.set_kernel_timing(prec == abi.PREC_FP32DEV)
-            t0 = time.perf_counter()
             rep = sb.step(steps, want_sums=True)
-            wall = time.perf_counter() - t0
-            k_ms, _ = sb.kernel_time_ms()
             fields[prec] = (sb.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1], rep)
-            if prec == abi.PREC_FP32DEV:
-                res = {"value": round(n * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
-                       "k_solute_ms": round(k_ms, 4),
-                       "roofline_frac": round(n * 40.0 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                       "bytes_per_cell": 40.0}
     ce32, r32 = fields[abi.PREC_FP32DEV]
     ce64, r64 = fields[abi.PREC_FP64]
     res["error_vs_fp64"] = {
+        "dt": half.dt, "steps": steps,
         "solute_l2_rel": float(np.linalg.norm(ce32 - ce64) / np.linalg.norm(ce64)),
         "solid_fraction_abs": abs(r32.solid_sum - r64.solid_sum) / n,
         "grains_fp32": r32.num_grains - 1, "grains_fp64": r64.num_grains - 1,
@@ -300,7 +340,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic(args.size, args.zm, args.zm_rows, args.xcd),
             "kernel": ("k_solute_zm<%d>" % args.zm_rows) if args.zm > 0 else "k_solute<true,true>",
             "bytes_per_cell": B_ALG_FP64}
     gpu.close()

@@ -9,6 +9,19 @@
 
 namespace mmca {
 
+// Device buffers reached through the domain table are generic pointers to
+// the compiler, whose loads and stores are then FLAT instructions: 64-bit
+// per-lane addresses, and a wait for LDS results also waits for them.  G(p)
+// states that p points to global memory, so that global (and, for uniform
+// addresses, scalar) loads and stores are used.
+#define MMCA_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ MMCA_GLOBAL T *G(T *p)
+{
+    return (MMCA_GLOBAL T *) p;
+}
+
 // MIN(x,y) of globals.h:19
 __device__ __forceinline__ double ref_min(double x, double y) { return (x < y) ? x : y; }
 
@@ -88,6 +101,100 @@ temp_model(const Consts &c, double sim_time, int x, int y, int z)
                 (lz * lz / (4. * lalpha * lt))) / Tlocal;
     }
     double Tfront = c.lab * lpow * lalpha / (pi15 * c.lthcon);
+    return c.liniT + Tfront * Tinteg;
+}
+
+// The Gaussian-beam quadrature of temp_model<true> with the factors of its
+// 400 terms that do not depend on the cell tabulated (same operations and
+// order as temp_model<true>, so the same values):
+//   vlt[i] = lv * lt,  A[i] = ls*ls + 4.*lalpha*lt,
+//   Tl[i] = sqrt(lalpha*lt) * (ls*ls + 4.*lalpha*lt),
+//   Z[i] = lz*lz / (4.*lalpha*lt) for one plane (lz depends on z only).
+constexpr int BEAM_NT = 400;
+struct BeamTab {
+    double vlt[BEAM_NT], A[BEAM_NT], Tl[BEAM_NT], Z[BEAM_NT];
+};
+
+__device__ __forceinline__ double beam_lalpha(const Consts &c) { return c.lthcon / (c.rho * c.lcp); }
+
+__device__ __forceinline__ double
+beam_ldt(const Consts &c)
+{
+    const double invlv = 1. / c.lv;
+    const double tsteady = 20 * beam_lalpha(c) * invlv * invlv;
+    return tsteady / BEAM_NT;
+}
+
+// fills vlt, A, Tl; threads tid = 0..nthreads-1 of the workgroup share the work
+__device__ __forceinline__ void
+beam_tab_terms(const Consts &c, BeamTab &t, int tid, int nthreads)
+{
+    const double ls = c.lsd / (2. * 1.414);
+    const double lalpha = beam_lalpha(c);
+    const double ldt = beam_ldt(c);
+    for (int i = tid; i < BEAM_NT; i += nthreads)
+    {
+        const double lt = (0.5 + i) * ldt;
+        t.vlt[i] = c.lv * lt;
+        t.A[i] = ls * ls + 4. * lalpha * lt;
+        t.Tl[i] = sqrt(lalpha * lt) * (ls * ls + 4. * lalpha * lt);
+    }
+}
+
+// fills Z for padded plane z
+__device__ __forceinline__ void
+beam_tab_plane(const Consts &c, BeamTab &t, int z, int tid, int nthreads)
+{
+    const double lalpha = beam_lalpha(c);
+    const double ldt = beam_ldt(c);
+    const double rz = c.origin[2] + ((c.off[2] + z) + 0.5) * c.cell_size;
+    const double lz = rz - c.z0;
+    for (int i = tid; i < BEAM_NT; i += nthreads)
+    {
+        const double lt = (0.5 + i) * ldt;
+        t.Z[i] = lz * lz / (4. * lalpha * lt);
+    }
+}
+
+// temp_model<true> on the tables (Z of plane z must be filled)
+__device__ __forceinline__ double
+beam_temp_tab(const Consts &c, const BeamTab &t, double sim_time, int x, int y, int z)
+{
+    double rx = c.origin[0] + ((c.off[0] + x) + 0.5) * c.cell_size;
+    double ry = c.origin[1] + ((c.off[1] + y) + 0.5) * c.cell_size;
+    double rz = c.origin[2] + ((c.off[2] + z) + 0.5) * c.cell_size;
+    double lv = c.lv;
+    double invlv = 1. / lv;
+    double xlen = c.gnsbx * c.nx * c.cell_size;
+    xlen = xlen + c.lsd;
+    double tline = xlen * invlv;
+    double tpause = 1.5e-4 * invlv;
+    tline = tline + tpause;
+    double trackno = 0;
+    if (c.am_tracks > 1)
+        trackno = floor(sim_time / tline);
+    double xnow = c.x0 + (sim_time - trackno * tline) * lv;
+    double ynow = c.y0 + trackno * c.lhs;
+    double znow = c.z0;
+    double lpow = c.lpow;
+    double lx = rx - xnow;
+    double ly = ry - ynow;
+    if (!(rz <= znow + c.cell_size))
+        return 0.0;
+    double pi15 = 5.568328;
+    double lalpha = beam_lalpha(c);
+    double ldt = beam_ldt(c);
+    if (sim_time < 1e-4)
+        lpow = 0;
+    double Tfront = c.lab * lpow * lalpha / (pi15 * c.lthcon);
+    if (Tfront == 0.0)
+        return c.liniT + Tfront * 0.0;     // every term is finite, so Tfront * Tinteg is this
+    double Tinteg = 0.0;
+    for (int i = 0; i < BEAM_NT; i++)
+    {
+        const double u = lx + t.vlt[i];
+        Tinteg += ldt * exp(-(u * u + ly * ly) / t.A[i] - t.Z[i]) / t.Tl[i];
+    }
     return c.liniT + Tfront * Tinteg;
 }
 

@@ -28,9 +28,9 @@ namespace mmca {
 
 // Ping-pong buffer `buf` (0/1) of a field, chosen with a select rather than an
 // indexed load, so that a private copy of DevBufs stays in registers.
-__device__ __forceinline__ double *fs_buf(const DevBufs &b, int buf) { return buf ? b.fs[1] : b.fs[0]; }
-__device__ __forceinline__ double *cl_buf(const DevBufs &b, int buf) { return buf ? b.cl[1] : b.cl[0]; }
-__device__ __forceinline__ float *cl32_buf(const DevBufs &b, int buf) { return buf ? b.cl32[1] : b.cl32[0]; }
+__device__ __forceinline__ MMCA_GLOBAL double *fs_buf(const DevBufs &b, int buf) { return G(buf ? b.fs[1] : b.fs[0]); }
+__device__ __forceinline__ MMCA_GLOBAL double *cl_buf(const DevBufs &b, int buf) { return G(buf ? b.cl[1] : b.cl[0]); }
+__device__ __forceinline__ MMCA_GLOBAL float *cl32_buf(const DevBufs &b, int buf) { return G(buf ? b.cl32[1] : b.cl32[0]); }
 
 // Solute storage: FP64 values, or FP32 deviations from c0 (F32).  Arithmetic
 // is FP64 in both cases.
@@ -38,7 +38,7 @@ template <bool F32>
 __device__ __forceinline__ double
 ld_ce(const DevBufs &b, const Consts &c, long long i)
 {
-    return F32 ? c.cinit + (double) b.ce32[i] : b.ce[i];
+    return F32 ? c.cinit + (double) G(b.ce32)[i] : G(b.ce)[i];
 }
 
 template <bool F32>
@@ -57,10 +57,10 @@ st_ce(const DevBufs &b, const Consts &c, long long i, double v)
     {
         const double dev = v - c.cinit;
         const float f = (float) dev;
-        b.ce32[i] = f;
+        G(b.ce32)[i] = f;
         return dev - (double) f;
     }
-    b.ce[i] = v;
+    G(b.ce)[i] = v;
     return 0.0;
 }
 
@@ -78,12 +78,12 @@ st_cl(const DevBufs &b, const Consts &c, int buf, long long i, double v)
 __device__ __forceinline__ void
 ledger_add(const DevBufs &b, long long slot, double v)
 {
-    double s = b.ledger[2 * slot], comp = b.ledger[2 * slot + 1];
+    double s = G(b.ledger)[2 * slot], comp = G(b.ledger)[2 * slot + 1];
     double y = v - comp;
     double t = s + y;
     comp = (t - s) - y;
-    b.ledger[2 * slot] = t;
-    b.ledger[2 * slot + 1] = comp;
+    G(b.ledger)[2 * slot] = t;
+    G(b.ledger)[2 * slot + 1] = comp;
 }
 
 // ---------------------------------------------------------------------------
@@ -171,7 +171,7 @@ k_nuc_detect(View V, int advance, int cur)
     const long long blk = blockIdx.x - V.off_nuc[d];
     const DevBufs b = V.dom[d].b;    // private copy: stores cannot alias it
     const Consts c = V.dom[d].c;
-    Counters *cnt = b.cnt;
+    const MMCA_GLOBAL Counters *cnt = G(b.cnt);
     if (cnt->status != MMCA_OK)
         return;
     const unsigned long long ts = cnt->timestep + (advance ? 1ull : 0ull);
@@ -180,7 +180,7 @@ k_nuc_detect(View V, int advance, int cur)
     int flag = 0;
     if (s < b.nsites)
     {
-        int idx = b.site_idx[s];
+        int idx = G(b.site_idx)[s];
         int z = (int) (idx / c.sxy);
         int rem = (int) (idx - z * c.sxy);
         int y = rem / c.sx;
@@ -191,10 +191,10 @@ k_nuc_detect(View V, int advance, int cur)
         {
             // tempUpdateSB_int sets lsindex before nucleation (temperature.c:256-259)
             if (T > c.Tliq)
-                b.site_ls[s] = 1;
+                G(b.site_ls)[s] = 1;
             if (c.am_layers == 1)
                 layerno = 1;
-            else if (c.am_layers > 1 && b.site_ls[s] == 1)
+            else if (c.am_layers > 1 && G(b.site_ls)[s] == 1)
                 layerno = 1;
             else
                 layerno = 0;
@@ -205,21 +205,21 @@ k_nuc_detect(View V, int advance, int cur)
             else
                 initnuc = 0;
         }
-        if (b.gr[idx] <= 0 && layerno && initnuc)
+        if (G(b.gr)[idx] <= 0 && layerno && initnuc)
         {
             double Tunder = c.Tliq - T;
             double m_solute0 = c.ma * T * T + c.mb * T + c.mc;
             Tunder += m_solute0 * ((c.f32 ? ld_cl<true>(b, c, cur, idx) : ld_cl<false>(b, c, cur, idx)) - c.cinit);
-            if (Tunder > (double) b.site_thr[s])
+            if (Tunder > (double) G(b.site_thr)[s])
                 flag = 1;
         }
     }
     int tot;
     int r = block_exscan_flag<NUC_BS / 64>(flag, &tot, s_w);
     if (flag)
-        b.ev_blk[blk * NUC_BS + r] = s;
+        G(b.ev_blk)[blk * NUC_BS + r] = s;
     if (threadIdx.x == 0)
-        b.ev_cnt[blk] = tot;
+        G(b.ev_cnt)[blk] = tot;
 }
 
 // Event ordering and activateNewGrains/createGrains on one rank
@@ -245,11 +245,11 @@ k_nuc_activate(View V, int mode)
         for (int base = 0; base < nblk; base += NUC_BS)
         {
             int blk = base + tid;
-            int v = (blk < nblk) ? b.ev_cnt[blk] : 0;
+            int v = (blk < nblk) ? G(b.ev_cnt)[blk] : 0;
             int tot;
             int off = carry + block_exscan_int<NUC_BS / 64>(v, &tot, s_w);
             for (int i = 0; i < v; ++i)
-                b.events[off + i] = b.ev_blk[blk * NUC_BS + i];
+                G(b.events)[off + i] = G(b.ev_blk)[blk * NUC_BS + i];
             carry += tot;
         }
         if (tid == 0)
@@ -272,8 +272,8 @@ k_nuc_activate(View V, int mode)
         {
             int e = base + tid;
             int isev = e < nev;
-            int s = isev ? b.events[e] : 0;
-            int fixed = isev ? b.site_fixed[s] : -1;
+            int s = isev ? G(b.events)[e] : 0;
+            int fixed = isev ? G(b.site_fixed)[s] : -1;
             int israndom = isev && fixed < 0;
             int rtot;
             int rrank = block_exscan_flag<NUC_BS / 64>(israndom, &rtot, s_w);
@@ -282,8 +282,8 @@ k_nuc_activate(View V, int mode)
                 long long id = (long long) base_id + e;
                 if (id <= maxg)
                 {
-                    int idx = b.site_idx[s];
-                    b.gr[idx] = (int) id;
+                    int idx = G(b.site_idx)[s];
+                    G(b.gr)[idx] = (int) id;
                     const double *rot;
                     int src;
                     if (fixed >= 0)
@@ -299,9 +299,9 @@ k_nuc_activate(View V, int mode)
                     }
                     for (int q = 0; q < 9; ++q)
                         b.g_rot[9 * id + q] = rot[q];
-                    b.g_cell[id] = idx;
-                    b.g_step[id] = ts;
-                    b.g_src[id] = src;
+                    G(b.g_cell)[id] = idx;
+                    G(b.g_step)[id] = ts;
+                    G(b.g_src)[id] = src;
                 }
             }
             rused += rtot;
@@ -422,7 +422,7 @@ k_solute(View V, int cur, int nxt)
     // this kernel itself may raise the status: read it once per workgroup so
     // that every wave takes the same branch before the barrier of block_sum
     if (threadIdx.x == 0 && threadIdx.y == 0)
-        s_stop = (cnt->status != MMCA_OK);
+        s_stop = (G(b.cnt)->status != MMCA_OK);
     __syncthreads();
     if (s_stop)
         return;
@@ -436,7 +436,7 @@ k_solute(View V, int cur, int nxt)
     if (active)
     {
         const long long idx = k * c.sxy + (long long) j * c.sx + x;
-        const double *fsc = fs_buf(b, cur);
+        const MMCA_GLOBAL double *fsc = fs_buf(b, cur);
         const double ce_old = ld_ce<F32>(b, c, idx);
         const double conc = ld_cl<F32>(b, c, cur, idx);
         const double lspfs = fsc[idx];
@@ -462,23 +462,23 @@ k_solute(View V, int cur, int nxt)
         }
         if (FSC)
         {
-            const double ce_prev = DIFF ? ce_old : b.oce[idx];
-            const int g_old = b.gr[idx];
+            const double ce_prev = DIFF ? ce_old : G(b.oce)[idx];
+            const int g_old = G(b.gr)[idx];
             double cl_out, fs_out;
             int g, branch;
-            partition_cell<AM>(c, cnt->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
+            partition_cell<AM>(c, G(b.cnt)->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
                                &cl_out, &fs_out, &g, &branch);
             if (REC)
             {
                 // branch record of this step (bounds: set by the host, checked here too)
-                const long long rs = (long long) (cnt->timestep - b.rec_t0) - 1;
+                const long long rs = (long long) (G(b.cnt)->timestep - b.rec_t0) - 1;
                 if (rs >= 0 && rs < b.rec_steps)
-                    b.rec[rs * c.P + idx] = (unsigned char) branch;
+                    G(b.rec)[rs * c.P + idx] = (unsigned char) branch;
             }
             st_cl<F32>(b, c, nxt, idx, cl_out);
             fs_buf(b, nxt)[idx] = fs_out;
             if (g != g_old)
-                b.gr[idx] = g;
+                G(b.gr)[idx] = g;
             g_new = g;
         }
     }
@@ -486,7 +486,7 @@ k_solute(View V, int cur, int nxt)
     {
         unsigned long long bits = __ballot(active && g_new > 0);
         if (threadIdx.x == 0 && rowok)
-            b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
+            G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
     }
     if (F32)
     {
@@ -530,7 +530,7 @@ k_solute_zm(View V, int cur, int nxt)
     // this kernel itself may raise the status: read it once per workgroup so
     // that every wave takes the same branch before the barriers below
     if (threadIdx.x == 0 && threadIdx.y == 0)
-        s_stop = (cnt->status != MMCA_OK);
+        s_stop = (G(b.cnt)->status != MMCA_OK);
     __syncthreads();
     if (s_stop)
         return;
@@ -546,17 +546,18 @@ k_solute_zm(View V, int cur, int nxt)
     const int k1 = min(c.nz, k0 + kz - 1);
     const bool rowok = (j <= c.ny);
     const bool active = rowok && (x <= c.nx);
-    const double *fsc = fs_buf(b, cur);
-    double *fsn_out = fs_buf(b, nxt);
+    const MMCA_GLOBAL double *fsc = fs_buf(b, cur);
+    MMCA_GLOBAL double *fsn_out = fs_buf(b, nxt);
     const long long col = active ? ((long long) j * c.sx + x) : 0;
-    const double sim_time = cnt->timestep * c.dt;
+    const double sim_time = G(b.cnt)->timestep * c.dt;
     const bool ylo_lds = (ty > 0);
     const bool yhi_lds = (ty + 1 < BY) && (j + 1 <= c.ny);
     const bool xlo_mem = (lane == 0);
     const bool xhi_mem = (lane == 63) || (x == c.nx);
     const int mo_x1 = (x == c.nx) ? c.pad[0] : 0, mo_x0 = (x == 1) ? c.pad[0] : 0;
     const int mo_y1 = (j == c.ny) ? c.pad[1] : 0, mo_y0 = (j == 1) ? c.pad[1] : 0;
-    double clm = 0, clv = 0, clp = 0, fsm = 0, fsv = 0, fsp = 0;
+    // cl/fs of planes k-1, k, k+1 in registers; plane k+2 is in flight
+    double clm = 0, clv = 0, clp = 0, clq = 0, fsm = 0, fsv = 0, fsp = 0, fsq = 0;
     double ce_nx = 0;
     int g_nx = 0;
     double lost = 0.0;
@@ -566,33 +567,30 @@ k_solute_zm(View V, int cur, int nxt)
         fsm = fsc[(k0 - 1) * c.sxy + col];
         clv = ld_cl<F32>(b, c, cur, k0 * c.sxy + col);
         fsv = fsc[k0 * c.sxy + col];
+        clp = ld_cl<F32>(b, c, cur, (k0 + 1) * c.sxy + col);
+        fsp = fsc[(k0 + 1) * c.sxy + col];
         ce_nx = ld_ce<F32>(b, c, k0 * c.sxy + col);
-        g_nx = b.gr[k0 * c.sxy + col];
+        g_nx = G(b.gr)[k0 * c.sxy + col];
     }
     for (int k = k0; k <= k1; ++k)
     {
         const long long idx = k * c.sxy + col;
         const double ce_old = ce_nx;
         const int g_old = g_nx;
+        double clxm = 0, fsxm = 0, clxp = 0, fsxp = 0, clym = 0, fsym = 0, clyp = 0, fsyp = 0;
         if (active)
         {
-            // next plane's values are loaded one iteration ahead
-            clp = ld_cl<F32>(b, c, cur, idx + c.sxy);
-            fsp = fsc[idx + c.sxy];
-            if (k < k1)
+            // loads for later planes first: plane k+2 (cl, fs; up to the halo
+            // plane k1+1) and plane k+1 (ce, gr)
+            if (k + 1 <= k1)
             {
+                clq = ld_cl<F32>(b, c, cur, idx + 2 * c.sxy);
+                fsq = fsc[idx + 2 * c.sxy];
                 ce_nx = ld_ce<F32>(b, c, idx + c.sxy);
-                g_nx = b.gr[idx + c.sxy];
+                g_nx = G(b.gr)[idx + c.sxy];
             }
-        }
-        s_cl[ty][lane] = clv;
-        s_fs[ty][lane] = fsv;
-        double clxm = __shfl_up(clv, 1, 64), fsxm = __shfl_up(fsv, 1, 64);
-        double clxp = __shfl_down(clv, 1, 64), fsxp = __shfl_down(fsv, 1, 64);
-        __syncthreads();
-        int g_new = 0;
-        if (active)
-        {
+            // tile-edge neighbours of plane k come from memory; issued before
+            // the barrier so that their latency overlaps it
             if (xlo_mem)
             {
                 clxm = ld_cl<F32>(b, c, cur, idx - 1);
@@ -603,26 +601,44 @@ k_solute_zm(View V, int cur, int nxt)
                 clxp = ld_cl<F32>(b, c, cur, idx + 1);
                 fsxp = fsc[idx + 1];
             }
-            double clym, fsym, clyp, fsyp;
+            if (!ylo_lds)
+            {
+                clym = ld_cl<F32>(b, c, cur, idx - c.sx);
+                fsym = fsc[idx - c.sx];
+            }
+            if (!yhi_lds)
+            {
+                clyp = ld_cl<F32>(b, c, cur, idx + c.sx);
+                fsyp = fsc[idx + c.sx];
+            }
+        }
+        s_cl[ty][lane] = clv;
+        s_fs[ty][lane] = fsv;
+        const double shxm = __shfl_up(clv, 1, 64), sfxm = __shfl_up(fsv, 1, 64);
+        const double shxp = __shfl_down(clv, 1, 64), sfxp = __shfl_down(fsv, 1, 64);
+        __syncthreads();
+        int g_new = 0;
+        if (active)
+        {
+            if (!xlo_mem)
+            {
+                clxm = shxm;
+                fsxm = sfxm;
+            }
+            if (!xhi_mem)
+            {
+                clxp = shxp;
+                fsxp = sfxp;
+            }
             if (ylo_lds)
             {
                 clym = s_cl[ty - 1][lane];
                 fsym = s_fs[ty - 1][lane];
             }
-            else
-            {
-                clym = ld_cl<F32>(b, c, cur, idx - c.sx);
-                fsym = fsc[idx - c.sx];
-            }
             if (yhi_lds)
             {
                 clyp = s_cl[ty + 1][lane];
                 fsyp = s_fs[ty + 1][lane];
-            }
-            else
-            {
-                clyp = ld_cl<F32>(b, c, cur, idx + c.sx);
-                fsyp = fsc[idx + c.sx];
             }
             const double cln[6] = { clxp, clxm, clyp, clym, clp, clm };
             const double fsn[6] = { fsxp, fsxm, fsyp, fsym, fsp, fsm };
@@ -639,16 +655,18 @@ k_solute_zm(View V, int cur, int nxt)
             st_cl<F32>(b, c, nxt, idx, cl_out);
             fsn_out[idx] = fs_out;
             if (g_new != g_old)
-                b.gr[idx] = g_new;
+                G(b.gr)[idx] = g_new;
         }
         unsigned long long bits = __ballot(active && g_new > 0);
         if (rowok && lane == 0)
-            b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
+            G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
         __syncthreads();
         clm = clv;
         clv = clp;
+        clp = clq;
         fsm = fsv;
         fsv = fsp;
+        fsp = fsq;
     }
     if (F32)
     {
@@ -669,10 +687,10 @@ k_mask_from_gr(DevBufs b, Consts c)
     const int k = blockIdx.z;
     bool solid = false;
     if (x <= c.nx)
-        solid = b.gr[k * c.sxy + (long long) j * c.sx + x] > 0;
+        solid = G(b.gr)[k * c.sxy + (long long) j * c.sx + x] > 0;
     unsigned long long bits = __ballot(solid);
     if (threadIdx.x == 0)
-        b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + blockIdx.x] = bits;
+        G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + blockIdx.x] = bits;
 }
 
 // ---------------------------------------------------------------------------
@@ -689,7 +707,7 @@ list_words_d(const Consts &c)
 __device__ __forceinline__ unsigned long long
 mask_word(const DevBufs &b, const Consts &c, int k, int j, int w)
 {
-    return b.mask[((long long) k * (c.ny + 2) + j) * c.w64 + w];
+    return G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + w];
 }
 
 __device__ __forceinline__ unsigned long long
@@ -726,7 +744,7 @@ k_list_count(View V)
     const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
     const Consts c = V.dom[dm].c;
     const long long nwords = list_words_d(c);
-    if (b.cnt->status != MMCA_OK)
+    if (G(b.cnt)->status != MMCA_OK)
         return;
     const long long wi = blk * LIST_BS + threadIdx.x;
     int v = 0;
@@ -738,7 +756,7 @@ k_list_count(View V)
     int tot;
     block_exscan_int<LIST_BS / 64>(v, &tot, s_w);
     if (threadIdx.x == 0)
-        b.blk_count[blk] = tot;
+        G(b.blk_count)[blk] = tot;
 }
 
 // Exclusive scan of the block counts (one workgroup).  When decide_realloc
@@ -759,11 +777,11 @@ k_list_scan(View V, int decide_realloc)
     for (int base = 0; base < nblk; base += SCAN_BS)
     {
         int i = base + threadIdx.x;
-        int v = (i < nblk) ? b.blk_count[i] : 0;
+        int v = (i < nblk) ? G(b.blk_count)[i] : 0;
         int tot;
         int ex = block_exscan_int<SCAN_BS / 64>(v, &tot, s_w);
         if (i < nblk)
-            b.blk_count[i] = carry + ex;
+            G(b.blk_count)[i] = carry + ex;
         carry += tot;
     }
     if (threadIdx.x == 0)
@@ -835,7 +853,7 @@ k_list_write(View V)
     const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
     const Consts c = V.dom[dm].c;
     const long long nwords = list_words_d(c);
-    if (b.cnt->status != MMCA_OK)
+    if (G(b.cnt)->status != MMCA_OK)
         return;
     const long long wi0 = blk * LIST_BS;
     const long long wi = wi0 + threadIdx.x;
@@ -872,7 +890,7 @@ k_list_write(View V)
         s_nb[n][threadIdx.x] = nb[n];
     s_ex[threadIdx.x] = ex;
     __syncthreads();
-    const int base = b.blk_count[blk];            // members before this workgroup
+    const int base = G(b.blk_count)[blk];            // members before this workgroup
     for (int i = threadIdx.x; i < tot; i += LIST_BS)
     {
         // word of member i: the last word whose prefix count is <= i
@@ -896,8 +914,8 @@ k_list_write(View V)
 #pragma unroll
         for (int n = 0; n < 6; ++n)
             pattern |= (unsigned int) ((s_nb[n][lo] >> bit) & 1ull) << n;
-        b.diff_id[base + i + 1] = (int) (k * c.sxy + (long long) j * c.sx + x);   // slots start at 1
-        b.nbr[base + i + 1] = (unsigned char) pattern;
+        G(b.diff_id)[base + i + 1] = (int) (k * c.sxy + (long long) j * c.sx + x);   // slots start at 1
+        G(b.nbr)[base + i + 1] = (unsigned char) pattern;
     }
 }
 
@@ -915,26 +933,26 @@ k_capture(View V, int cb)
     const long long nthr = (long long) (V.off_list[dm + 1] - V.off_list[dm]) * LIST_BS;
     const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
     const Consts c = V.dom[dm].c;
-    const Counters *cnt = b.cnt;
+    const MMCA_GLOBAL Counters *cnt = G(b.cnt);
     if (cnt->status != MMCA_OK)
         return;
     const int nmem = cnt->gindex;
     const bool realloc = cnt->realloc_old >= 0;
     const long long maxg = c.max_total_grains;
-    double *fsb = fs_buf(b, cb);
-    double *clb = cl_buf(b, cb);
+    MMCA_GLOBAL double *fsb = fs_buf(b, cb);
+    MMCA_GLOBAL double *clb = cl_buf(b, cb);
     for (long long t = blk * LIST_BS + threadIdx.x; t < nmem; t += nthr)
     {
         const int slot = (int) t + 1;
-        const unsigned idx = (unsigned) b.diff_id[slot];
+        const unsigned idx = (unsigned) G(b.diff_id)[slot];
         double tx, ty, tz;
         if (realloc)
             tx = ty = tz = 0.0;
         else
         {
-            tx = b.dtx[slot];
-            ty = b.dty[slot];
-            tz = b.dtz[slot];
+            tx = G(b.dtx)[slot];
+            ty = G(b.dty)[slot];
+            tz = G(b.dtz)[slot];
         }
         // neighbour order +x, +y, -x, -y, +z, -z (growth.c:686-700)
         const int nnx[6] = { 1, 0, -1, 0, 0, 0 };
@@ -944,7 +962,7 @@ k_capture(View V, int cb)
                                     (long long) idx - c.sx, (long long) idx + c.sxy, (long long) idx - c.sxy };
         // solid neighbours before capture (recorded by the list writer); their
         // grain ids, centres and fs are independent loads issued together
-        const unsigned int sol = b.nbr[slot];
+        const unsigned int sol = G(b.nbr)[slot];
         int gids[6];
         double pdc[6][3], pfs[6];
 #pragma unroll
@@ -954,10 +972,10 @@ k_capture(View V, int cb)
             pdc[n][0] = pdc[n][1] = pdc[n][2] = pfs[n] = 0.0;
             if ((sol >> n) & 1u)
             {
-                gids[n] = b.gr[ndxs[n]];
-                pdc[n][0] = b.dc[3 * ndxs[n]];
-                pdc[n][1] = b.dc[3 * ndxs[n] + 1];
-                pdc[n][2] = b.dc[3 * ndxs[n] + 2];
+                gids[n] = G(b.gr)[ndxs[n]];
+                pdc[n][0] = G(b.dc)[3 * ndxs[n]];
+                pdc[n][1] = G(b.dc)[3 * ndxs[n] + 1];
+                pdc[n][2] = G(b.dc)[3 * ndxs[n] + 2];
                 pfs[n] = fsb[ndxs[n]];
             }
         }
@@ -971,7 +989,7 @@ k_capture(View V, int cb)
             const int gid = gids[n];
             if (found >= 0 || !(gid > 0 && gid < maxg))
                 continue;
-            const double *R = b.g_rot + 9 * (long long) gid;
+            const MMCA_GLOBAL double *R = G(b.g_rot) + 9 * (long long) gid;
             g00 = R[0]; g01 = R[1]; g02 = R[2];
             g10 = R[3]; g11 = R[4]; g12 = R[5];
             g20 = R[6]; g21 = R[7]; g22 = R[8];
@@ -1022,22 +1040,22 @@ k_capture(View V, int cb)
             tx = ddx + nnx[found] + g00 * ncx + g01 * ncy + g02 * ncz;
             ty = ddy + nny[found] + g10 * ncx + g11 * ncy + g12 * ncz;
             tz = ddz + nnz[found] + g20 * ncx + g21 * ncy + g22 * ncz;
-            b.gr[idx] = gfound;
+            G(b.gr)[idx] = gfound;
             fsb[idx] = 0.;
             if (c.f32)
-                cl32_buf(b, cb)[idx] = b.ce32[idx];
+                cl32_buf(b, cb)[idx] = G(b.ce32)[idx];
             else
-                clb[idx] = b.ce[idx];
+                clb[idx] = G(b.ce)[idx];
         }
         if (found > -1 || realloc)
         {
-            b.dtx[slot] = tx;
-            b.dty[slot] = ty;
-            b.dtz[slot] = tz;
+            G(b.dtx)[slot] = tx;
+            G(b.dty)[slot] = ty;
+            G(b.dtz)[slot] = tz;
         }
-        b.dc[3 * (long long) idx] = tx;
-        b.dc[3 * (long long) idx + 1] = ty;
-        b.dc[3 * (long long) idx + 2] = tz;
+        G(b.dc)[3 * (long long) idx] = tx;
+        G(b.dc)[3 * (long long) idx + 1] = ty;
+        G(b.dc)[3 * (long long) idx + 2] = tz;
     }
 }
 
@@ -1051,7 +1069,7 @@ k_reduce_partial(DevBufs b, Consts c, int cb)
     __shared__ double s_a[RED_BS], s_b[RED_BS];
     double a = 0.0, s = 0.0;
     const long long N = (long long) c.nx * c.ny * c.nz;
-    const double *fs = fs_buf(b, cb);
+    const MMCA_GLOBAL double *fs = fs_buf(b, cb);
     for (long long i = (long long) blockIdx.x * RED_BS + threadIdx.x; i < N;
          i += (long long) gridDim.x * RED_BS)
     {
@@ -1061,7 +1079,7 @@ k_reduce_partial(DevBufs b, Consts c, int cb)
         int k = (int) (row / c.ny) + 1;
         long long idx = k * c.sxy + (long long) j * c.sx + x;
         a += ref_min(1.0, fs[idx]);
-        s += c.f32 ? (double) b.ce32[idx] : b.ce[idx];     // FP32: deviation from c0
+        s += c.f32 ? (double) G(b.ce32)[idx] : G(b.ce)[idx];     // FP32: deviation from c0
     }
     s_a[threadIdx.x] = a;
     s_b[threadIdx.x] = s;
@@ -1077,8 +1095,8 @@ k_reduce_partial(DevBufs b, Consts c, int cb)
     }
     if (threadIdx.x == 0)
     {
-        b.red_part[2 * blockIdx.x] = s_a[0];
-        b.red_part[2 * blockIdx.x + 1] = s_b[0];
+        G(b.red_part)[2 * blockIdx.x] = s_a[0];
+        G(b.red_part)[2 * blockIdx.x + 1] = s_b[0];
     }
 }
 
@@ -1089,8 +1107,8 @@ k_reduce_final(DevBufs b, int nblk, double *out)
     double a = 0.0, s = 0.0;
     for (int i = threadIdx.x; i < nblk; i += RED_BS)
     {
-        a += b.red_part[2 * i];
-        s += b.red_part[2 * i + 1];
+        a += G(b.red_part)[2 * i];
+        s += G(b.red_part)[2 * i + 1];
     }
     s_a[threadIdx.x] = a;
     s_b[threadIdx.x] = s;
@@ -1139,19 +1157,19 @@ k_grow_octahedra(DevBufs b, Consts c, int cb)
     int x = (int) (rem - (long long) y * c.sx);
     if (x < 1 || x > c.nx || y < 1 || y > c.ny || z < 1 || z > c.nz)
         return;
-    int grid = b.gr[i];
+    int grid = G(b.gr)[i];
     if (grid > 0 && grid < c.max_total_grains)
     {
-        const double *R = b.g_rot + 9 * (long long) grid;
-        double rx0 = 0.0 - b.dc[3 * i];
-        double ry0 = 0.0 - b.dc[3 * i + 1];
-        double rz0 = 0.0 - b.dc[3 * i + 2];
+        const MMCA_GLOBAL double *R = G(b.g_rot) + 9 * (long long) grid;
+        double rx0 = 0.0 - G(b.dc)[3 * i];
+        double ry0 = 0.0 - G(b.dc)[3 * i + 1];
+        double rz0 = 0.0 - G(b.dc)[3 * i + 2];
         double gx0 = R[0] * rx0 + R[3] * ry0 + R[6] * rz0;
         double gy0 = R[1] * rx0 + R[4] * ry0 + R[7] * rz0;
         double gz0 = R[2] * rx0 + R[5] * ry0 + R[8] * rz0;
         double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
         ds1 = ref_min(ds1, 1.5);
-        b.d[i] = ds1 + fs_buf(b, cb)[i] * c.rev_fsgrow;
+        G(b.d)[i] = ds1 + fs_buf(b, cb)[i] * c.rev_fsgrow;
     }
 }
 

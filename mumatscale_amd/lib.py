@@ -23,6 +23,15 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    # PyTorch's ROCm wheels carry their own HIP runtime with the same soname
+    # (libamdhip64.so.7) as /opt/rocm's.  The first one loaded serves the
+    # whole process; torch fails to find a GPU on a runtime other than its
+    # own, so when torch is installed it is loaded first and this library
+    # shares its runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("%s is not built: run __graft_entry__.build() (make -C mumatscale_amd/csrc)"
                            % LIB_PATH)

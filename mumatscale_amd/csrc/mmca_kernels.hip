@@ -334,13 +334,12 @@ k_nuc_activate(View V, int mode)
 }
 
 // Partitioning of one cell (fs_change_diffuse, growth.c:221-273).  The
-// temperature is only needed for cells with gr > 0.  *branch receives the
-// MMCA_BRANCH_* code of the branch taken.
-template <bool AM>
+// temperature, temp(), is only evaluated for cells with gr > 0.  *branch
+// receives the MMCA_BRANCH_* code of the branch taken.
+template <typename TempFn>
 __device__ __forceinline__ void
-partition_cell(const Consts &c, double sim_time, int x, int j, int k, double ce_new,
-               double ce_prev, double conc, double lspfs, int g_old, double *cl_out,
-               double *fs_out, int *g_out, int *branch)
+partition_cell(const Consts &c, TempFn temp, double ce_new, double ce_prev, double conc, double lspfs,
+               int g_old, double *cl_out, double *fs_out, int *g_out, int *branch)
 {
     int g = g_old;
     *branch = MMCA_BRANCH_LIQUID;
@@ -351,7 +350,7 @@ partition_cell(const Consts &c, double sim_time, int x, int j, int k, double ce_
         *fs_out = 0;
         return;
     }
-    const double Tcell = temp_model<AM>(c, sim_time, x, j, k);
+    const double Tcell = temp();
     double c_int = c.cinit + c.m_inv * (Tcell - c.Tliq);
     double Tunder = c.Tliq - Tcell;
     Tunder += c.liq_slope * (conc - c.cinit);
@@ -466,8 +465,9 @@ k_solute(View V, int cur, int nxt)
             const int g_old = G(b.gr)[idx];
             double cl_out, fs_out;
             int g, branch;
-            partition_cell<AM>(c, G(b.cnt)->timestep * c.dt, x, j, k, ce_new, ce_prev, conc, lspfs, g_old,
-                               &cl_out, &fs_out, &g, &branch);
+            const double sim_time = G(b.cnt)->timestep * c.dt;
+            partition_cell(c, [&]() { return temp_model<AM>(c, sim_time, x, j, k); }, ce_new, ce_prev, conc,
+                           lspfs, g_old, &cl_out, &fs_out, &g, &branch);
             if (REC)
             {
                 // branch record of this step (bounds: set by the host, checked here too)
@@ -509,6 +509,10 @@ k_solute_zm(View V, int cur, int nxt)
     __shared__ double s_fs[BY][64];
     __shared__ double s_red[BY];
     __shared__ int s_stop;
+    // Gaussian-beam tables (AM only): the cell-independent factors of the
+    // 400 quadrature terms, and the plane-dependent one for the current plane
+    __shared__ double s_beam[AM ? sizeof(BeamTab) / sizeof(double) : 1];
+    BeamTab *beam = reinterpret_cast<BeamTab *>(s_beam);
     // Tile of this workgroup.  With xcd_remap the workgroups that the
     // hardware hands to one XCD (linear id mod 8) take a contiguous run of
     // tiles (x fastest, then y, then z, then domain), so that tiles sharing
@@ -536,6 +540,8 @@ k_solute_zm(View V, int cur, int nxt)
         return;
     const int lane = threadIdx.x;
     const int ty = threadIdx.y;
+    if constexpr (AM)
+        beam_tab_terms(c, *beam, ty * 64 + lane, 64 * BY);
     const int ntx = (c.nx + 63) / 64, nty = (c.ny + BY - 1) / BY;
     const int bx = (int) (tile % ntx);
     const int by = (int) ((tile / ntx) % nty);
@@ -612,6 +618,8 @@ k_solute_zm(View V, int cur, int nxt)
                 fsyp = fsc[idx + c.sx];
             }
         }
+        if constexpr (AM)
+            beam_tab_plane(c, *beam, k, ty * 64 + lane, 64 * BY);   // read after the barrier below
         s_cl[ty][lane] = clv;
         s_fs[ty][lane] = fsv;
         const double shxm = __shfl_up(clv, 1, 64), sfxm = __shfl_up(fsv, 1, 64);
@@ -650,8 +658,15 @@ k_solute_zm(View V, int cur, int nxt)
             lost += st_ce<F32>(b, c, idx, ce_new);
             double cl_out, fs_out;
             int branch;
-            partition_cell<AM>(c, sim_time, x, j, k, ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out,
-                               &g_new, &branch);
+            partition_cell(
+                c,
+                [&]() {
+                    if constexpr (AM)
+                        return beam_temp_tab(c, *beam, sim_time, x, j, k);
+                    else
+                        return temp_model<false>(c, sim_time, x, j, k);
+                },
+                ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out, &g_new, &branch);
             st_cl<F32>(b, c, nxt, idx, cl_out);
             fsn_out[idx] = fs_out;
             if (g_new != g_old)

@@ -271,8 +271,10 @@ def run_am_track(steps=160):
 
 
 def cpu_baseline(cfg, target_s):
-    """Oracle (scalar C port of the reference path), one thread, bounded sample."""
-    from oracle.oracle import OracleSubblock
+    """The oracle (C restatement of the reference path) on the host cores:
+    its per-cell loops run with OpenMP (OMP_NUM_THREADS threads), list
+    building and nucleation serially; a bounded sample of the same workload."""
+    from oracle.oracle import OracleSubblock, threads
     t0 = time.time()
     orc = OracleSubblock(cfg.params(check_negative_ce=False), cfg.nucleation())
     log("cpu baseline: oracle set-up %.1f s" % (time.time() - t0))
@@ -285,10 +287,11 @@ def cpu_baseline(cfg, target_s):
     orc.step(steps)
     dt = time.time() - t0
     orc.close()
-    return {"value": round(n * steps / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
+    nthr = threads()
+    return {"value": round(n * steps / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": nthr,
             "kind": "port",
-            "sample": "same workload at full size, steps 2-%d after set-up (%.1f s), one thread"
-                      % (steps + 1, dt)}
+            "sample": "same workload at full size, steps 2-%d after set-up (%.1f s), %d OpenMP threads"
+                      % (steps + 1, dt, nthr)}
 
 
 def main():

@@ -20,6 +20,11 @@
  *   - diffusion:       mass conservation, uniform fixed point, 1-D closed form
  *   - fs_change:       per-cell closed forms of each branch
  *   - capture corner:  truth table of the 5-test chain
+ *
+ * Loops whose cells are independent (temperature, diffusion, fs_change,
+ * octahedra, capture over list slots) run with OpenMP; list building,
+ * nucleation order and sums stay serial in raster order, so results do not
+ * depend on the thread count.
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -29,6 +34,9 @@
 #include <string.h>
 
 #include "../include/mmca.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define O_MIN(x, y) (((x) < (y)) ? (x) : (y))
 #define O_MAX(x, y) (((x) > (y)) ? (x) : (y))
@@ -331,6 +339,7 @@ void
 orc_temp_update(orc_state *s)
 {
     int ana = (s->p.temp_model == MMCA_TEMP_AM_BEAM);
+#pragma omp parallel for schedule(static)
     for (int z = 0; z < s->nz + 2; z++)
         for (int y = 0; y < s->ny + 2; y++)
             for (int x = 0; x < s->nx + 2; x++)
@@ -824,6 +833,10 @@ orc_diffuse(orc_state *s)
     }
     int8_t *mold = s->mold;
     double *fs = s->fs, *cl = s->cl, *ce = s->ce;
+    int negative = 0;
+    /* cells are independent (ce from oce-time cl and fs); a negative ce stops
+     * the step as the reference's abort does (growth.c:163-165) */
+#pragma omp parallel for schedule(static) reduction(|:negative)
     for (int k = 1; k <= dimz; k++)
         for (int j = 1; j <= dimy; j++)
             for (int i = 1; i <= dimx; i++)
@@ -859,12 +872,14 @@ orc_diffuse(orc_state *s)
                 nbsum += r * (cl[idzm] - conc);
                 ce[idx] += nbsum;
                 if (p->check_negative_ce && ce[idx] < 0)
-                {
-                    s->status = MMCA_ERR_NEGATIVE_CE;
-                    s->failed_step = s->timestep;
-                    return s->status;
-                }
+                    negative = 1;
             }
+    if (negative)
+    {
+        s->status = MMCA_ERR_NEGATIVE_CE;
+        s->failed_step = s->timestep;
+        return s->status;
+    }
     return MMCA_OK;
 }
 
@@ -885,6 +900,7 @@ orc_fs_change(orc_state *s)
     const double Tliq = p->liquidus_temp;
     double *oce = s->oce, *ce = s->ce, *fs = s->fs, *cl = s->cl, *temperature = s->temperature;
     int *gr = s->gr;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= dimz; k++)
         for (int j = 1; j <= dimy; j++)
             for (int i = 1; i <= dimx; i++)
@@ -935,6 +951,7 @@ orc_grow_octahedra(orc_state *s)
 {
     const int dimx = s->nx, dimy = s->ny, dimz = s->nz;
     const double fsgrow = s->rev_fsgrow;
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= dimz; k++)
         for (int j = 1; j <= dimy; j++)
             for (int i = 1; i <= dimx; i++)
@@ -1013,6 +1030,9 @@ orc_capture(orc_state *s)
     }
     double *dc_tmp = s->dc_tmp;
     int *diff_id = s->diff_id;
+    /* listed cells are independent: each reads solid neighbours through the
+     * ogr snapshot and writes only itself and its slot */
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= gindex; i++)
     {
         int idx = diff_id[i];
@@ -1073,6 +1093,7 @@ orc_capture(orc_state *s)
             cl[idx] = ce[idx];
         }
     }
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= gindex; i++)
     {
         int idx = diff_id[i];
@@ -1166,6 +1187,16 @@ orc_field(orc_state *s, int which)
 }
 
 void orc_set_record(orc_state *s, uint8_t *rec) { s->rec = rec; }
+/* threads the parallel loops use (1 without OpenMP) */
+int
+orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 /* grain_cache records 0..n-1 and bp->num_grains, as a restart restores them
  * (addGrainCache, checkpoint.c:676-745) */
 int

@@ -8,12 +8,12 @@
 
 HERE := $(patsubst %/,%,$(dir $(lastword $(MAKEFILE_LIST))))
 CC   := gcc
-CFLAGS := -std=gnu99 -O2 -fPIC -ffp-contract=off -fno-fast-math -Wall -Wno-unused-function
+CFLAGS := -std=gnu99 -O2 -fPIC -ffp-contract=off -fno-fast-math -Wall -Wno-unused-function -fopenmp
 
 all: $(HERE)/liboracle.so
 
 $(HERE)/liboracle.so: $(HERE)/mmca_oracle.c $(HERE)/../include/mmca.h
-	$(CC) $(CFLAGS) -shared $(HERE)/mmca_oracle.c -o $@ -lm
+	$(CC) $(CFLAGS) -shared $(HERE)/mmca_oracle.c -o $@ -lm -lgomp
 
 clean:
 	rm -f $(HERE)/liboracle.so

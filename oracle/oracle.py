@@ -64,6 +64,7 @@ def lib():
         L.orc_failed_step.restype = C.c_uint64
         L.orc_failed_step.argtypes = [vp]
         L.orc_set_grains.argtypes = [vp, C.POINTER(abi.Grain), C.c_int]
+        L.orc_threads.restype = C.c_int
         L.orc_set_record.argtypes = [vp, C.c_void_p]
         L.orc_set_record.restype = None
         L.orc_set_timestep.argtypes = [vp, C.c_uint64]
@@ -85,6 +86,11 @@ _DTYPES = {
     abi.FIELD_TEMP: (np.float64, 1), abi.FIELD_THRESHOLD: (np.float32, 1),
     abi.FIELD_MOLD: (np.int8, 1), abi.FIELD_OCE: (np.float64, 1),
 }
+
+
+def threads():
+    """Threads of the oracle's parallel loops."""
+    return lib().orc_threads()
 
 
 def fnv1a(arr):

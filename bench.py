@@ -229,7 +229,7 @@ def run_adjoint(n=128, nseeds=16, steps=4000):
             "max_rel_err_best_h": max(min(r.values()) for r in rel.values())}
 
 
-def run_am_track(steps=160):
+def run_am_track(steps=60):
     """The AM single track over a 2000-grain Voronoi substrate (1024 x 512 x
     256, dx = 0.5 um, Gaussian-beam temperature) with FP32-deviation storage,
     and its error against the FP64 run of the same steps.  Every solid cell
@@ -265,8 +265,15 @@ def run_am_track(steps=160):
         "solid_fraction_abs": abs(r32.solid_sum - r64.solid_sum) / n,
         "grains_fp32": r32.num_grains - 1, "grains_fp64": r64.num_grains - 1,
         "solute_drift_with_ledger": (r32.solute_sum + r32.solute_ledger - n * cfg.c_init) / (n * cfg.c_init)}
+    # The compiled quadrature loop issues 51 FP64 VALU instructions per term
+    # (two divisions and an exp each); MI355X issues 78.6e12 / 2 FP64 lane
+    # operations per second.
+    valu_per_term = 51
+    for key in ("fp32dev", "fp64"):
+        t = out[key]["beam_terms_per_s"]
+        out[key]["valu_frac"] = round(t * valu_per_term / (78.6e12 / 2), 3) if t else None
     out.update({"cells": n, "steps": steps, "dt": cfg.dt, "substrate_grains": len(recs) - 1,
-                "bound": "FP64 VALU (beam quadrature on every solid cell)"})
+                "bound": "FP64 VALU (beam quadrature on every solid cell)", "valu_ops_per_term": valu_per_term})
     return out
 
 
@@ -309,8 +316,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the FP32-deviation and batched-ensemble measurements")
     ap.add_argument("--extra-steps", type=int, default=1000)
-    ap.add_argument("--am", action="store_true",
-                    help="also run the AM single track (FP32 against FP64, 1024x512x256)")
+    ap.add_argument("--no-am", action="store_true",
+                    help="skip the AM single track (FP32 against FP64, 1024x512x256)")
+    ap.add_argument("--am-steps", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
@@ -355,8 +363,8 @@ def main():
         log("ensemble:", extra["batched_ensemble"])
         extra["adjoint"] = run_adjoint()
         log("adjoint:", extra["adjoint"])
-        if args.am:
-            extra["am_track"] = run_am_track()
+        if not args.no_am:
+            extra["am_track"] = run_am_track(args.am_steps)
             log("am track:", extra["am_track"])
     cpu = None
     if not args.no_cpu:

@@ -219,7 +219,35 @@ int  mmca_get_interface_list(mmca_ctx *ctx, int32_t *out, int32_t max, int32_t *
  * Cells refer to grains through the gr field (mmca_upload_field).
  */
 int  mmca_set_grains(mmca_ctx *ctx, const mmca_grain *grains, int32_t num_grains);
+/*
+ * Restart image of a block: fields, counters, interface list and dc_tmp
+ * slots, grain table, and the random stream after the orientations
+ * consumed so far.  The reference's restart (checkpoint.c:955-1238,
+ * mainRestart/taskRestart) stores neither the random stream nor dc_tmp, so
+ * a restarted run diverges from an uninterrupted one; a block restored with
+ * mmca_load_state continues bit for bit.  mmca_save_state writes at most
+ * `capacity` bytes to host memory `buf` and the size to *size; with
+ * buf == NULL it only reports the size.  mmca_load_state needs a context
+ * created and given its nucleation set-up with the same parameters.
+ */
+int  mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size);
+int  mmca_load_state(mmca_ctx *ctx, const void *buf, int64_t size);
 int  mmca_get_timestep(mmca_ctx *ctx, uint64_t *timestep);
+/*
+ * cache_io_data (loop.c:74-113) and the host copies the output writers read
+ * (io_xdmf.c:605-655): padded FP32 copies of fs, ce and the temperature and
+ * a copy of gr, converted on the device.  A NULL output is skipped.
+ */
+int  mmca_io_fields(mmca_ctx *ctx, float *fs_io, float *ce_io, float *temperature_io, int32_t *gr_io);
+/*
+ * The next n values of rand() from this block's random stream, for callers
+ * that share the stream with the CA, as the XDMF writer's grain renumbering
+ * does (getRenumberArray, io_xdmf.c:345-376: rand() % num_grains on rank 0).
+ * The library draws grain orientations ahead of time; after this call the
+ * orientations of grains created later come from the stream after these
+ * values, as in the reference.
+ */
+int  mmca_rand(mmca_ctx *ctx, int32_t *out, int32_t n);
 int  mmca_padded_cells(mmca_ctx *ctx, int64_t *cells);
 
 /* --- host set-up without a device (for checking the set-up on a CPU) ----- */

@@ -22,6 +22,8 @@ hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, in
                          hipStream_t s);
 hipError_t launch_mask_from_gr(const DevBufs &b, const Consts &c, hipStream_t s);
 hipError_t launch_list(const View &v, int cb, int capture, int decide_realloc, hipStream_t s);
+hipError_t launch_io_fields(const DevBufs &b, const Consts &c, int cur, float *fs_io, float *ce_io, float *t_io,
+                            int *gr_io, hipStream_t s);
 hipError_t launch_capture_only(const View &v, const DevBufs &b, int cb, hipStream_t s);
 int nuc_blocks(int nsites);
 long long cell_blocks(const Consts &c);
@@ -87,6 +89,16 @@ struct mmca_ctx {
     long long time_n = 0;
     // grain records set by mmca_set_grains (ids below ext_grains.size())
     std::vector<mmca_grain> ext_grains;
+    // staging buffer of mmca_io_fields: 4 x P 32-bit values (allocated on first use)
+    float *io_buf = nullptr;
+    // The random stream at the start of the orientation queue (before draw
+    // q0).  mmca_rand draws from rng_now, the stream right after the
+    // orientations consumed so far; the queue from there on is then redrawn
+    // before the next step (rand_dirty).
+    GlibcRand rng_q0;
+    long long q0 = 0;
+    GlibcRand rng_now;
+    bool rand_dirty = false;
 };
 
 namespace {
@@ -150,6 +162,22 @@ int ensure_random(mmca_ctx *ctx, long long upto)
                           hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->rand_filled = (int) upto;
+    return MMCA_OK;
+}
+
+// After mmca_rand: the queued orientations past the consumed ones were drawn
+// from the stream before those raw values; drop them so that the next ones
+// are drawn after them.
+int replan_random(mmca_ctx *ctx)
+{
+    if (!ctx->rand_dirty)
+        return MMCA_OK;
+    ctx->rng = ctx->rng_now;
+    ctx->rng_q0 = ctx->rng_now;
+    ctx->q0 = ctx->hc.random_used;
+    ctx->rand_filled = ctx->hc.random_used;
+    ctx->rand_angles.resize((size_t) ctx->rand_filled * 3);
+    ctx->rand_dirty = false;
     return MMCA_OK;
 }
 
@@ -438,7 +466,7 @@ int mmca_destroy(mmca_ctx *ctx)
     void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.ce32, b.cl32[0], b.cl32[1], b.ledger, b.fs[0], b.fs[1], b.gr, b.dc, b.mask,
                      b.diff_id, b.nbr, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
                      b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.fixed_rot, b.rand_rot,
-                     b.oce, b.d, b.red_part, b.cnt, ctx->d_red, ctx->d_desc, ctx->d_off };
+                     b.oce, b.d, b.red_part, b.cnt, ctx->d_red, ctx->d_desc, ctx->d_off, ctx->io_buf };
     for (void *p : ptrs)
         if (p)
             (void) hipFree(p);
@@ -493,6 +521,9 @@ int mmca_init_nucleation(mmca_ctx *ctx, const mmca_nucleation_cfg *cfg)
                      hipMemcpyHostToDevice));
     ctx->nucleation_ready = true;
     ctx->view_dirty = true;
+    ctx->rng_q0 = ctx->rng;
+    ctx->q0 = 0;
+    ctx->rand_dirty = false;
     // draw the first orientations now, outside any timed stepping
     TRY(ensure_random(ctx, std::min<long long>(ctx->rand_capacity,
                                                std::max<long long>(65536, 256LL * (long long) ns))));
@@ -509,6 +540,7 @@ int mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report)
         return ctx->hc.status;
     if (ctx->b.rec && ctx->hc.timestep + (uint64_t) nsteps > ctx->b.rec_t0 + (uint64_t) ctx->b.rec_steps)
         return MMCA_ERR_ARG;     // past the end of the branch record
+    TRY(replan_random(ctx));
     ctx->hc.new_grains = 0;
     HIPCHK(hipMemcpyAsync(&ctx->b.cnt->new_grains, &ctx->hc.new_grains, sizeof(int),
                           hipMemcpyHostToDevice, ctx->stream));
@@ -634,6 +666,7 @@ int mmca_activate_new_grains(mmca_ctx *ctx)
         return MMCA_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     TRY(sync_counters(ctx));
+    TRY(replan_random(ctx));
     TRY(ensure_random(ctx, (long long) ctx->hc.random_used + ctx->hc.n_events));
     TRY(ensure_view(ctx));
     HIPCHK(launch_nucleate(ctx->view, 2, ctx->cur, ctx->stream));
@@ -719,6 +752,7 @@ int mmca_doiteration(mmca_ctx *ctx)
     int st = courant_check(ctx);
     if (st != MMCA_OK)
         return st;
+    TRY(replan_random(ctx));
     TRY(ensure_random(ctx, (long long) ctx->hc.random_used + ctx->b.nsites));
     TRY(enqueue_iteration(ctx, false));
     TRY(check_status(ctx));
@@ -935,6 +969,255 @@ int mmca_get_grains(mmca_ctx *ctx, mmca_grain *out, int32_t max, int32_t *num_gr
         }
     }
     return MMCA_OK;
+}
+
+int mmca_rand(mmca_ctx *ctx, int32_t *out, int32_t n)
+{
+    if (!ctx || n < 0 || (n > 0 && !out))
+        return MMCA_ERR_ARG;
+    if (!ctx->nucleation_ready)
+        return MMCA_ERR_STATE;
+    HIPCHK(hipSetDevice(ctx->device));
+    TRY(sync_counters(ctx));
+    if (!ctx->rand_dirty) {
+        // the stream after the orientations consumed so far (3 draws each,
+        // grain.c:118-129; none when orientations are not randomised)
+        ctx->rng_now = ctx->rng_q0;
+        const long long draws = (ctx->oriented ? 3LL : 0LL) * ((long long) ctx->hc.random_used - ctx->q0);
+        for (long long i = 0; i < draws; i++)
+            (void) ctx->rng_now.next();
+        ctx->rand_dirty = true;
+    }
+    for (int32_t i = 0; i < n; i++)
+        out[i] = ctx->rng_now.next();
+    return MMCA_OK;
+}
+
+namespace {
+
+// Restart image of one block: a header, the counters, the random stream at
+// the orientations consumed so far, then the device arrays in a fixed order.
+struct StateHeader {
+    char magic[8];
+    int32_t version, precision;
+    int64_t P;
+    int32_t nx, ny, nz, nsites;
+    int64_t max_total_grains;
+    int32_t cur, oriented, n_ext, pad_;
+    int64_t nledger, n_angles;
+};
+
+constexpr char STATE_MAGIC[8] = { 'M', 'M', 'C', 'A', 'S', 'T', 'A', 'T' };
+
+// the stream right after the orientations consumed so far
+GlibcRand stream_at_consumption(const mmca_ctx *ctx)
+{
+    if (ctx->rand_dirty)
+        return ctx->rng_now;
+    GlibcRand g = ctx->rng_q0;
+    const long long draws = (ctx->oriented ? 3LL : 0LL) * ((long long) ctx->hc.random_used - ctx->q0);
+    for (long long i = 0; i < draws; i++)
+        (void) g.next();
+    return g;
+}
+
+// Appends (save) or extracts (load) each piece in the same order.
+struct StateIO {
+    char *buf;           // null: count only
+    const char *in;      // load source
+    int64_t pos = 0, cap = 0;
+    bool load;
+    mmca_ctx *ctx;
+    hipStream_t s;
+    int host(void *p, size_t n)
+    {
+        if (load) {
+            if (pos + (int64_t) n > cap)
+                return MMCA_ERR_ARG;
+            memcpy(p, in + pos, n);
+        } else if (buf) {
+            if (pos + (int64_t) n > cap)
+                return MMCA_ERR_ARG;
+            memcpy(buf + pos, p, n);
+        }
+        pos += (int64_t) n;
+        return MMCA_OK;
+    }
+    int dev(void *d, size_t n)
+    {
+        if (n == 0)
+            return MMCA_OK;
+        if (load) {
+            if (pos + (int64_t) n > cap)
+                return MMCA_ERR_ARG;
+            HIPCHK(hipMemcpyAsync(d, in + pos, n, hipMemcpyHostToDevice, s));
+        } else if (buf) {
+            if (pos + (int64_t) n > cap)
+                return MMCA_ERR_ARG;
+            HIPCHK(hipMemcpyAsync(buf + pos, d, n, hipMemcpyDeviceToHost, s));
+        }
+        pos += (int64_t) n;
+        return MMCA_OK;
+    }
+};
+
+int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc)
+{
+    DevBufs &b = ctx->b;
+    const size_t P = (size_t) h.P;
+    const int cur = h.cur;
+    TRY(io.host(&hc, sizeof(Counters)));
+    GlibcRand g = io.load ? GlibcRand() : stream_at_consumption(ctx);
+    TRY(io.host(&g, sizeof(GlibcRand)));
+    if (io.load) {
+        ctx->rng = g;
+        ctx->rng_q0 = g;
+        ctx->rng_now = g;
+    }
+    if (h.precision == MMCA_PREC_FP32DEV) {
+        TRY(io.dev(b.ce32, P * sizeof(float)));
+        TRY(io.dev(b.cl32[cur], P * sizeof(float)));
+        TRY(io.dev(b.ledger, (size_t) (2 * h.nledger) * sizeof(double)));
+    } else {
+        TRY(io.dev(b.ce, P * sizeof(double)));
+        TRY(io.dev(b.cl[cur], P * sizeof(double)));
+    }
+    TRY(io.dev(b.fs[cur], P * sizeof(double)));
+    TRY(io.dev(b.gr, P * sizeof(int)));
+    TRY(io.dev(b.dc, 3 * P * sizeof(double)));
+    // interface list (slots 0..gindex) and the persistent dc_tmp slots 0..dc_tmp_size
+    TRY(io.dev(b.diff_id, (size_t) (hc.gindex + 1) * sizeof(int)));
+    const size_t nslot = (size_t) std::max(hc.dc_tmp_size, 0) + 1;
+    TRY(io.dev(b.dtx, nslot * sizeof(double)));
+    TRY(io.dev(b.dty, nslot * sizeof(double)));
+    TRY(io.dev(b.dtz, nslot * sizeof(double)));
+    const size_t ng = (size_t) hc.num_grains;
+    TRY(io.dev(b.g_rot, 9 * ng * sizeof(double)));
+    TRY(io.dev(b.g_cell, ng * sizeof(int)));
+    TRY(io.dev(b.g_step, ng * sizeof(unsigned long long)));
+    TRY(io.dev(b.g_src, ng * sizeof(int)));
+    TRY(io.dev(b.site_ls, (size_t) h.nsites * sizeof(int)));
+    if (io.load)
+        ctx->rand_angles.assign((size_t) (3 * h.n_angles), 0.0);
+    TRY(io.host(ctx->rand_angles.data(), (size_t) (3 * h.n_angles) * sizeof(double)));
+    if (io.load)
+        ctx->ext_grains.assign((size_t) h.n_ext, mmca_grain());
+    TRY(io.host(ctx->ext_grains.data(), (size_t) h.n_ext * sizeof(mmca_grain)));
+    return MMCA_OK;
+}
+
+}  // namespace
+
+int mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size)
+{
+    if (!ctx || capacity < 0 || (buf && capacity == 0))
+        return MMCA_ERR_ARG;
+    if (!ctx->nucleation_ready)
+        return MMCA_ERR_STATE;
+    HIPCHK(hipSetDevice(ctx->device));
+    TRY(sync_counters(ctx));
+    StateHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, STATE_MAGIC, 8);
+    h.version = 1;
+    h.precision = ctx->c.f32 ? MMCA_PREC_FP32DEV : MMCA_PREC_FP64;
+    h.P = ctx->c.P;
+    h.nx = ctx->c.nx;
+    h.ny = ctx->c.ny;
+    h.nz = ctx->c.nz;
+    h.nsites = ctx->b.nsites;
+    h.max_total_grains = ctx->p.max_total_grains;
+    h.cur = ctx->cur;
+    h.oriented = ctx->oriented;
+    h.n_ext = (int32_t) ctx->ext_grains.size();
+    h.nledger = ctx->c.f32 ? ctx->b.nledger : 0;
+    h.n_angles = ctx->hc.random_used;
+    if ((int64_t) ctx->rand_angles.size() < 3 * h.n_angles)
+        return MMCA_ERR_STATE;
+    Counters hc = ctx->hc;
+    StateIO io;
+    io.buf = (char *) buf;
+    io.in = nullptr;
+    io.cap = capacity;
+    io.load = false;
+    io.ctx = ctx;
+    io.s = ctx->stream;
+    TRY(io.host(&h, sizeof(h)));
+    TRY(state_body(ctx, io, h, hc));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (size)
+        *size = io.pos;
+    return MMCA_OK;
+}
+
+int mmca_load_state(mmca_ctx *ctx, const void *buf, int64_t size)
+{
+    if (!ctx || !buf || size < (int64_t) sizeof(StateHeader))
+        return MMCA_ERR_ARG;
+    if (!ctx->nucleation_ready)
+        return MMCA_ERR_STATE;
+    HIPCHK(hipSetDevice(ctx->device));
+    TRY(sync_counters(ctx));
+    StateHeader h;
+    memcpy(&h, buf, sizeof(h));
+    if (memcmp(h.magic, STATE_MAGIC, 8) != 0 || h.version != 1 || h.P != ctx->c.P || h.nx != ctx->c.nx ||
+        h.ny != ctx->c.ny || h.nz != ctx->c.nz || h.nsites != ctx->b.nsites ||
+        h.max_total_grains != ctx->p.max_total_grains || h.oriented != ctx->oriented ||
+        h.precision != (ctx->c.f32 ? MMCA_PREC_FP32DEV : MMCA_PREC_FP64) || h.cur < 0 || h.cur > 1 ||
+        (ctx->c.f32 && h.nledger != ctx->b.nledger) || h.n_angles < 0 || h.n_angles > ctx->rand_capacity ||
+        h.n_ext < 0)
+        return MMCA_ERR_ARG;
+    // the counters come first after the header: check the sizes they imply
+    Counters hc;
+    memcpy(&hc, (const char *) buf + sizeof(h), std::min<size_t>(sizeof(Counters), (size_t) size - sizeof(h)));
+    if (hc.num_grains < 1 || (int64_t) hc.num_grains > ctx->p.max_total_grains + 1 || hc.gindex < 0 ||
+        (int64_t) hc.gindex > (int64_t) ctx->c.nx * ctx->c.ny * ctx->c.nz || hc.dc_tmp_size < 0 ||
+        (int64_t) hc.dc_tmp_size > (int64_t) ctx->c.nx * ctx->c.ny * ctx->c.nz + 100 ||
+        hc.random_used != h.n_angles)
+        return MMCA_ERR_ARG;
+    StateIO io;
+    io.buf = nullptr;
+    io.in = (const char *) buf;
+    io.cap = size;
+    io.load = true;
+    io.ctx = ctx;
+    io.s = ctx->stream;
+    io.pos = sizeof(h);
+    TRY(state_body(ctx, io, h, hc));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->cur = h.cur;
+    ctx->q0 = hc.random_used;
+    ctx->rand_filled = hc.random_used;
+    ctx->rand_dirty = false;
+    ctx->have_oce = false;
+    ctx->view_dirty = true;
+    ctx->hc = hc;
+    TRY(push_counters(ctx));
+    return MMCA_OK;
+}
+
+int mmca_io_fields(mmca_ctx *ctx, float *fs_io, float *ce_io, float *temperature_io, int32_t *gr_io)
+{
+    if (!ctx)
+        return MMCA_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t P = (size_t) ctx->c.P;
+    if (!ctx->io_buf)
+        TRY(dev_alloc(&ctx->io_buf, 4 * P));
+    float *d_fs = ctx->io_buf, *d_ce = d_fs + P, *d_t = d_ce + P;
+    int *d_gr = (int *) (d_t + P);
+    HIPCHK(launch_io_fields(ctx->b, ctx->c, ctx->cur, fs_io ? d_fs : nullptr, ce_io ? d_ce : nullptr,
+                            temperature_io ? d_t : nullptr, gr_io ? d_gr : nullptr, ctx->stream));
+    if (fs_io)
+        HIPCHK(hipMemcpyAsync(fs_io, d_fs, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (ce_io)
+        HIPCHK(hipMemcpyAsync(ce_io, d_ce, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (temperature_io)
+        HIPCHK(hipMemcpyAsync(temperature_io, d_t, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (gr_io)
+        HIPCHK(hipMemcpyAsync(gr_io, d_gr, P * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return check_status(ctx);
 }
 
 int mmca_set_grains(mmca_ctx *ctx, const mmca_grain *grains, int32_t num_grains)
@@ -1280,6 +1563,7 @@ int mmca_batch_step(mmca_batch *bt, int nsteps, int want_sums, mmca_report *repo
     TRY(batch_view(bt));
     TRY(batch_sync(bt));
     for (mmca_ctx *c : bt->dom) {
+        TRY(replan_random(c));
         c->hc.new_grains = 0;
         HIPCHK(hipMemcpyAsync(&c->b.cnt->new_grains, &c->hc.new_grains, sizeof(int), hipMemcpyHostToDevice,
                               bt->stream));

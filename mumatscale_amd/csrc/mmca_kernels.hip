@@ -1159,6 +1159,43 @@ k_fill_temp(DevBufs b, Consts c, double *out)
     out[i] = temp_at(c, b.cnt->timestep * c.dt, x, y, z);
 }
 
+// cache_io_data (loop.c:74-113): FP32 copies of fs, ce and the temperature
+// and a copy of gr over the padded box, for the output writers.  A null
+// output is skipped.
+template <bool F32>
+__global__ void
+k_io_fields(DevBufs b, Consts c, int cur, float *fs_io, float *ce_io, float *t_io, int *gr_io)
+{
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.P)
+        return;
+    if (fs_io)
+        fs_io[i] = (float) fs_buf(b, cur)[i];
+    if (ce_io)
+        ce_io[i] = (float) ld_ce<F32>(b, c, i);
+    if (gr_io)
+        gr_io[i] = G(b.gr)[i];
+    if (t_io)
+    {
+        int z = (int) (i / c.sxy);
+        long long rem = i - z * c.sxy;
+        int y = (int) (rem / c.sx);
+        int x = (int) (rem - (long long) y * c.sx);
+        t_io[i] = (float) temp_at(c, G(b.cnt)->timestep * c.dt, x, y, z);
+    }
+}
+
+hipError_t launch_io_fields(const DevBufs &b, const Consts &c, int cur, float *fs_io, float *ce_io, float *t_io,
+                            int *gr_io, hipStream_t s)
+{
+    const dim3 grid((unsigned) ((c.P + 255) / 256));
+    if (c.f32)
+        hipLaunchKernelGGL(k_io_fields<true>, grid, dim3(256), 0, s, b, c, cur, fs_io, ce_io, t_io, gr_io);
+    else
+        hipLaunchKernelGGL(k_io_fields<false>, grid, dim3(256), 0, s, b, c, cur, fs_io, ce_io, t_io, gr_io);
+    return hipGetLastError();
+}
+
 // grow_octahedra (growth.c:339-407) into the d buffer
 __global__ void
 k_grow_octahedra(DevBufs b, Consts c, int cb)

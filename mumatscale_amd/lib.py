@@ -60,6 +60,10 @@ def lib():
     L.mmca_get_grains.argtypes = [vp, C.POINTER(abi.Grain), C.c_int32, i32p]
     L.mmca_get_interface_list.argtypes = [vp, i32p, C.c_int32, i32p]
     L.mmca_set_grains.argtypes = [vp, C.POINTER(abi.Grain), C.c_int32]
+    L.mmca_io_fields.argtypes = [vp, vp, vp, vp, vp]
+    L.mmca_rand.argtypes = [vp, i32p, C.c_int32]
+    L.mmca_save_state.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.mmca_load_state.argtypes = [vp, vp, C.c_int64]
     L.mmca_padded_cells.argtypes = [vp, C.POINTER(C.c_int64)]
     L.mmca_set_kernel_timing.argtypes = [vp, C.c_int]
     L.mmca_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -257,6 +261,35 @@ class Subblock:
         arr = (abi.Grain * max(1, n))()
         _check(lib().mmca_get_grains(self._h, arr, n, None), "mmca_get_grains")
         return [arr[i] for i in range(n)]
+
+    def save_state(self):
+        """Restart image of this block (bytes)."""
+        n = C.c_int64(0)
+        _check(lib().mmca_save_state(self._h, None, 0, C.byref(n)), "mmca_save_state")
+        buf = C.create_string_buffer(n.value)
+        _check(lib().mmca_save_state(self._h, buf, n.value, C.byref(n)), "mmca_save_state")
+        return buf.raw[:n.value]
+
+    def load_state(self, image):
+        buf = C.create_string_buffer(image, len(image))
+        _check(lib().mmca_load_state(self._h, buf, len(image)), "mmca_load_state")
+
+    def rand(self, n):
+        """The next n rand() values of this block's random stream."""
+        out = np.zeros(n, np.int32)
+        _check(lib().mmca_rand(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), n), "mmca_rand")
+        return out
+
+    def io_fields(self):
+        """cache_io_data (loop.c:74-113): FP32 fs, ce, temperature and gr,
+        padded, as the output writers read them."""
+        fs = np.empty(self.shape, np.float32)
+        ce = np.empty(self.shape, np.float32)
+        t = np.empty(self.shape, np.float32)
+        gr = np.empty(self.shape, np.int32)
+        _check(lib().mmca_io_fields(self._h, fs.ctypes.data, ce.ctypes.data, t.ctypes.data, gr.ctypes.data),
+               "mmca_io_fields")
+        return {"fs": fs, "ce": ce, "temperature": t, "gr": gr}
 
     def set_grains(self, grains):
         """Replace the grain table (list of abi.Grain, record 0 = liquid)."""

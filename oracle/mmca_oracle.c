@@ -1187,6 +1187,15 @@ orc_field(orc_state *s, int which)
 }
 
 void orc_set_record(orc_state *s, uint8_t *rec) { s->rec = rec; }
+/* n raw rand() values of the block's stream (the XDMF writer's grain
+ * renumbering draws them between steps, io_xdmf.c:345-376) */
+void
+orc_rand(orc_state *s, int n, int32_t *out)
+{
+    for (int i = 0; i < n; i++)
+        out[i] = rng_next(&s->rng);
+}
+
 /* threads the parallel loops use (1 without OpenMP) */
 int
 orc_threads(void)

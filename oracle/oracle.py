@@ -65,6 +65,8 @@ def lib():
         L.orc_failed_step.argtypes = [vp]
         L.orc_set_grains.argtypes = [vp, C.POINTER(abi.Grain), C.c_int]
         L.orc_threads.restype = C.c_int
+        L.orc_rand.argtypes = [vp, C.c_int, C.POINTER(C.c_int32)]
+        L.orc_rand.restype = None
         L.orc_set_record.argtypes = [vp, C.c_void_p]
         L.orc_set_record.restype = None
         L.orc_set_timestep.argtypes = [vp, C.c_uint64]
@@ -155,6 +157,11 @@ class OracleSubblock:
 
     def download(self, which):
         return self.field(which).copy()
+
+    def rand(self, n):
+        out = np.zeros(n, np.int32)
+        lib().orc_rand(self._h, n, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
 
     def set_grains(self, grains):
         arr = (abi.Grain * len(grains))(*grains)

@@ -80,3 +80,4 @@ def test_am_substrate_matches_oracle():
     assert melted > final.size // 4                # the beam remelted part of the substrate
     assert np.sum(final <= 0) < melted // 2        # which regrows from the grains that survived
     assert set(np.unique(final[final > 0]).tolist()) <= set(range(1, 41))
+

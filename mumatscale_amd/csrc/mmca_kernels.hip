@@ -48,6 +48,16 @@ ld_cl(const DevBufs &b, const Consts &c, int buf, long long i)
     return F32 ? c.cinit + (double) cl32_buf(b, buf)[i] : cl_buf(b, buf)[i];
 }
 
+// The solute kernel writes each cell of ce, cl and fs once per step and the
+// later kernels of the step read few of them back (capture), so these stores
+// are non-temporal: streamed past the caches, coherent at the kernel's end.
+template <typename T>
+__device__ __forceinline__ void
+st_stream(MMCA_GLOBAL T *p, long long i, T v)
+{
+    __builtin_nontemporal_store(v, &p[i]);
+}
+
 // store ce; returns the FP64 solute lost to the FP32 rounding (0 for FP64)
 template <bool F32>
 __device__ __forceinline__ double
@@ -57,10 +67,10 @@ st_ce(const DevBufs &b, const Consts &c, long long i, double v)
     {
         const double dev = v - c.cinit;
         const float f = (float) dev;
-        G(b.ce32)[i] = f;
+        st_stream(G(b.ce32), i, f);
         return dev - (double) f;
     }
-    G(b.ce)[i] = v;
+    st_stream(G(b.ce), i, v);
     return 0.0;
 }
 
@@ -69,9 +79,9 @@ __device__ __forceinline__ void
 st_cl(const DevBufs &b, const Consts &c, int buf, long long i, double v)
 {
     if (F32)
-        cl32_buf(b, buf)[i] = (float) (v - c.cinit);
+        st_stream(cl32_buf(b, buf), i, (float) (v - c.cinit));
     else
-        cl_buf(b, buf)[i] = v;
+        st_stream(cl_buf(b, buf), i, v);
 }
 
 // Compensated (Kahan) addition of a workgroup's residual into its ledger slot.
@@ -402,6 +412,7 @@ flag_negative_ce(Counters *cnt)
 // into the domain's branch record.
 constexpr int SOL_BX = 64;
 constexpr int SOL_BY = 4;
+constexpr int SOL_BZ = 8;    // planes per workgroup: spreads the per-workgroup set-up over 8 x 256 cells
 
 template <bool DIFF, bool FSC, bool AM, bool F32, bool REC = false>
 __global__ void __launch_bounds__(SOL_BX *SOL_BY)
@@ -410,13 +421,16 @@ k_solute(View V, int cur, int nxt)
     __shared__ double s_red[SOL_BX * SOL_BY / 64];
     __shared__ int s_stop;
     const int dm = find_dom(V.off_cell, V.ndom, blockIdx.x);
-    const long long blk = blockIdx.x - V.off_cell[dm];
     const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
     const Consts c = V.dom[dm].c;
-    const int gbx = (c.nx + SOL_BX - 1) / SOL_BX, gby = (c.ny + SOL_BY - 1) / SOL_BY;
-    const int bx = (int) (blk % gbx);
-    const int by = (int) ((blk / gbx) % gby);
-    const int bz = (int) (blk / ((long long) gbx * gby));
+    // workgroup -> 64 x 4 tile and SOL_BZ planes (32-bit index arithmetic:
+    // a domain has fewer than 2^31 cells)
+    const unsigned blk = (unsigned) (blockIdx.x - V.off_cell[dm]);
+    const unsigned gbx = (unsigned) (c.nx + SOL_BX - 1) / SOL_BX, gby = (unsigned) (c.ny + SOL_BY - 1) / SOL_BY;
+    const unsigned rest = blk / gbx;
+    const int bx = (int) (blk - rest * gbx);
+    const int by = (int) (rest % gby);
+    const int bz = (int) (rest / gby);
     Counters *cnt = b.cnt;
     // this kernel itself may raise the status: read it once per workgroup so
     // that every wave takes the same branch before the barrier of block_sum
@@ -427,66 +441,70 @@ k_solute(View V, int cur, int nxt)
         return;
     const int x = bx * SOL_BX + threadIdx.x + 1;   // interior x
     const int j = by * SOL_BY + threadIdx.y + 1;
-    const int k = bz + 1;
+    const int k0 = bz * SOL_BZ + 1;
+    const int k1 = min(c.nz, k0 + SOL_BZ - 1);
     const bool rowok = (j <= c.ny);
     const bool active = rowok && (x <= c.nx);
-    int g_new = 0;
+    const double sim_time = G(b.cnt)->timestep * c.dt;
     double lost = 0.0;                                   // FP32 storage rounding of ce
-    if (active)
+    for (int k = k0; k <= k1; ++k)
     {
-        const long long idx = k * c.sxy + (long long) j * c.sx + x;
-        const MMCA_GLOBAL double *fsc = fs_buf(b, cur);
-        const double ce_old = ld_ce<F32>(b, c, idx);
-        const double conc = ld_cl<F32>(b, c, cur, idx);
-        const double lspfs = fsc[idx];
-        double ce_new = ce_old;
-        if (DIFF)
+        int g_new = 0;
+        if (active)
         {
-            const long long nb[6] = { idx + 1, idx - 1, idx + c.sx, idx - c.sx, idx + c.sxy, idx - c.sxy };
-            // mold of a neighbour: a padded face of the box (interior cells are never mold)
-            const int mo[6] = { (x == c.nx) ? c.pad[0] : 0, (x == 1) ? c.pad[0] : 0,
-                                (j == c.ny) ? c.pad[1] : 0, (j == 1) ? c.pad[1] : 0,
-                                (k == c.nz) ? c.pad[2] : 0, (k == 1) ? c.pad[2] : 0 };
-            double cln[6], fsn[6];
-#pragma unroll
-            for (int n = 0; n < 6; ++n)
+            const long long idx = k * c.sxy + (long long) j * c.sx + x;
+            const MMCA_GLOBAL double *fsc = fs_buf(b, cur);
+            const double ce_old = ld_ce<F32>(b, c, idx);
+            const double conc = ld_cl<F32>(b, c, cur, idx);
+            const double lspfs = fsc[idx];
+            double ce_new = ce_old;
+            if (DIFF)
             {
-                cln[n] = ld_cl<F32>(b, c, cur, nb[n]);
-                fsn[n] = fsc[nb[n]];
+                const long long nb[6] = { idx + 1, idx - 1, idx + c.sx, idx - c.sx, idx + c.sxy, idx - c.sxy };
+                // mold of a neighbour: a padded face of the box (interior cells are never mold)
+                const int mo[6] = { (x == c.nx) ? c.pad[0] : 0, (x == 1) ? c.pad[0] : 0,
+                                    (j == c.ny) ? c.pad[1] : 0, (j == 1) ? c.pad[1] : 0,
+                                    (k == c.nz) ? c.pad[2] : 0, (k == 1) ? c.pad[2] : 0 };
+                double cln[6], fsn[6];
+#pragma unroll
+                for (int n = 0; n < 6; ++n)
+                {
+                    cln[n] = ld_cl<F32>(b, c, cur, nb[n]);
+                    fsn[n] = fsc[nb[n]];
+                }
+                ce_new = diffuse_cell(c, ce_old, conc, lspfs, cln, fsn, mo);
+                if (c.check_neg && ce_new < 0)
+                    flag_negative_ce(cnt);
+                lost += st_ce<F32>(b, c, idx, ce_new);
             }
-            ce_new = diffuse_cell(c, ce_old, conc, lspfs, cln, fsn, mo);
-            if (c.check_neg && ce_new < 0)
-                flag_negative_ce(cnt);
-            lost = st_ce<F32>(b, c, idx, ce_new);
+            if (FSC)
+            {
+                const double ce_prev = DIFF ? ce_old : G(b.oce)[idx];
+                const int g_old = G(b.gr)[idx];
+                double cl_out, fs_out;
+                int g, branch;
+                partition_cell(c, [&]() { return temp_model<AM>(c, sim_time, x, j, k); }, ce_new, ce_prev,
+                               conc, lspfs, g_old, &cl_out, &fs_out, &g, &branch);
+                if (REC)
+                {
+                    // branch record of this step (bounds: set by the host, checked here too)
+                    const long long rs = (long long) (G(b.cnt)->timestep - b.rec_t0) - 1;
+                    if (rs >= 0 && rs < b.rec_steps)
+                        G(b.rec)[rs * c.P + idx] = (unsigned char) branch;
+                }
+                st_cl<F32>(b, c, nxt, idx, cl_out);
+                st_stream(fs_buf(b, nxt), idx, fs_out);
+                if (g != g_old)
+                    G(b.gr)[idx] = g;
+                g_new = g;
+            }
         }
         if (FSC)
         {
-            const double ce_prev = DIFF ? ce_old : G(b.oce)[idx];
-            const int g_old = G(b.gr)[idx];
-            double cl_out, fs_out;
-            int g, branch;
-            const double sim_time = G(b.cnt)->timestep * c.dt;
-            partition_cell(c, [&]() { return temp_model<AM>(c, sim_time, x, j, k); }, ce_new, ce_prev, conc,
-                           lspfs, g_old, &cl_out, &fs_out, &g, &branch);
-            if (REC)
-            {
-                // branch record of this step (bounds: set by the host, checked here too)
-                const long long rs = (long long) (G(b.cnt)->timestep - b.rec_t0) - 1;
-                if (rs >= 0 && rs < b.rec_steps)
-                    G(b.rec)[rs * c.P + idx] = (unsigned char) branch;
-            }
-            st_cl<F32>(b, c, nxt, idx, cl_out);
-            fs_buf(b, nxt)[idx] = fs_out;
-            if (g != g_old)
-                G(b.gr)[idx] = g;
-            g_new = g;
+            unsigned long long bits = __ballot(active && g_new > 0);
+            if (threadIdx.x == 0 && rowok)
+                G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
         }
-    }
-    if (FSC)
-    {
-        unsigned long long bits = __ballot(active && g_new > 0);
-        if (threadIdx.x == 0 && rowok)
-            G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
     }
     if (F32)
     {
@@ -668,7 +686,7 @@ k_solute_zm(View V, int cur, int nxt)
                 },
                 ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out, &g_new, &branch);
             st_cl<F32>(b, c, nxt, idx, cl_out);
-            fsn_out[idx] = fs_out;
+            st_stream(fsn_out, idx, fs_out);
             if (g_new != g_old)
                 G(b.gr)[idx] = g_new;
         }
@@ -725,13 +743,24 @@ mask_word(const DevBufs &b, const Consts &c, int k, int j, int w)
     return G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + w];
 }
 
+// mask word wi (< nz * ny * w64 < 2^31) -> padded plane k, row j, word w;
+// 32-bit arithmetic (64-bit division is a long instruction sequence)
+__device__ __forceinline__ void
+word_pos(const Consts &c, long long wi, int *k, int *j, int *w)
+{
+    const unsigned u = (unsigned) wi, w64 = (unsigned) c.w64, ny = (unsigned) c.ny;
+    const unsigned nrow = u / w64;
+    *w = (int) (u - nrow * w64);
+    const unsigned kk = nrow / ny;
+    *j = (int) (nrow - kk * ny) + 1;
+    *k = (int) kk + 1;
+}
+
 __device__ __forceinline__ unsigned long long
 member_bits(const DevBufs &b, const Consts &c, long long wi, int *pk, int *pj, int *pw)
 {
-    const long long nrow = wi / c.w64;
-    const int w = (int) (wi - nrow * c.w64);
-    const int j = (int) (nrow % c.ny) + 1;
-    const int k = (int) (nrow / c.ny) + 1;
+    int k, j, w;
+    word_pos(c, wi, &k, &j, &w);
     *pk = k;
     *pj = j;
     *pw = w;
@@ -876,10 +905,8 @@ k_list_write(View V)
     unsigned long long nb[6] = { 0, 0, 0, 0, 0, 0 };
     if (wi < nwords)
     {
-        const long long nrow = wi / c.w64;
-        const int w = (int) (wi - nrow * c.w64);
-        const int j = (int) (nrow % c.ny) + 1;
-        const int k = (int) (nrow / c.ny) + 1;
+        int k, j, w;
+        word_pos(c, wi, &k, &j, &w);
         const unsigned long long own = mask_word(b, c, k, j, w);
         // bit b of each word: the neighbour of interior x = 64 w + b + 1 in the
         // capture order +x, +y, -x, -y, +z, -z (growth.c:604-610)
@@ -919,11 +946,8 @@ k_list_write(View V)
                 hi = mid - 1;
         }
         const int bit = nth_set_bit(s_mem[lo], i - s_ex[lo]);
-        const long long wq = wi0 + lo;
-        const long long nrow = wq / c.w64;
-        const int w = (int) (wq - nrow * c.w64);
-        const int j = (int) (nrow % c.ny) + 1;
-        const int k = (int) (nrow / c.ny) + 1;
+        int k, j, w;
+        word_pos(c, wi0 + lo, &k, &j, &w);
         const int x = w * 64 + bit + 1;
         unsigned int pattern = 0;
 #pragma unroll
@@ -1088,10 +1112,12 @@ k_reduce_partial(DevBufs b, Consts c, int cb)
     for (long long i = (long long) blockIdx.x * RED_BS + threadIdx.x; i < N;
          i += (long long) gridDim.x * RED_BS)
     {
-        long long row = i / c.nx;
-        int x = (int) (i - row * c.nx) + 1;
-        int j = (int) (row % c.ny) + 1;
-        int k = (int) (row / c.ny) + 1;
+        const unsigned ui = (unsigned) i, unx = (unsigned) c.nx, uny = (unsigned) c.ny;   // N < 2^31
+        const unsigned row = ui / unx;
+        int x = (int) (ui - row * unx) + 1;
+        const unsigned kk = row / uny;
+        int j = (int) (row - kk * uny) + 1;
+        int k = (int) kk + 1;
         long long idx = k * c.sxy + (long long) j * c.sx + x;
         a += ref_min(1.0, fs[idx]);
         s += c.f32 ? (double) G(b.ce32)[idx] : G(b.ce)[idx];     // FP32: deviation from c0
@@ -1304,7 +1330,8 @@ int list_blocks(const Consts &c) { return (int) ((list_words(c) + LIST_BS - 1) /
 int nuc_blocks(int nsites) { return (nsites + NUC_BS - 1) / NUC_BS; }
 long long cell_blocks(const Consts &c)
 {
-    return (long long) ((c.nx + SOL_BX - 1) / SOL_BX) * ((c.ny + SOL_BY - 1) / SOL_BY) * c.nz;
+    return (long long) ((c.nx + SOL_BX - 1) / SOL_BX) * ((c.ny + SOL_BY - 1) / SOL_BY) *
+           ((c.nz + SOL_BZ - 1) / SOL_BZ);
 }
 long long zm_tiles(const Consts &c, int by)
 {

@@ -277,6 +277,71 @@ def run_am_track(steps=60):
     return out
 
 
+def _split(dims, ranks):
+    """Subblock counts per axis for `ranks` MPI ranks (z first, then y, x)."""
+    nsub = [1, 1, 1]
+    left = ranks
+    for a in (2, 1, 0):
+        for f in range(left, 0, -1):
+            if left % f == 0 and dims[a] % (nsub[a] * f) == 0 and dims[a] // (nsub[a] * f) >= 16:
+                nsub[a] *= f
+                left //= f
+                break
+    return nsub if left == 1 else None
+
+
+def reference_baseline(cfg, ranks, lo=2, hi=14, repeats=2):
+    """The reference's own CPU build (oracle/_ref, built from its sources) on
+    `ranks` MPI ranks of the host, one OpenMP thread each, the domain split
+    into as many subblocks.
+    Per-step time = (elapsed(hi steps) - elapsed(lo steps)) / (hi - lo), which
+    cancels the set-up and the end-of-run grain statistics; each elapsed time
+    is the least of `repeats` runs.  Both step counts stay below the step
+    (about 18 on 8 ranks) at which this workload's Courant check stops the CPU
+    build.  None when the build or
+    mpiexec is missing or a run fails."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    from mumatscale_amd.ctrl import write_ctrl
+    ref = os.path.join(ROOT, "oracle", "_ref", "mumatscale_ref")
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    nsub = _split(cfg.gdim, ranks)
+    if not (os.path.exists(ref) and os.path.exists(mpiexec) and nsub):
+        return None
+    times = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for steps in (lo, hi) * repeats:
+            c = directional_config(*cfg.gdim)
+            c.finish_time = steps * c.dt
+            c.screenpfreq = 100000
+            c.basefilename = "refbase"
+            path = os.path.join(tmp, "run%d.in" % steps)
+            write_ctrl(c, path, nsub=nsub)
+            try:
+                r = subprocess.run([mpiexec, "-hosts", "127.0.0.1", "-n", str(ranks), ref, "-c", path], cwd=tmp,
+                                   env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True, text=True,
+                                   timeout=600)
+            except (OSError, subprocess.SubprocessError) as e:
+                log("reference baseline: %s" % e)
+                return None
+            m = re.search(r"Elapsed time:\s+(\d+) iterations, ([0-9.eE+-]+) sec", r.stdout)
+            if r.returncode != 0 or not m or int(m.group(1)) != steps:
+                log("reference baseline: run of %d steps failed (rc %d)" % (steps, r.returncode))
+                return None
+            times[steps] = min(times.get(steps, float("inf")), float(m.group(2)))
+    n = cfg.gdim[0] * cfg.gdim[1] * cfg.gdim[2]
+    per_step = (times[hi] - times[lo]) / (hi - lo)
+    if per_step <= 0:
+        log("reference baseline: elapsed times %s do not grow with the step count" % times)
+        return None
+    return {"value": round(n / per_step / 1e6, 3), "unit": "Mcell-updates/s", "cores": ranks, "kind": "reference",
+            "sample": "reference CPU build, %d MPI ranks (subblocks %dx%dx%d), full-size workload, steps %d-%d: "
+                      "elapsed(%d steps) %.2f s - elapsed(%d steps) %.2f s, least of %d runs each"
+                      % (ranks, nsub[0], nsub[1], nsub[2], lo + 1, hi, hi, times[hi], lo, times[lo], repeats)}
+
+
 def cpu_baseline(cfg, target_s):
     """The oracle (C restatement of the reference path) on the host cores:
     its per-cell loops run with OpenMP (OMP_NUM_THREADS threads), list
@@ -368,8 +433,15 @@ def main():
             log("am track:", extra["am_track"])
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, args.cpu_seconds)
-        log("cpu baseline:", cpu)
+        ranks = min(16, os.cpu_count() or 1)
+        cpu = reference_baseline(cfg, ranks)
+        log("reference cpu baseline:", cpu)
+        port = cpu_baseline(cfg, args.cpu_seconds)
+        log("oracle cpu baseline:", port)
+        if cpu is None:
+            cpu = port
+        else:
+            cpu["oracle_port"] = port
     out = {
         "metric": METRIC,
         "value": round(value, 1),

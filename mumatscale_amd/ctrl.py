@@ -301,5 +301,74 @@ class Config:
         return (g[0] * g[1] * g[2]) * math.pow(self.cell_size, 3)
 
 
+def write_ctrl(cfg, path, nsub=None):
+    """Write `cfg` as a control file the reference reads (read_ctrl.c keys),
+    optionally with another subblock decomposition `nsub` of the same
+    domain.  Round-trips through Config.read."""
+    nsub = list(cfg.nsub) if nsub is None else list(nsub)
+    g = cfg.gdim
+    if any(g[a] % nsub[a] for a in range(3)):
+        raise ValueError("domain %s does not split into %s subblocks" % (g, nsub))
+    sb = [g[a] // nsub[a] for a in range(3)]
+    r = repr
+    L = cfg.laser
+    lines = [
+        "BaseFileName %s" % cfg.basefilename,
+        "NSubblocks %d %d %d" % tuple(nsub),
+        "NCellsPerSB %d %d %d" % tuple(sb),
+        "FaceCtrl %d %d %d %d %d %d" % (cfg.pad_bndy[0], cfg.pad_bndy[0], cfg.pad_bndy[1], cfg.pad_bndy[1],
+                                        cfg.pad_bndy[2], cfg.pad_bndy[2]),
+        "CellSize %s" % r(cfg.cell_size),
+        "CourantLimit %s" % r(cfg.courant_limit),
+        "FSGrow %s" % r(cfg.fsgrow),
+        "TimeStep %s" % r(cfg.dt),
+        "RandSeedVal %d" % cfg.random_seed,
+        "FinishTime %s" % r(cfg.finish_time),
+        "FsFinish %s" % r(cfg.fs_finish),
+        "ScreenPFreq %d" % cfg.screenpfreq,
+        "LiquidusTemp %s" % r(cfg.liquidus_temp),
+        "SolidusTemp %s" % r(cfg.solidus_temp),
+        "Cinit %s" % r(cfg.c_init),
+        "m_solute0 %s %s %s" % tuple(r(v) for v in cfg.m_solute),
+        "part_coef0 0 0 %s" % r(cfg.part_coef),
+        "Dsol %s" % r(cfg.d_sol),
+        "Dliq %s" % r(cfg.d_liq),
+        "InitialTemperature %s" % r(cfg.t_init),
+        "CoolingRate %s" % r(cfg.cool_rate),
+        "Gradient %s" % r(cfg.gradient),
+        "Grad_Coef %s" % r(cfg.grad_coef),
+        "Velocity %s" % r(cfg.velocity),
+        "Velo_Coef %s" % r(cfg.velo_coef),
+        "MaxGrainDensity %s" % r(cfg.bulk_density),
+        "GNGaussCentre %s" % r(cfg.bulk_mean),
+        "GNGaussSigma %s" % r(cfg.bulk_sigma),
+        "MaxGrainDensitySurf %s" % r(cfg.surf_density),
+        "GNGaussCentreSurf %s" % r(cfg.surf_mean),
+        "GNGaussSigmaSurf %s" % r(cfg.surf_sigma),
+        "MaxTotGrains %d" % cfg.max_total_grains,
+        "MouldNuc %d" % cfg.do_surface,
+        "GNOriented %d" % cfg.oriented,
+        "AnalyticalThermal %d" % cfg.thermal_ana,
+        "LaserPower %s" % r(L["power"]),
+        "LaserSpeed %s" % r(L["speed"]),
+        "LaserSpotSize %s" % r(L["spot"]),
+        "LaserAbsorption %s" % r(L["absorp"]),
+        "LaserHatchSpacing %s" % r(L["hatch"]),
+        "SubInitialTemp %s" % r(L["sub_init_temp"]),
+        "ThermalConductivity %s" % r(L["th_conductivity"]),
+        "Density %s" % r(L["density"]),
+        "SpecificHeat %s" % r(L["specific_heat"]),
+        "LaserStartLocation %s %s %s" % tuple(r(v) for v in L["start"]),
+        "NoOfLayers %d" % cfg.am_layers,
+        "NoOfTracks %d" % cfg.am_tracks,
+    ]
+    for f in cfg.fixed:
+        lines.append("FixedNuc %s %s %s %s %s %s %s" % tuple(r(v) for v in f))
+    for v in cfg.volumes:
+        lines.append("NucVolume %s %s %s %s %s %s %s %s" % ((r(v[0] * 100.0),) + tuple(r(x) for x in v[1:])))
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
 def read_ctrl(path):
     return Config().read(path)

@@ -95,3 +95,36 @@ def test_one_grain_through_c_driver(tmp_path):
     assert len(csv) == 2
     size = float(csv[1].split(",")[-1])
     assert math.isclose(size, 0.000737875, rel_tol=1e-6)
+
+
+@pytest.mark.parametrize("path", CTRL_FILES, ids=[os.path.basename(p) for p in CTRL_FILES])
+def test_write_ctrl_round_trips(path, tmp_path):
+    """write_ctrl (used to hand a workload to the reference's CPU build) reads
+    back to the same parameters, also under another subblock split."""
+    from mumatscale_amd.ctrl import write_ctrl
+    cfg = read_ctrl(path)
+    splits = [list(cfg.nsub)]
+    if cfg.gdim[2] % (2 * cfg.nsub[2]) == 0:
+        splits.append([cfg.nsub[0], cfg.nsub[1], 2 * cfg.nsub[2]])
+    for nsub in splits:
+        out = tmp_path / "w.in"
+        write_ctrl(cfg, str(out), nsub=nsub)
+        back = read_ctrl(str(out))
+        assert list(back.nsub) == nsub
+        assert list(back.gdim) == list(cfg.gdim)
+        a, b = cfg.params(), back.params()
+        for name, _ in a._fields_:
+            if name in ("nx", "ny", "nz", "gnsbx"):
+                continue
+            va, vb = getattr(a, name), getattr(b, name)
+            if hasattr(va, "__len__"):
+                va, vb = list(va), list(vb)
+            assert va == vb, name
+        na, nb = cfg.nucleation(), back.nucleation()
+        for name, _ in na._fields_:
+            if name not in ("fixed", "volumes"):
+                assert getattr(na, name) == getattr(nb, name), name
+        assert [list(t) for t in back.fixed] == [list(t) for t in cfg.fixed]
+        assert [list(t) for t in back.volumes] == [list(t) for t in cfg.volumes]
+        assert (back.finish_time, back.fs_finish, back.basefilename) == (cfg.finish_time, cfg.fs_finish,
+                                                                          cfg.basefilename)

@@ -514,15 +514,71 @@ k_solute(View V, int cur, int nxt)
     }
 }
 
+// Raw storage of ce and cl (FP64 values, or FP32 deviations from c0), kept
+// unconverted while a load is in flight, so that a prefetched value is only
+// waited for where it is used.
+template <bool F32> struct Store { using T = double; };
+template <> struct Store<true> { using T = float; };
+
+template <bool F32>
+__device__ __forceinline__ typename Store<F32>::T
+raw_ce(const DevBufs &b, long long i)
+{
+    if constexpr (F32)
+        return G(b.ce32)[i];
+    else
+        return G(b.ce)[i];
+}
+
+template <bool F32>
+__device__ __forceinline__ typename Store<F32>::T
+raw_cl(const DevBufs &b, int buf, long long i)
+{
+    if constexpr (F32)
+        return cl32_buf(b, buf)[i];
+    else
+        return cl_buf(b, buf)[i];
+}
+
+template <bool F32>
+__device__ __forceinline__ double
+from_raw(const Consts &c, typename Store<F32>::T r)
+{
+    if constexpr (F32)
+        return c.cinit + (double) r;
+    else
+        return r;
+}
+
+// What the z-marching kernel loads for plane q one plane ahead: ce and gr of
+// q, the tile-edge neighbours of q, and cl and fs of q+1 (the +z neighbour at
+// plane q, the cell's own value at q+1).
+template <bool F32>
+struct ZmAhead
+{
+    typename Store<F32>::T ce, cl2, clx, cly;
+    double fs2, fsx, fsy;
+    int g;
+};
+
 // z-marching fused kernel: a workgroup of 64 x BY threads owns a 64 x BY
 // column tile and walks `kz` planes upward.  cl/fs of the planes k-1, k, k+1
-// stay in registers, x neighbours come from the neighbouring lanes
-// (wave shuffles), y neighbours from the rows of the other waves in LDS;
-// only tile edges load neighbours from memory.
+// stay in registers, x neighbours come from the neighbouring lanes (wave
+// shuffles), y neighbours from the rows of the other waves in LDS.
+//
+// Every load of plane k+1 (ce, gr, the tile-edge neighbours, and cl, fs of
+// plane k+2) is issued before plane k is computed, into one of two register
+// sets used in turn, so that it is in flight for a whole plane.  Lanes one
+// past the last column or row of the domain load the padding cell as their
+// own value (and compute nothing), so the cell next to the boundary reads it
+// by shuffle or LDS like any other neighbour; then only lane 0 (x-1) and lane
+// 63 (x+1), and only row 0 (y-1) and row BY-1 (y+1), read neighbours from
+// memory, one of each kind per lane.
 template <int BY, bool AM, bool F32>
 __global__ void __launch_bounds__(64 * BY)
 k_solute_zm(View V, int cur, int nxt)
 {
+    static_assert(BY >= 2, "row 0 and row BY-1 must be different waves");
     __shared__ double s_cl[BY][64];
     __shared__ double s_fs[BY][64];
     __shared__ double s_red[BY];
@@ -570,72 +626,63 @@ k_solute_zm(View V, int cur, int nxt)
     const int k1 = min(c.nz, k0 + kz - 1);
     const bool rowok = (j <= c.ny);
     const bool active = rowok && (x <= c.nx);
+    // Own cl/fs: every computing cell, and the padding cells one past the
+    // last column and row (the neighbours of the boundary cells).  Loads are
+    // issued by every lane, at addresses clamped into the padded box, so that
+    // no branch skips one and each wait covers exactly the plane it needs.
+    const int xc = min(x, c.nx + 1), jc = min(j, c.ny + 1);
+    const long long col = (long long) jc * c.sx + xc;
+    const long long row0 = (long long) jc * c.sx + bx * 64;   // x-1 of lane 0
     const MMCA_GLOBAL double *fsc = fs_buf(b, cur);
     MMCA_GLOBAL double *fsn_out = fs_buf(b, nxt);
-    const long long col = active ? ((long long) j * c.sx + x) : 0;
     const double sim_time = G(b.cnt)->timestep * c.dt;
-    const bool ylo_lds = (ty > 0);
-    const bool yhi_lds = (ty + 1 < BY) && (j + 1 <= c.ny);
-    const bool xlo_mem = (lane == 0);
-    const bool xhi_mem = (lane == 63) || (x == c.nx);
+    const bool xm_mem = (lane == 0);
+    const bool xp_mem = (lane == 63);
+    const bool ym_mem = (ty == 0);
+    const bool yp_mem = (ty == BY - 1);
+    // tile-edge neighbours: lane 63 reads x+1, every other lane the x-1 of
+    // lane 0 (one shared line); row 0 reads y-1, row BY-1 y+1, the other rows
+    // the first cell of their row
+    const long long xe = xp_mem ? col + 1 : row0;
+    const long long ye = ym_mem ? col - c.sx : yp_mem ? col + c.sx : row0 + 1;
     const int mo_x1 = (x == c.nx) ? c.pad[0] : 0, mo_x0 = (x == 1) ? c.pad[0] : 0;
     const int mo_y1 = (j == c.ny) ? c.pad[1] : 0, mo_y0 = (j == 1) ? c.pad[1] : 0;
-    // cl/fs of planes k-1, k, k+1 in registers; plane k+2 is in flight
-    double clm = 0, clv = 0, clp = 0, clq = 0, fsm = 0, fsv = 0, fsp = 0, fsq = 0;
-    double ce_nx = 0;
-    int g_nx = 0;
+    auto fetch = [&](ZmAhead<F32> &a, int q) {
+        // plane q's ce, gr and edge neighbours; plane q+1's cl and fs
+        const long long pq = q * c.sxy;
+        a.cl2 = raw_cl<F32>(b, cur, pq + c.sxy + col);
+        a.fs2 = fsc[pq + c.sxy + col];
+        a.ce = raw_ce<F32>(b, pq + col);
+        a.g = G(b.gr)[pq + col];
+        a.clx = raw_cl<F32>(b, cur, pq + xe);
+        a.fsx = fsc[pq + xe];
+        a.cly = raw_cl<F32>(b, cur, pq + ye);
+        a.fsy = fsc[pq + ye];
+    };
+    double clm = from_raw<F32>(c, raw_cl<F32>(b, cur, (k0 - 1) * c.sxy + col));
+    double fsm = fsc[(k0 - 1) * c.sxy + col];
+    double clv = from_raw<F32>(c, raw_cl<F32>(b, cur, k0 * c.sxy + col));
+    double fsv = fsc[k0 * c.sxy + col];
+    ZmAhead<F32> A, B;
+    fetch(B, k0);
+    // the loop starts with nothing in flight, so that its waits are the
+    // steady-state ones
+    __builtin_amdgcn_s_waitcnt(0);
     double lost = 0.0;
-    if (active)
-    {
-        clm = ld_cl<F32>(b, c, cur, (k0 - 1) * c.sxy + col);
-        fsm = fsc[(k0 - 1) * c.sxy + col];
-        clv = ld_cl<F32>(b, c, cur, k0 * c.sxy + col);
-        fsv = fsc[k0 * c.sxy + col];
-        clp = ld_cl<F32>(b, c, cur, (k0 + 1) * c.sxy + col);
-        fsp = fsc[(k0 + 1) * c.sxy + col];
-        ce_nx = ld_ce<F32>(b, c, k0 * c.sxy + col);
-        g_nx = G(b.gr)[k0 * c.sxy + col];
-    }
-    for (int k = k0; k <= k1; ++k)
-    {
+    bool neg = false;                             // a negative ce seen by this lane
+    // one plane: `ld` receives plane k+1, `use` holds plane k
+    auto plane = [&](int k, ZmAhead<F32> &ld, ZmAhead<F32> &use) {
         const long long idx = k * c.sxy + col;
-        const double ce_old = ce_nx;
-        const int g_old = g_nx;
-        double clxm = 0, fsxm = 0, clxp = 0, fsxp = 0, clym = 0, fsym = 0, clyp = 0, fsyp = 0;
-        if (active)
+        if (k > k0)
         {
-            // loads for later planes first: plane k+2 (cl, fs; up to the halo
-            // plane k1+1) and plane k+1 (ce, gr)
-            if (k + 1 <= k1)
-            {
-                clq = ld_cl<F32>(b, c, cur, idx + 2 * c.sxy);
-                fsq = fsc[idx + 2 * c.sxy];
-                ce_nx = ld_ce<F32>(b, c, idx + c.sxy);
-                g_nx = G(b.gr)[idx + c.sxy];
-            }
-            // tile-edge neighbours of plane k come from memory; issued before
-            // the barrier so that their latency overlaps it
-            if (xlo_mem)
-            {
-                clxm = ld_cl<F32>(b, c, cur, idx - 1);
-                fsxm = fsc[idx - 1];
-            }
-            if (xhi_mem)
-            {
-                clxp = ld_cl<F32>(b, c, cur, idx + 1);
-                fsxp = fsc[idx + 1];
-            }
-            if (!ylo_lds)
-            {
-                clym = ld_cl<F32>(b, c, cur, idx - c.sx);
-                fsym = fsc[idx - c.sx];
-            }
-            if (!yhi_lds)
-            {
-                clyp = ld_cl<F32>(b, c, cur, idx + c.sx);
-                fsyp = fsc[idx + c.sx];
-            }
+            clm = clv;
+            fsm = fsv;
+            clv = from_raw<F32>(c, ld.cl2);       // landed: it was plane k-1's +z neighbour
+            fsv = ld.fs2;
         }
+        // on the last plane this reloads plane k1 (unused): every plane
+        // issues the same loads, so every wait is the same
+        fetch(ld, min(k + 1, k1));
         if constexpr (AM)
             beam_tab_plane(c, *beam, k, ty * 64 + lane, 64 * BY);   // read after the barrier below
         s_cl[ty][lane] = clv;
@@ -646,33 +693,22 @@ k_solute_zm(View V, int cur, int nxt)
         int g_new = 0;
         if (active)
         {
-            if (!xlo_mem)
-            {
-                clxm = shxm;
-                fsxm = sfxm;
-            }
-            if (!xhi_mem)
-            {
-                clxp = shxp;
-                fsxp = sfxp;
-            }
-            if (ylo_lds)
-            {
-                clym = s_cl[ty - 1][lane];
-                fsym = s_fs[ty - 1][lane];
-            }
-            if (yhi_lds)
-            {
-                clyp = s_cl[ty + 1][lane];
-                fsyp = s_fs[ty + 1][lane];
-            }
+            const double ex_cl = from_raw<F32>(c, use.clx), ey_cl = from_raw<F32>(c, use.cly);
+            const double clxm = xm_mem ? ex_cl : shxm, fsxm = xm_mem ? use.fsx : sfxm;
+            const double clxp = xp_mem ? ex_cl : shxp, fsxp = xp_mem ? use.fsx : sfxp;
+            const double clym = ym_mem ? ey_cl : s_cl[ym_mem ? 0 : ty - 1][lane];
+            const double fsym = ym_mem ? use.fsy : s_fs[ym_mem ? 0 : ty - 1][lane];
+            const double clyp = yp_mem ? ey_cl : s_cl[yp_mem ? 0 : ty + 1][lane];
+            const double fsyp = yp_mem ? use.fsy : s_fs[yp_mem ? 0 : ty + 1][lane];
+            const double clp = from_raw<F32>(c, use.cl2), fsp = use.fs2;
+            const double ce_old = from_raw<F32>(c, use.ce);
+            const int g_old = use.g;
             const double cln[6] = { clxp, clxm, clyp, clym, clp, clm };
             const double fsn[6] = { fsxp, fsxm, fsyp, fsym, fsp, fsm };
             const int mo[6] = { mo_x1, mo_x0, mo_y1, mo_y0, (k == c.nz) ? c.pad[2] : 0,
                                 (k == 1) ? c.pad[2] : 0 };
             const double ce_new = diffuse_cell(c, ce_old, clv, fsv, cln, fsn, mo);
-            if (c.check_neg && ce_new < 0)
-                flag_negative_ce(cnt);
+            neg |= (ce_new < 0);
             lost += st_ce<F32>(b, c, idx, ce_new);
             double cl_out, fs_out;
             int branch;
@@ -694,13 +730,17 @@ k_solute_zm(View V, int cur, int nxt)
         if (rowok && lane == 0)
             G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
         __syncthreads();
-        clm = clv;
-        clv = clp;
-        clp = clq;
-        fsm = fsv;
-        fsv = fsp;
-        fsp = fsq;
+    };
+    for (int k = k0; k <= k1; k += 2)
+    {
+        plane(k, A, B);
+        if (k + 1 <= k1)
+            plane(k + 1, B, A);
     }
+    // status and step of a negative ce (any plane of the tile: the step is
+    // the same for all)
+    if (c.check_neg && neg)
+        flag_negative_ce(cnt);
     if (F32)
     {
         const double tot = block_sum<64 * BY>(lost, s_red);

@@ -374,7 +374,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=str, default="256x256x512")
-    ap.add_argument("--zm", type=int, default=64,
+    ap.add_argument("--zm", type=int, default=128,
                     help="z-marching solute kernel, planes per workgroup (0: cell-parallel)")
     ap.add_argument("--zm-rows", type=int, default=16, help="rows per z-marching workgroup (4, 8, 16)")
     ap.add_argument("--xcd", type=int, default=1, help="1: contiguous tile runs per XCD")

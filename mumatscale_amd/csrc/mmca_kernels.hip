@@ -663,14 +663,16 @@ k_solute_zm(View V, int cur, int nxt)
     double fsm = fsc[(k0 - 1) * c.sxy + col];
     double clv = from_raw<F32>(c, raw_cl<F32>(b, cur, k0 * c.sxy + col));
     double fsv = fsc[k0 * c.sxy + col];
-    ZmAhead<F32> A, B;
-    fetch(B, k0);
+    // Loads run one plane ahead, in two register sets used in turn.  (Two
+    // planes ahead, with a third set, measured slower.)
+    ZmAhead<F32> P0, P1;
+    fetch(P0, k0);
     // the loop starts with nothing in flight, so that its waits are the
     // steady-state ones
     __builtin_amdgcn_s_waitcnt(0);
     double lost = 0.0;
     bool neg = false;                             // a negative ce seen by this lane
-    // one plane: `ld` receives plane k+1, `use` holds plane k
+    // one plane: `ld` (plane k-1's set) receives plane k+1, `use` holds plane k
     auto plane = [&](int k, ZmAhead<F32> &ld, ZmAhead<F32> &use) {
         const long long idx = k * c.sxy + col;
         if (k > k0)
@@ -680,8 +682,8 @@ k_solute_zm(View V, int cur, int nxt)
             clv = from_raw<F32>(c, ld.cl2);       // landed: it was plane k-1's +z neighbour
             fsv = ld.fs2;
         }
-        // on the last plane this reloads plane k1 (unused): every plane
-        // issues the same loads, so every wait is the same
+        // near the top this reloads plane k1 (unused): every plane issues
+        // the same loads, so every wait is the same
         fetch(ld, min(k + 1, k1));
         if constexpr (AM)
             beam_tab_plane(c, *beam, k, ty * 64 + lane, 64 * BY);   // read after the barrier below
@@ -731,12 +733,17 @@ k_solute_zm(View V, int cur, int nxt)
             G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
         __syncthreads();
     };
-    for (int k = k0; k <= k1; k += 2)
+    // Whole rounds of two planes, so that the loop is only re-entered after
+    // a whole round and its waits see the same loads on every path; then the
+    // plane left over.
+    int k = k0;
+    for (; k + 1 <= k1; k += 2)
     {
-        plane(k, A, B);
-        if (k + 1 <= k1)
-            plane(k + 1, B, A);
+        plane(k, P1, P0);
+        plane(k + 1, P0, P1);
     }
+    if (k <= k1)
+        plane(k, P1, P0);
     // status and step of a negative ce (any plane of the tile: the step is
     // the same for all)
     if (c.check_neg && neg)

@@ -41,16 +41,28 @@ def log(*a):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_solute_zm_directional_256x256x512_fp64.md")
 
 
-def measured_traffic(size, zm, rows, xcd):
+def apply_shape(sb, shape):
+    """Solute-kernel tile shape (planes, rows, XCD remap) of a subblock."""
+    sb.set_tuning(abi.TUNE_ZM_PLANES, shape["zm"])
+    sb.set_tuning(abi.TUNE_ZM_ROWS, shape["rows"])
+    sb.set_tuning(abi.TUNE_XCD_REMAP, shape["xcd"])
+
+
+def measured_traffic(size, shape):
     """HBM bytes per launch of the solute kernel from the committed
     FETCH_SIZE/WRITE_SIZE summary, when this run has the shape it was
     measured on (None otherwise)."""
     import re
-    if (size, zm, rows, xcd) != ("256x256x512", 64, 16, 1) or not os.path.exists(TRAFFIC_PROFILE):
+    if not os.path.exists(TRAFFIC_PROFILE):
         return None
     with open(TRAFFIC_PROFILE) as fh:
-        m = re.search(r"traffic_bytes_per_launch = ([0-9.eE+]+)", fh.read())
-    return float(m.group(1)) if m else None
+        text = fh.read()
+    rec = re.search(r"shape = (\S+) zm_planes (\d+) zm_rows (\d+) xcd_remap (\d+)", text)
+    m = re.search(r"traffic_bytes_per_launch = ([0-9.eE+]+)", text)
+    want = (size, str(shape["zm"]), str(shape["rows"]), str(shape["xcd"]))
+    if not rec or not m or rec.groups() != want:
+        return None
+    return float(m.group(1))
 
 
 def run_ensemble(steps, baseline_steps):
@@ -109,7 +121,7 @@ def run_ensemble(steps, baseline_steps):
             "max_solid_fraction_err_vs_fp64": fs_err}
 
 
-def run_fp32_variant(cfg, steps, warmup, zm, rows, xcd):
+def run_fp32_variant(cfg, steps, warmup, shape):
     """The headline workload with FP32-deviation storage: throughput at the
     headline dt, and its error against FP64 after the same steps at half
     that dt (at the explicit limit the partitioning update is unstable and
@@ -119,9 +131,7 @@ def run_fp32_variant(cfg, steps, warmup, zm, rows, xcd):
     n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
     res = {}
     with Subblock(cfg.params(check_negative_ce=False, precision=abi.PREC_FP32DEV), cfg.nucleation()) as sb:
-        sb.set_tuning(abi.TUNE_ZM_PLANES, zm)
-        sb.set_tuning(abi.TUNE_ZM_ROWS, rows)
-        sb.set_tuning(abi.TUNE_XCD_REMAP, xcd)
+        apply_shape(sb, shape)
         sb.step(warmup)
         sb.set_kernel_timing(True)
         t0 = time.perf_counter()
@@ -137,8 +147,7 @@ def run_fp32_variant(cfg, steps, warmup, zm, rows, xcd):
     fields = {}
     for prec in (abi.PREC_FP32DEV, abi.PREC_FP64):
         with Subblock(half.params(check_negative_ce=False, precision=prec), half.nucleation()) as sb:
-            sb.set_tuning(abi.TUNE_ZM_PLANES, zm)
-            sb.set_tuning(abi.TUNE_ZM_ROWS, rows)
+            apply_shape(sb, shape)
             rep = sb.step(steps, want_sums=True)
             fields[prec] = (sb.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1], rep)
     ce32, r32 = fields[abi.PREC_FP32DEV]
@@ -366,6 +375,40 @@ def cpu_baseline(cfg, target_s):
                       % (steps + 1, dt, nthr)}
 
 
+def capture_footprint(steps=(250, 500, 750, 1000)):
+    """The capture kernel's read floor on the directional workload: at each
+    of `steps`, the interface-list length and the distinct 128-byte lines of
+    gr, dc and fs that its solid face neighbours occupy, plus the list's own
+    streamed bytes (slot id, dc_tmp, neighbour bits).  Under rocprofv3 with
+    FETCH_SIZE this gives the re-read factor of the kernel."""
+    import numpy as np
+    from mumatscale_amd.lib import Subblock
+    cfg = directional_config()
+    sb = Subblock(cfg.params(check_negative_ce=False), cfg.nucleation(), device=0)
+    done = 0
+    for t in steps:
+        rep = sb.step(t - done)
+        done = t
+        gr = sb.download(abi.FIELD_GR).reshape(-1)
+        lst = sb.interface_list().astype(np.int64)
+        sx = cfg.sbdim[0] + 2
+        sxy = sx * (cfg.sbdim[1] + 2)
+        refs = []
+        for off in (1, sx, -1, -sx, sxy, -sxy):
+            n = lst + off
+            refs.append(n[gr[n] > 0])
+        nref = sum(r.size for r in refs)
+        sol = np.unique(np.concatenate(refs))
+        lines = (np.unique(sol * 4 // 128).size
+                 + np.unique(np.concatenate([sol * 24 // 128, (sol * 24 + 23) // 128])).size
+                 + np.unique(sol * 8 // 128).size)
+        floor = lines * 128 + lst.size * (4 + 24 + 1)
+        print(json.dumps({"step": rep.timestep, "members": int(lst.size), "solid_neighbour_refs": int(nref),
+                          "distinct_solid_neighbours": int(sol.size), "read_floor_bytes": int(floor)}),
+              flush=True)
+    sb.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,7 +427,12 @@ def main():
     ap.add_argument("--no-am", action="store_true",
                     help="skip the AM single track (FP32 against FP64, 1024x512x256)")
     ap.add_argument("--am-steps", type=int, default=60)
+    ap.add_argument("--capture-footprint", action="store_true",
+                    help="print the capture kernel's read floor at steps 250-1000 and exit")
     args = ap.parse_args()
+    if args.capture_footprint:
+        capture_footprint()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
               % args.gpus, file=sys.stderr)
@@ -395,9 +443,8 @@ def main():
     from mumatscale_amd.lib import Subblock
     t0 = time.time()
     gpu = Subblock(cfg.params(check_negative_ce=False), cfg.nucleation(), device=0)
-    gpu.set_tuning(abi.TUNE_ZM_PLANES, args.zm)
-    gpu.set_tuning(abi.TUNE_ZM_ROWS, args.zm_rows)
-    gpu.set_tuning(abi.TUNE_XCD_REMAP, args.xcd)
+    shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd}
+    apply_shape(gpu, shape)
     log("gpu set-up %.1f s" % (time.time() - t0))
     n = nx * ny * nz
     if args.warmup:
@@ -416,13 +463,13 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": measured_traffic(args.size, args.zm, args.zm_rows, args.xcd),
+            "traffic": measured_traffic(args.size, shape),
             "kernel": ("k_solute_zm<%d>" % args.zm_rows) if args.zm > 0 else "k_solute<true,true>",
             "bytes_per_cell": B_ALG_FP64}
     gpu.close()
     extra = {}
     if not args.no_extra:
-        extra["fp32dev"] = run_fp32_variant(cfg, args.extra_steps, args.warmup, args.zm, args.zm_rows, args.xcd)
+        extra["fp32dev"] = run_fp32_variant(cfg, args.extra_steps, args.warmup, shape)
         log("fp32dev:", extra["fp32dev"])
         extra["batched_ensemble"] = run_ensemble(3000, 200)
         log("ensemble:", extra["batched_ensemble"])

@@ -41,3 +41,27 @@ def test_fp32dev_against_fp64(case, steps):
         assert rb.num_grains == ra.num_grains
         assert fs_err <= 1e-3
         assert l2 <= 1e-3
+
+
+def test_fp32dev_fields_do_not_depend_on_tile_shape():
+    """The FP32-deviation fields are the same bits for every solute-kernel
+    shape (only the ledger's summation order follows the workgroups)."""
+    cfg, _, _, _ = load_case("c1_directional")
+    p32 = cfg.params(precision=abi.PREC_FP32DEV)
+    out = []
+    for planes, rows, xcd in ((16, 16, 1), (7, 4, 0), (32, 8, 1), (0, 4, 0)):
+        print("fp32 solute kernel shape", planes, rows, xcd, flush=True)
+        with Subblock(p32, cfg.nucleation()) as sb:
+            sb.set_tuning(abi.TUNE_ZM_PLANES, planes)
+            sb.set_tuning(abi.TUNE_ZM_ROWS, rows)
+            sb.set_tuning(abi.TUNE_XCD_REMAP, xcd)
+            rep = sb.step(300, want_sums=True)
+            out.append(([sb.download(f) for f in (abi.FIELD_CE, abi.FIELD_CL, abi.FIELD_FS, abi.FIELD_GR)], rep))
+    f0, r0 = out[0]
+    for fields, rep in out[1:]:
+        for a, b in zip(f0, fields):
+            assert np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+        assert rep.num_grains == r0.num_grains
+        assert rep.solute_sum == r0.solute_sum
+        total, total0 = rep.solute_sum + rep.solute_ledger, r0.solute_sum + r0.solute_ledger
+        assert abs(total - total0) <= 1e-12 * abs(total0)

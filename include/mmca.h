@@ -293,6 +293,23 @@ int  mmca_batch_domain(mmca_batch *batch, int i, mmca_ctx **ctx);
 /* n steps of every domain; reports[ndomains] may be NULL.  Returns the first
  * non-OK domain status (the others have stepped). */
 int  mmca_batch_step(mmca_batch *batch, int nsteps, int want_sums, mmca_report *reports);
+/*
+ * Couples the domains of a fresh batch into one global domain split into
+ * nsub[0] x nsub[1] x nsub[2] subblocks, as the reference runs one subblock
+ * per MPI rank (ca_main.c:186-212, calculate.c:193-352).  Domain d is rank
+ * d of the MPI Cart grid (x fastest); its params must give sb_coords =
+ * (d % nsub[0], d / nsub[0] % nsub[1], d / (nsub[0] nsub[1])), the same
+ * NCellsPerSB, gdim = nsub * NCellsPerSB, and its nucleation set-up must
+ * name rank d (each rank samples its subblock from its own stream).  Each
+ * step then exchanges the face halos of cl, fs and dc before diffusion and
+ * of gr and fs after partitioning (ExchangeFacesForVar), and activates new
+ * grains across the ranks in rank order into one shared grain table
+ * (activateNewGrains, grain.c:841-916), while every rank keeps its own
+ * interface list and dc_tmp slots.  Padded faces only (FaceCtrl 1 on every
+ * axis split in more than one subblock).  Step the batch only through
+ * mmca_batch_step afterwards.
+ */
+int  mmca_batch_couple(mmca_batch *batch, const int32_t nsub[3]);
 int  mmca_batch_set_tuning(mmca_batch *batch, int key, int value);
 int  mmca_batch_set_kernel_timing(mmca_batch *batch, int enable);
 int  mmca_batch_kernel_time_ms(mmca_batch *batch, double *avg_ms, int64_t *launches);

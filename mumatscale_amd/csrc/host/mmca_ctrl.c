@@ -345,6 +345,20 @@ mmca_ctrl_params(const mmca_run_cfg *c, int check_negative_ce, int precision, mm
     p->timestep = 0;
 }
 
+/* The subblock of MPI rank `rank` (ca_main.c:197-212: Cart grid, x fastest). */
+void
+mmca_ctrl_params_rank(const mmca_run_cfg *c, int check_negative_ce, int precision, int rank, mmca_params *p)
+{
+    mmca_ctrl_params(c, check_negative_ce, precision, p);
+    p->nx = c->sbdim[0];
+    p->ny = c->sbdim[1];
+    p->nz = c->sbdim[2];
+    p->sb_coords[0] = rank % c->nsub[0];
+    p->sb_coords[1] = (rank / c->nsub[0]) % c->nsub[1];
+    p->sb_coords[2] = rank / (c->nsub[0] * c->nsub[1]);
+    p->gnsbx = c->nsub[0];              /* the beam track spans gnsbx * nx (temperature.c:68) */
+}
+
 void
 mmca_ctrl_nucleation(const mmca_run_cfg *c, mmca_nucleation_cfg *n)
 {

@@ -42,6 +42,9 @@ void mmca_ctrl_free(mmca_run_cfg *cfg);
 
 /* Parameters of the whole domain (NSubblocks x NCellsPerSB) as one block. */
 void mmca_ctrl_params(const mmca_run_cfg *cfg, int check_negative_ce, int precision, mmca_params *p);
+/* the subblock of one MPI rank (NCellsPerSB at its Cart coordinates) */
+void mmca_ctrl_params_rank(const mmca_run_cfg *cfg, int check_negative_ce, int precision, int rank,
+                           mmca_params *p);
 /* Nucleation set-up of that block (random stream of rank 0); the lists
  * point into cfg. */
 void mmca_ctrl_nucleation(const mmca_run_cfg *cfg, mmca_nucleation_cfg *n);

@@ -10,14 +10,15 @@
  *   output_grains grain.c:752-833 <BaseFileName>_grains.csv and the
  *                                 "N grains. Min ... Average" line
  *
- * The whole domain (NSubblocks x NCellsPerSB) runs as one block with the
- * random stream of rank 0, so runs with NSubblocks 1 1 1 reproduce the
- * reference; with more subblocks the random sites and orientations differ
- * from a multi-rank reference run.  Steps between two screen or finish
- * checks are issued as one mmca_step call.
+ * With NSubblocks x y z the domain runs as the reference runs it on x*y*z
+ * MPI ranks: one subblock per rank, each with its own random stream, halos
+ * exchanged between them and grain ids given in rank order (a coupled
+ * batch, mmca_batch_couple), all on one GPU.  --one-block runs the whole
+ * domain as a single block with rank 0's stream instead.  Steps between two
+ * screen or finish checks are issued as one step call.
  *
  * Usage: mmca_run <ctrl-file> [--device N] [--max-steps N] [--fp32]
- *                 [--no-ce-check] [--dump-ctrl]
+ *                 [--no-ce-check] [--one-block] [--dump-ctrl]
  */
 #include <math.h>
 #include <stdio.h>
@@ -92,34 +93,49 @@ dump_ctrl(const mmca_run_cfg *c, const mmca_params *p, const mmca_nucleation_cfg
 }
 
 /* output_grains (grain.c:752-833) on the downloaded gr field and grain table */
+/* output_grains (grain.c:752-833) over the subblocks doms[0..ndom) of one
+ * domain (params ps); the grain table is the one of doms[0], shared by a
+ * coupled batch */
 static int
-output_grains(mmca_ctx *ctx, const mmca_run_cfg *c, const mmca_params *p)
+output_grains(mmca_ctx *const *doms, const mmca_params *ps, int ndom, const mmca_run_cfg *c)
 {
-    int64_t P = 0;
     int32_t ng = 0;
-    if (check(mmca_padded_cells(ctx, &P), "mmca_padded_cells") ||
-        check(mmca_get_grains(ctx, NULL, 0, &ng), "mmca_get_grains"))
+    if (check(mmca_get_grains(doms[0], NULL, 0, &ng), "mmca_get_grains"))
         return 1;
-    int32_t *gr = malloc(sizeof(int32_t) * (size_t) P);
     mmca_grain *g = calloc((size_t) (ng > 0 ? ng : 1), sizeof(mmca_grain));
     size_t *sizes = calloc((size_t) (ng > 0 ? ng : 1), sizeof(size_t));
-    if (!gr || !g || !sizes)
+    if (!g || !sizes)
     {
         fprintf(stderr, "out of host memory\n");
         return 1;
     }
-    if (check(mmca_download_field(ctx, MMCA_FIELD_GR, gr), "mmca_download_field") ||
-        check(mmca_get_grains(ctx, g, ng, NULL), "mmca_get_grains"))
+    if (check(mmca_get_grains(doms[0], g, ng, NULL), "mmca_get_grains"))
         return 1;
-    const int sx = p->nx + 2, sxy = (p->nx + 2) * (p->ny + 2);
-    for (int z = 1; z <= p->nz; z++)          /* calculateGrainSizes (grain.c:715-745) */
-        for (int y = 1; y <= p->ny; y++)
-            for (int x = 1; x <= p->nx; x++)
-            {
-                int id = gr[(size_t) z * sxy + (size_t) y * sx + x];
-                if (id > 0 && id < ng)
-                    sizes[id]++;
-            }
+    for (int d = 0; d < ndom; d++)        /* calculateGrainSizes (grain.c:715-745), summed over ranks */
+    {
+        int64_t P = 0;
+        const mmca_params q = ps[d];
+        if (check(mmca_padded_cells(doms[d], &P), "mmca_padded_cells"))
+            return 1;
+        int32_t *gr = malloc(sizeof(int32_t) * (size_t) P);
+        if (!gr)
+        {
+            fprintf(stderr, "out of host memory\n");
+            return 1;
+        }
+        if (check(mmca_download_field(doms[d], MMCA_FIELD_GR, gr), "mmca_download_field"))
+            return 1;
+        const int sx = q.nx + 2, sxy = (q.nx + 2) * (q.ny + 2);
+        for (int z = 1; z <= q.nz; z++)
+            for (int y = 1; y <= q.ny; y++)
+                for (int x = 1; x <= q.nx; x++)
+                {
+                    int id = gr[(size_t) z * sxy + (size_t) y * sx + x];
+                    if (id > 0 && id < ng)
+                        sizes[id]++;
+                }
+        free(gr);
+    }
     char fname[512];
     snprintf(fname, sizeof(fname), "%s_grains.csv", c->basefilename);
     FILE *fp = fopen(fname, "w");
@@ -158,7 +174,6 @@ output_grains(mmca_ctx *ctx, const mmca_run_cfg *c, const mmca_params *p)
         fclose(fp);
     printf("%d grains.  Min: %lg mm^3   Max: %lg mm^3   Average:  %lg mm^3\n", ng - 1, min_size * cell_volume,
            max_size * cell_volume, ((double) sum / ((double) ((ng - 1) - (int) zero)) * cell_volume));
-    free(gr);
     free(g);
     free(sizes);
     return 0;
@@ -168,7 +183,7 @@ int
 main(int argc, char **argv)
 {
     const char *path = NULL;
-    int device = 0, fp32 = 0, ce_check = 1, dump = 0;
+    int device = 0, fp32 = 0, ce_check = 1, dump = 0, one_block = 0;
     long long max_steps = -1;
     for (int i = 1; i < argc; i++)
     {
@@ -180,6 +195,8 @@ main(int argc, char **argv)
             fp32 = 1;
         else if (!strcmp(argv[i], "--no-ce-check"))
             ce_check = 0;
+        else if (!strcmp(argv[i], "--one-block"))
+            one_block = 1;
         else if (!strcmp(argv[i], "--dump-ctrl"))
             dump = 1;
         else if (argv[i][0] != '-' && !path)
@@ -187,7 +204,7 @@ main(int argc, char **argv)
         else
         {
             fprintf(stderr, "usage: %s <ctrl-file> [--device N] [--max-steps N] [--fp32] [--no-ce-check] "
-                            "[--dump-ctrl]\n", argv[0]);
+                            "[--one-block] [--dump-ctrl]\n", argv[0]);
             return 2;
         }
     }
@@ -200,9 +217,10 @@ main(int argc, char **argv)
     mmca_ctrl_defaults(&cfg);
     if (mmca_ctrl_read(path, &cfg))
         return 1;
+    const int prec = fp32 ? MMCA_PREC_FP32DEV : MMCA_PREC_FP64;
     mmca_params p;
     mmca_nucleation_cfg nuc;
-    mmca_ctrl_params(&cfg, ce_check, fp32 ? MMCA_PREC_FP32DEV : MMCA_PREC_FP64, &p);
+    mmca_ctrl_params(&cfg, ce_check, prec, &p);
     mmca_ctrl_nucleation(&cfg, &nuc);
     if (dump)
     {
@@ -210,10 +228,39 @@ main(int argc, char **argv)
         mmca_ctrl_free(&cfg);
         return 0;
     }
+    /* one subblock per MPI rank of the reference, or the whole domain */
+    const int nr = one_block ? 1 : cfg.nsub[0] * cfg.nsub[1] * cfg.nsub[2];
+    mmca_params *ps = calloc((size_t) nr, sizeof(mmca_params));
+    mmca_nucleation_cfg *ns = calloc((size_t) nr, sizeof(mmca_nucleation_cfg));
+    mmca_ctx **doms = calloc((size_t) nr, sizeof(mmca_ctx *));
+    mmca_report *reps = calloc((size_t) nr, sizeof(mmca_report));
+    if (!ps || !ns || !doms || !reps)
+    {
+        fprintf(stderr, "out of host memory\n");
+        return 1;
+    }
+    for (int r = 0; r < nr; r++)
+    {
+        if (nr > 1)
+            mmca_ctrl_params_rank(&cfg, ce_check, prec, r, &ps[r]);
+        else
+            ps[r] = p;
+        ns[r] = nuc;
+        ns[r].rank = r;
+    }
     const double t_start = wall_now();
-    mmca_ctx *ctx = NULL;
-    if (check(mmca_create(&p, device, &ctx), "mmca_create") ||
-        check(mmca_init_nucleation(ctx, &nuc), "mmca_init_nucleation"))
+    mmca_batch *bt = NULL;
+    if (nr > 1)
+    {
+        if (check(mmca_batch_create(ps, ns, nr, device, &bt), "mmca_batch_create") ||
+            check(mmca_batch_couple(bt, cfg.nsub), "mmca_batch_couple"))
+            return 1;
+        for (int r = 0; r < nr; r++)
+            if (check(mmca_batch_domain(bt, r, &doms[r]), "mmca_batch_domain"))
+                return 1;
+    }
+    else if (check(mmca_create(&ps[0], device, &doms[0]), "mmca_create") ||
+             check(mmca_init_nucleation(doms[0], &ns[0]), "mmca_init_nucleation"))
         return 1;
     printf("Initial Subblocks assigned.  Starting calculations.\n");
     const double total_volume = (p.gdim[0] * p.gdim[1] * p.gdim[2]) * pow(cfg.cell_size, 3);
@@ -222,8 +269,6 @@ main(int argc, char **argv)
         printf("%10g  %10g  %8.4lf%% %8d\n", wall_now() - t_start, 0.0, 0.0, 0);
     uint64_t t = 0;
     double fs = 0.0;
-    mmca_report rep;
-    memset(&rep, 0, sizeof(rep));
     const uint64_t fin = cfg.finish_time ? finish_step(1, cfg.dt, cfg.finish_time) : 0;
     int status = MMCA_OK;
     for (;;)
@@ -238,18 +283,28 @@ main(int argc, char **argv)
         if (next == UINT64_MAX)
             next = t + 1000;
         const int screen = cfg.screenpfreq > 0 && next % cfg.screenpfreq == 0;
-        status = mmca_step(ctx, (int) (next - t), screen, &rep);
+        if (bt)
+            status = mmca_batch_step(bt, (int) (next - t), screen, reps);
+        else
+            status = mmca_step(doms[0], (int) (next - t), screen, &reps[0]);
         if (status != MMCA_OK)
         {
-            fprintf(stderr, "step %llu: %s\n", (unsigned long long) rep.failed_step, mmca_status_string(status));
+            uint64_t failed = 0;
+            for (int r = 0; r < nr; r++)
+                if (reps[r].status != MMCA_OK && (failed == 0 || reps[r].failed_step < failed))
+                    failed = reps[r].failed_step;
+            fprintf(stderr, "step %llu: %s\n", (unsigned long long) failed, mmca_status_string(status));
             break;
         }
         t = next;
         if (screen)
         {
-            fs = rep.solid_sum * cell_vol / total_volume;     /* computeFS */
+            double solid = 0.0;                       /* computeFS: sum over the ranks */
+            for (int r = 0; r < nr; r++)
+                solid += reps[r].solid_sum;
+            fs = solid * cell_vol / total_volume;
             printf("%10g  %10g  %8.4lf%% %8d\n", wall_now() - t_start, t * cfg.dt, 100.0 * fs,
-                   rep.num_grains - 1);
+                   reps[0].num_grains - 1);
             fflush(stdout);
         }
         int done = 0;
@@ -268,8 +323,15 @@ main(int argc, char **argv)
         if (done)
             break;
     }
-    int rc = status == MMCA_OK ? output_grains(ctx, &cfg, &p) : 1;
-    mmca_destroy(ctx);
+    int rc = status == MMCA_OK ? output_grains(doms, ps, nr, &cfg) : 1;
+    if (bt)
+        mmca_batch_destroy(bt);
+    else
+        mmca_destroy(doms[0]);
+    free(ps);
+    free(ns);
+    free(doms);
+    free(reps);
     mmca_ctrl_free(&cfg);
     if (rc == 0)
         printf("\nSimulation complete.  Shutting down.\n");

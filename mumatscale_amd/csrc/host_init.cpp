@@ -144,8 +144,12 @@ int build_consts(const mmca_params &p, Consts &c)
     if (p.precision != MMCA_PREC_FP64 && p.precision != MMCA_PREC_FP32DEV)
         return MMCA_ERR_ARG;
     c.f32 = (p.precision == MMCA_PREC_FP32DEV) ? 1 : 0;
+    const int n3[3] = { p.nx, p.ny, p.nz };
     for (int a = 0; a < 3; a++) {
-        c.pad[a] = p.pad_bndy[a] ? 1 : 0;
+        // a face is a mold wall where the global domain ends (one subblock:
+        // both faces of a padded axis)
+        c.pad_lo[a] = (p.pad_bndy[a] && p.sb_coords[a] == 0) ? 1 : 0;
+        c.pad_hi[a] = (p.pad_bndy[a] && (long long) (p.sb_coords[a] + 1) * n3[a] >= p.gdim[a]) ? 1 : 0;
     }
     c.off[0] = p.sb_coords[0] * p.nx;
     c.off[1] = p.sb_coords[1] * p.ny;
@@ -260,7 +264,10 @@ int plan_nucleation(const mmca_params &params, const Consts &c, const mmca_nucle
     if ((cfg->n_fixed > 0 && !cfg->fixed) || (cfg->n_volumes > 0 && !cfg->volumes))
         return MMCA_ERR_ARG;
     const mmca_params &p = params;
-    rng.seed((unsigned int) (cfg->rank + 1 + cfg->random_seed));
+    // init_nucleation_sites runs on rank 0, after srand(0 + 1 + seed)
+    // (ca_main.c:215, 244-245); each rank then samples its own subblock
+    // (init_sb_nucleation) from its own stream, srand(rank + 1 + seed)
+    rng.seed((unsigned int) (0 + 1 + cfg->random_seed));
     const int oriented = cfg->oriented;
     auto real2cell = [&](double x, double y, double z, size_t *i, size_t *j, size_t *k) {
         // realcoord2scoord (functions.c:92-106)
@@ -354,6 +361,8 @@ int plan_nucleation(const mmca_params &params, const Consts &c, const mmca_nucle
         if (st != MMCA_OK)
             return st;
     }
+    if (cfg->rank != 0)
+        rng.seed((unsigned int) (cfg->rank + 1 + cfg->random_seed));
     // points of this subblock (grain.c:695-711)
     out.nuc_pt.clear();
     for (const NucPt &n : g) {
@@ -368,8 +377,8 @@ int plan_nucleation(const mmca_params &params, const Consts &c, const mmca_nucle
     const long long P = c.P;
     std::vector<float> thr((size_t) P, INFINITY);
     auto is_mold = [&](long x, long y, long z) {
-        return (c.pad[0] && (x == 0 || x == c.nx + 1)) || (c.pad[1] && (y == 0 || y == c.ny + 1)) ||
-               (c.pad[2] && (z == 0 || z == c.nz + 1));
+        return (c.pad_lo[0] && x == 0) || (c.pad_hi[0] && x == c.nx + 1) || (c.pad_lo[1] && y == 0) ||
+               (c.pad_hi[1] && y == c.ny + 1) || (c.pad_lo[2] && z == 0) || (c.pad_hi[2] && z == c.nz + 1);
     };
     for (const NucPt &n : out.nuc_pt) {
         size_t x = 1 + n.cx % p.nx, y = 1 + n.cy % p.ny, z = 1 + n.cz % p.nz;

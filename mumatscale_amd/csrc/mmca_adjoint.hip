@@ -80,12 +80,12 @@ stencil(const Consts &c, const Cell &q, long long nb[6], int mo[6])
     nb[3] = q.idx - c.sx;
     nb[4] = q.idx + c.sxy;
     nb[5] = q.idx - c.sxy;
-    mo[0] = (q.x == c.nx) ? c.pad[0] : 0;
-    mo[1] = (q.x == 1) ? c.pad[0] : 0;
-    mo[2] = (q.j == c.ny) ? c.pad[1] : 0;
-    mo[3] = (q.j == 1) ? c.pad[1] : 0;
-    mo[4] = (q.k == c.nz) ? c.pad[2] : 0;
-    mo[5] = (q.k == 1) ? c.pad[2] : 0;
+    mo[0] = (q.x == c.nx) ? c.pad_hi[0] : 0;
+    mo[1] = (q.x == 1) ? c.pad_lo[0] : 0;
+    mo[2] = (q.j == c.ny) ? c.pad_hi[1] : 0;
+    mo[3] = (q.j == 1) ? c.pad_lo[1] : 0;
+    mo[4] = (q.k == c.nz) ? c.pad_hi[2] : 0;
+    mo[5] = (q.k == 1) ? c.pad_lo[2] : 0;
 }
 
 template <bool AM>

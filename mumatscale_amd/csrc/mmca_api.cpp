@@ -22,6 +22,8 @@ hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, in
                          hipStream_t s);
 hipError_t launch_mask_from_gr(const DevBufs &b, const Consts &c, hipStream_t s);
 hipError_t launch_list(const View &v, int cb, int capture, int decide_realloc, hipStream_t s);
+hipError_t launch_nucleate_coupled(const View &v, int cur, hipStream_t s);
+hipError_t launch_halo_exchange(const View &v, int phase, int cb, long long max_face, hipStream_t s);
 hipError_t launch_io_fields(const DevBufs &b, const Consts &c, int cur, float *fs_io, float *ce_io, float *t_io,
                             int *gr_io, hipStream_t s);
 hipError_t launch_capture_only(const View &v, const DevBufs &b, int cb, hipStream_t s);
@@ -99,6 +101,12 @@ struct mmca_ctx {
     long long q0 = 0;
     GlibcRand rng_now;
     bool rand_dirty = false;
+    // a domain of a coupled batch other than rank 0: its grain table is rank
+    // 0's (its own was released)
+    bool shares_grains = false;
+    // the domains of a coupled batch in rank order (empty otherwise): grain
+    // records are read through the domain that created the grain
+    std::vector<mmca_ctx *> peers;
 };
 
 namespace {
@@ -403,6 +411,8 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     TRY(dev_alloc(&b.dc, 3 * P));
     const size_t mask_words = (size_t) (c.ny + 2) * (c.nz + 2) * c.w64;
     TRY(dev_alloc(&b.mask, mask_words));
+    const size_t xhalo_bytes = (size_t) 2 * (c.ny + 2) * (c.nz + 2);
+    TRY(dev_alloc(&b.xhalo, xhalo_bytes));
     TRY(dev_alloc(&b.diff_id, N + 2));
     TRY(dev_alloc(&b.nbr, N + 2));
     // dc_tmp slots 0 .. N + 101 (dc_tmp_size <= N + 100, slot dc_tmp_size used)
@@ -419,6 +429,7 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     TRY(dev_alloc(&b.g_cell, G));
     TRY(dev_alloc(&b.g_step, G));
     TRY(dev_alloc(&b.g_src, G));
+    TRY(dev_alloc(&b.g_dom, G));
     ctx->rand_capacity = (int) ctx->p.max_total_grains;
     TRY(dev_alloc(&b.rand_rot, (size_t) ctx->rand_capacity * 9));
     TRY(dev_alloc(&b.cnt, 1));
@@ -439,6 +450,7 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     HIPCHK(hipMemsetAsync(b.gr, 0, P * sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(b.dc, 0, 3 * P * sizeof(double), ctx->stream));
     HIPCHK(hipMemsetAsync(b.mask, 0, mask_words * sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipMemsetAsync(b.xhalo, 0, xhalo_bytes, ctx->stream));
     HIPCHK(hipMemsetAsync(b.diff_id, 0, (N + 2) * sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(b.dtx, 0, (N + 102) * sizeof(double), ctx->stream));
     HIPCHK(hipMemsetAsync(b.dty, 0, (N + 102) * sizeof(double), ctx->stream));
@@ -447,6 +459,7 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     HIPCHK(hipMemsetAsync(b.g_cell, 0, G * sizeof(int), ctx->stream));
     HIPCHK(hipMemsetAsync(b.g_step, 0, G * sizeof(unsigned long long), ctx->stream));
     HIPCHK(hipMemsetAsync(b.g_src, 0, G * sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(b.g_dom, 0, G * sizeof(int), ctx->stream));
     memset(&ctx->hc, 0, sizeof(ctx->hc));
     ctx->hc.timestep = ctx->p.timestep;
     ctx->hc.num_grains = 1;      // read_ctrl.c:52: the "zero" grain
@@ -463,9 +476,13 @@ int mmca_destroy(mmca_ctx *ctx)
     if (ctx->stream)
         (void) hipStreamSynchronize(ctx->stream);
     DevBufs &b = ctx->b;
+    if (ctx->shares_grains)
+        b.g_rot = nullptr, b.g_cell = nullptr, b.g_step = nullptr, b.g_src = nullptr, b.g_dom = nullptr;
     void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.ce32, b.cl32[0], b.cl32[1], b.ledger, b.fs[0], b.fs[1], b.gr, b.dc, b.mask,
+                     b.xhalo,
                      b.diff_id, b.nbr, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
-                     b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.fixed_rot, b.rand_rot,
+                     b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.g_dom, b.fixed_rot,
+                     b.rand_rot,
                      b.oce, b.d, b.red_part, b.cnt, ctx->d_red, ctx->d_desc, ctx->d_off, ctx->io_buf };
     for (void *p : ptrs)
         if (p)
@@ -886,9 +903,9 @@ int mmca_download_field(mmca_ctx *ctx, int field, void *host)
         for (int z = 0; z < c.nz + 2; z++)
             for (int y = 0; y < c.ny + 2; y++)
                 for (int x = 0; x < c.sx; x++) {
-                    bool m = (c.pad[0] && (x == 0 || x == c.nx + 1)) ||
-                             (c.pad[1] && (y == 0 || y == c.ny + 1)) ||
-                             (c.pad[2] && (z == 0 || z == c.nz + 1));
+                    bool m = (c.pad_lo[0] && x == 0) || (c.pad_hi[0] && x == c.nx + 1) ||
+                             (c.pad_lo[1] && y == 0) || (c.pad_hi[1] && y == c.ny + 1) ||
+                             (c.pad_lo[2] && z == 0) || (c.pad_hi[2] && z == c.nz + 1);
                     h[x + (size_t) y * c.sx + (size_t) z * c.sxy] = m ? 1 : 0;
                 }
         break;
@@ -919,26 +936,30 @@ int mmca_get_grains(mmca_ctx *ctx, mmca_grain *out, int32_t max, int32_t *num_gr
     if (n <= 0)
         return MMCA_OK;
     std::vector<double> rot((size_t) n * 9);
-    std::vector<int> cell(n), src(n);
+    std::vector<int> cell(n), src(n), dom(n, 0);
     std::vector<unsigned long long> step(n);
     HIPCHK(hipMemcpy(rot.data(), ctx->b.g_rot, rot.size() * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(cell.data(), ctx->b.g_cell, n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(src.data(), ctx->b.g_src, n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(step.data(), ctx->b.g_step, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    const Consts &c = ctx->c;
-    const mmca_params &p = ctx->p;
+    if (!ctx->peers.empty())
+        HIPCHK(hipMemcpy(dom.data(), ctx->b.g_dom, n * sizeof(int), hipMemcpyDeviceToHost));
     memset(out, 0, sizeof(mmca_grain) * (size_t) n);
     for (int id = 1; id < n; id++) {
         mmca_grain &g = out[id];
-        if (src[id] == GRAIN_SET && id < (int) ctx->ext_grains.size()) {
-            g = ctx->ext_grains[id];   // record given to mmca_set_grains
+        // the subblock (rank) that created the grain: its set-up and stream
+        const mmca_ctx *owner = ctx->peers.empty() ? ctx : ctx->peers[(size_t) dom[id]];
+        const Consts &c = owner->c;
+        const mmca_params &p = owner->p;
+        if (src[id] == GRAIN_SET && id < (int) owner->ext_grains.size()) {
+            g = owner->ext_grains[id];   // record given to mmca_set_grains
             continue;
         }
         for (int q = 0; q < 9; q++)
             g.rotmat[q / 3][q % 3] = rot[(size_t) id * 9 + q];
         g.nuc_timestep = step[id];
         if (src[id] >= 0) {      // nucleate_grain with a FixedNuc/NucVolume entry
-            const NucPt &np = ctx->nuc_pt[src[id]];
+            const NucPt &np = owner->nuc_pt[src[id]];
             g.nuc_x = np.x;
             g.nuc_y = np.y;
             g.nuc_z = np.z;
@@ -957,14 +978,14 @@ int mmca_get_grains(mmca_ctx *ctx, mmca_grain *out, int32_t max, int32_t *num_gr
             g.nuc_x = p.origin[0] + (gx + 0.5) * p.cell_size;
             g.nuc_y = p.origin[1] + (gy + 0.5) * p.cell_size;
             g.nuc_z = p.origin[2] + (gz + 0.5) * p.cell_size;
-            auto it = std::lower_bound(ctx->site_idx.begin(), ctx->site_idx.end(), idx);
-            g.nuc_threshold = (it != ctx->site_idx.end() && *it == idx)
-                                  ? (double) ctx->site_thr[it - ctx->site_idx.begin()]
+            auto it = std::lower_bound(owner->site_idx.begin(), owner->site_idx.end(), idx);
+            g.nuc_threshold = (it != owner->site_idx.end() && *it == idx)
+                                  ? (double) owner->site_thr[it - owner->site_idx.begin()]
                                   : INFINITY;
-            if (k < (int) (ctx->rand_angles.size() / 3)) {
-                g.rotang[0] = ctx->rand_angles[(size_t) k * 3 + 0];
-                g.rotang[1] = ctx->rand_angles[(size_t) k * 3 + 1];
-                g.rotang[2] = ctx->rand_angles[(size_t) k * 3 + 2];
+            if (k < (int) (owner->rand_angles.size() / 3)) {
+                g.rotang[0] = owner->rand_angles[(size_t) k * 3 + 0];
+                g.rotang[1] = owner->rand_angles[(size_t) k * 3 + 1];
+                g.rotang[2] = owner->rand_angles[(size_t) k * 3 + 2];
             }
         }
     }
@@ -1356,6 +1377,9 @@ struct mmca_batch {
     long long time_n = 0;
     bool use_graph = false;
     hipGraphExec_t graph[2] = { nullptr, nullptr };
+    // the domains are the subblocks of one global domain (mmca_batch_couple)
+    bool coupled = false;
+    long long max_face = 0;      // largest face (cells) or halo-row word count
 };
 
 namespace {
@@ -1390,11 +1414,16 @@ int batch_sync(mmca_batch *bt)
 }
 
 // one doiteration() of every domain: seven launches whatever the batch size
+// (ten for a coupled batch: two halo exchanges and the cross-rank activation)
 int batch_enqueue(mmca_batch *bt, int cur)
 {
     const Consts &c0 = bt->dom[0]->c;
     const int nxt = cur ^ 1;
-    HIPCHK(launch_nucleate(bt->view, 7, cur, bt->stream));
+    if (bt->coupled) {
+        HIPCHK(launch_halo_exchange(bt->view, 0, cur, bt->max_face, bt->stream));
+        HIPCHK(launch_nucleate_coupled(bt->view, cur, bt->stream));
+    } else
+        HIPCHK(launch_nucleate(bt->view, 7, cur, bt->stream));
     const size_t e0 = bt->ev_used;
     if (bt->timing) {
         while (bt->ev.size() < e0 + 2) {
@@ -1410,6 +1439,8 @@ int batch_enqueue(mmca_batch *bt, int cur)
         HIPCHK(hipEventRecord(bt->ev[e0 + 1], bt->stream));
         bt->ev_used = e0 + 2;
     }
+    if (bt->coupled)
+        HIPCHK(launch_halo_exchange(bt->view, 1, nxt, bt->max_face, bt->stream));
     HIPCHK(launch_list(bt->view, nxt, 1, 1, bt->stream));
     return MMCA_OK;
 }
@@ -1465,6 +1496,64 @@ int mmca_batch_create(const mmca_params *params, const mmca_nucleation_cfg *nucl
     HIPCHK(hipStreamCreate(&bt->stream));
     for (mmca_ctx *c : bt->dom)
         HIPCHK(hipStreamSynchronize(c->stream));
+    return MMCA_OK;
+}
+
+int mmca_batch_couple(mmca_batch *bt, const int32_t nsub[3])
+{
+    if (!bt || !nsub || nsub[0] < 1 || nsub[1] < 1 || nsub[2] < 1)
+        return MMCA_ERR_ARG;
+    const int n = (int) bt->dom.size();
+    if ((long long) nsub[0] * nsub[1] * nsub[2] != n || bt->coupled)
+        return MMCA_ERR_ARG;
+    HIPCHK(hipSetDevice(bt->device));
+    TRY(batch_sync(bt));
+    const mmca_params &p0 = bt->dom[0]->p;
+    const int n3[3] = { p0.nx, p0.ny, p0.nz };
+    for (int d = 0; d < n; d++) {
+        const mmca_ctx *c = bt->dom[d];
+        const mmca_params &p = c->p;
+        // rank d of an MPI Cart grid ordered (z, y, x) (ca_main.c:197-212)
+        const int rc[3] = { d % nsub[0], (d / nsub[0]) % nsub[1], d / (nsub[0] * nsub[1]) };
+        for (int a = 0; a < 3; a++) {
+            const int na = a == 0 ? p.nx : a == 1 ? p.ny : p.nz;
+            if (na != n3[a] || p.sb_coords[a] != rc[a] || p.gdim[a] != nsub[a] * n3[a])
+                return MMCA_ERR_ARG;
+            if (!p.pad_bndy[a] && nsub[a] > 1)
+                return MMCA_ERR_ARG;     // periodic faces across subblocks are not emulated
+        }
+        if (p.precision != p0.precision || p.max_total_grains != p0.max_total_grains ||
+            c->hc.num_grains != 1 || c->hc.timestep != bt->dom[0]->hc.timestep || !c->nucleation_ready)
+            return MMCA_ERR_ARG;     // fresh, initialised subblocks of one run
+    }
+    long long mf = 0;
+    for (int d = 0; d < n; d++) {
+        mmca_ctx *c = bt->dom[d];
+        for (int a = 0; a < 3; a++)
+            c->c.nsub[a] = nsub[a];
+        mf = std::max({ mf, (long long) c->c.ny * c->c.nz, (long long) c->c.nx * c->c.nz,
+                        (long long) c->c.nx * c->c.ny, (long long) c->c.nz * c->c.w64,
+                        (long long) c->c.ny * c->c.w64 });
+        c->peers = bt->dom;
+        if (d > 0) {
+            // one grain table for all ranks (the reference replicates it)
+            DevBufs &b = c->b;
+            void *own[] = { b.g_rot, b.g_cell, b.g_step, b.g_src, b.g_dom };
+            for (void *q : own)
+                HIPCHK(hipFree(q));
+            const DevBufs &b0 = bt->dom[0]->b;
+            b.g_rot = b0.g_rot;
+            b.g_cell = b0.g_cell;
+            b.g_step = b0.g_step;
+            b.g_src = b0.g_src;
+            b.g_dom = b0.g_dom;
+            c->shares_grains = true;
+        }
+        c->view_dirty = true;
+    }
+    bt->max_face = mf;
+    bt->coupled = true;
+    bt->view_dirty = true;
     return MMCA_OK;
 }
 

@@ -18,8 +18,10 @@ struct Consts {
     long long sxy;          // (nx+2)*(ny+2) (plane pitch in cells)
     long long P;            // padded cells
     int w64;                // 64-bit mask words per padded row
-    int pad[3];             // FaceCtrl per axis
+    int pad_lo[3], pad_hi[3];   // mold wall on the low / high face of each axis: FaceCtrl pad on a
+                                // face of the global domain (blank_moldface, temperature.c:133-152)
     int off[3];             // subblock offset in global cells (sb_coords * n)
+    int nsub[3];            // subblocks per axis of a coupled batch (0: not coupled)
     int gdimz;              // global z extent (2-D runs use y as the axis)
     int zm_planes;          // > 0: z-marching solute kernel with this many planes per workgroup
     int zm_by;              // rows per z-marching workgroup (4, 8, 16)
@@ -69,6 +71,7 @@ struct DevBufs {
     int *gr;
     double *dc;                      // decentered octahedron centres, x,y,z per cell (decentered_t)
     unsigned long long *mask;        // 1 bit per padded cell: gr > 0
+    unsigned char *xhalo;            // per padded row: gr > 0 of its x = 0 and x = nx+1 halo cells (2 bytes)
     // interface list and dc_tmp emulation
     int *diff_id;                    // slots 1..gindex
     unsigned char *nbr;              // per slot: solid neighbours before capture, bit n = capture direction n
@@ -89,6 +92,7 @@ struct DevBufs {
     int *g_cell;                     // padded index of the nucleation cell
     unsigned long long *g_step;      // nucleation timestep
     int *g_src;                      // >=0: fixed table index; <0: -(k+1) random draw k
+    int *g_dom;                      // coupled batches: domain (rank) that created the grain
     double *fixed_rot;               // 9 per fixed nucleation point
     double *rand_rot;                // 9 per random orientation draw
     int rand_cap;                    // random orientations uploaded

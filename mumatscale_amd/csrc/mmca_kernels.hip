@@ -343,6 +343,179 @@ k_nuc_activate(View V, int mode)
         cnt->timestep = ts;
 }
 
+// activateNewGrains over the ranks of a coupled batch (grain.c:841-916), one
+// workgroup: each domain's ordered events (k_nuc_activate mode 1) get ids
+// in rank order after the grains existing so far (the displacements of
+// grain.c:893-898), orientations from the domain's own stream (its fixed
+// entries or its next random draws), in the one grain table the domains
+// share.  Every domain advances its timestep; running past MaxTotGrains
+// stops all of them, as createGrains' error() stops every rank.
+__global__ void __launch_bounds__(NUC_BS)
+k_nuc_activate_coupled(View V)
+{
+    __shared__ int s_w[NUC_BS / 64];
+    const int tid = threadIdx.x;
+    for (int d = 0; d < V.ndom; ++d)
+        if (V.dom[d].b.cnt->status != MMCA_OK)
+            return;                               // uniform: all domains stop together
+    const Counters *c0 = V.dom[0].b.cnt;
+    const unsigned long long ts = c0->timestep + 1ull;
+    const int base_id = c0->num_grains;
+    const long long maxg = V.dom[0].c.max_total_grains;
+    long long total = base_id;
+    for (int d = 0; d < V.ndom; ++d)
+        total += V.dom[d].b.cnt->n_events;
+    const bool over = total > maxg + 1;
+    int next = base_id;
+    for (int d = 0; d < V.ndom && !over; ++d)
+    {
+        const DevBufs b = V.dom[d].b;    // private copy: stores cannot alias it
+        const int nev = b.cnt->n_events;
+        const int rnd0 = b.cnt->random_used;
+        int rused = 0;
+        for (int base = 0; base < nev; base += NUC_BS)
+        {
+            int e = base + tid;
+            int isev = e < nev;
+            int s = isev ? G(b.events)[e] : 0;
+            int fixed = isev ? G(b.site_fixed)[s] : -1;
+            int israndom = isev && fixed < 0;
+            int rtot;
+            int rrank = block_exscan_flag<NUC_BS / 64>(israndom, &rtot, s_w);
+            if (isev)
+            {
+                const long long id = (long long) next + e;
+                int idx = G(b.site_idx)[s];
+                G(b.gr)[idx] = (int) id;
+                const double *rot;
+                int src;
+                if (fixed >= 0)
+                {
+                    rot = b.fixed_rot + 9 * (long long) fixed;
+                    src = fixed;
+                }
+                else
+                {
+                    int k = rnd0 + rused + rrank;
+                    rot = b.rand_rot + 9 * (long long) k;
+                    src = -(k + 1);
+                }
+                for (int q = 0; q < 9; ++q)
+                    b.g_rot[9 * id + q] = rot[q];
+                G(b.g_cell)[id] = idx;
+                G(b.g_step)[id] = ts;
+                G(b.g_src)[id] = src;
+                G(b.g_dom)[id] = d;
+            }
+            rused += rtot;
+        }
+        __syncthreads();
+        if (tid == 0)
+        {
+            b.cnt->random_used = rnd0 + rused;
+            b.cnt->new_grains += nev;
+        }
+        next += nev;
+    }
+    __syncthreads();
+    if (tid == 0)
+        for (int d = 0; d < V.ndom; ++d)
+        {
+            Counters *cnt = V.dom[d].b.cnt;
+            if (over)
+            {
+                // grain.c:868-888, 361-366: with every slot taken nothing is
+                // activated; otherwise createGrains stops the run
+                if ((long long) base_id != maxg + 1)
+                {
+                    cnt->status = MMCA_ERR_GRAIN_CAPACITY;
+                    cnt->failed_step = ts;
+                }
+            }
+            else
+                cnt->num_grains = (int) total;
+            cnt->timestep = ts;
+        }
+}
+
+// Halo exchange between the subblocks of a coupled batch (ExchangeFacesForVar
+// / FinishExchangeForVar, calculate.c:227-330): the interior layer of each
+// neighbouring subblock is copied into this subblock's halo layer, face by
+// face.  Phase 0 (start of the step): cl and fs of buffer `cb` and dc.  Phase
+// 1 (after partitioning): gr, fs of buffer `cb`, and the solid bits of the
+// halo cells (mask words of the y/z halo rows, x-halo bytes).  Only face
+// cells are exchanged: the step reads no edge or corner halo cell.
+// grid.y = domain * 6 + face (face 2a: low side of axis a, 2a+1: high side).
+__global__ void __launch_bounds__(256)
+k_halo_exchange(View V, int phase, int cb)
+{
+    const int d = blockIdx.y / 6, f = blockIdx.y % 6;
+    const int ax = f / 2, hi = f % 2;
+    const DevBufs b = V.dom[d].b;    // private copy: stores cannot alias it
+    const Consts c = V.dom[d].c;
+    if (c.nsub[0] == 0)
+        return;
+    const int n3[3] = { c.nx, c.ny, c.nz };
+    int cn[3];
+    for (int a = 0; a < 3; ++a)
+        cn[a] = c.off[a] / n3[a];
+    cn[ax] += hi ? 1 : -1;
+    if (cn[ax] < 0 || cn[ax] >= c.nsub[ax])
+        return;                                   // a face of the global domain
+    const int r = (cn[2] * c.nsub[1] + cn[1]) * c.nsub[0] + cn[0];   // MPI Cart rank (z, y, x)
+    const DevBufs nb = V.dom[r].b;
+    // the two axes along the face
+    const int u = (ax == 0) ? 1 : 0, v = (ax == 2) ? 1 : 2;
+    const long long nface = (long long) n3[u] * n3[v];
+    const long long t = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    const int hal = hi ? n3[ax] + 1 : 0;          // halo layer of this subblock
+    const int src = hi ? 1 : n3[ax];              // the neighbour's interior layer
+    if (t < nface)
+    {
+        int q[3];
+        q[u] = (int) (t % n3[u]) + 1;
+        q[v] = (int) (t / n3[u]) + 1;
+        q[ax] = hal;
+        const long long dst = q[0] + (long long) q[1] * c.sx + (long long) q[2] * c.sxy;
+        q[ax] = src;
+        const long long from = q[0] + (long long) q[1] * c.sx + (long long) q[2] * c.sxy;
+        if (phase == 0)
+        {
+            if (c.f32)
+                cl32_buf(b, cb)[dst] = cl32_buf(nb, cb)[from];
+            else
+                cl_buf(b, cb)[dst] = cl_buf(nb, cb)[from];
+            fs_buf(b, cb)[dst] = fs_buf(nb, cb)[from];
+            for (int e = 0; e < 3; ++e)
+                G(b.dc)[3 * dst + e] = G(nb.dc)[3 * from + e];
+        }
+        else
+        {
+            const int g = G(nb.gr)[from];
+            G(b.gr)[dst] = g;
+            fs_buf(b, cb)[dst] = fs_buf(nb, cb)[from];
+            if (ax == 0)
+            {
+                const long long row = (long long) q[2] * (c.ny + 2) + q[1];
+                G(b.xhalo)[2 * row + hi] = (unsigned char) (g > 0);
+            }
+        }
+    }
+    if (phase == 1 && ax > 0)
+    {
+        // mask words of the halo rows: y faces, rows (hal, k) for k = 1..nz;
+        // z faces, rows (j, hal) for j = 1..ny
+        const int nrow = (ax == 1) ? c.nz : c.ny;
+        if (t < (long long) nrow * c.w64)
+        {
+            const int w = (int) (t % c.w64), i = (int) (t / c.w64) + 1;
+            const long long rd = (ax == 1) ? ((long long) i * (c.ny + 2) + hal) : ((long long) hal * (c.ny + 2) + i);
+            const long long rs = (ax == 1) ? ((long long) i * (c.ny + 2) + src) : ((long long) src * (c.ny + 2) + i);
+            G(b.mask)[rd * c.w64 + w] = G(nb.mask)[rs * c.w64 + w];
+        }
+    }
+}
+
 // Partitioning of one cell (fs_change_diffuse, growth.c:221-273).  The
 // temperature, temp(), is only evaluated for cells with gr > 0.  *branch
 // receives the MMCA_BRANCH_* code of the branch taken.
@@ -462,9 +635,9 @@ k_solute(View V, int cur, int nxt)
             {
                 const long long nb[6] = { idx + 1, idx - 1, idx + c.sx, idx - c.sx, idx + c.sxy, idx - c.sxy };
                 // mold of a neighbour: a padded face of the box (interior cells are never mold)
-                const int mo[6] = { (x == c.nx) ? c.pad[0] : 0, (x == 1) ? c.pad[0] : 0,
-                                    (j == c.ny) ? c.pad[1] : 0, (j == 1) ? c.pad[1] : 0,
-                                    (k == c.nz) ? c.pad[2] : 0, (k == 1) ? c.pad[2] : 0 };
+                const int mo[6] = { (x == c.nx) ? c.pad_hi[0] : 0, (x == 1) ? c.pad_lo[0] : 0,
+                                    (j == c.ny) ? c.pad_hi[1] : 0, (j == 1) ? c.pad_lo[1] : 0,
+                                    (k == c.nz) ? c.pad_hi[2] : 0, (k == 1) ? c.pad_lo[2] : 0 };
                 double cln[6], fsn[6];
 #pragma unroll
                 for (int n = 0; n < 6; ++n)
@@ -645,8 +818,8 @@ k_solute_zm(View V, int cur, int nxt)
     // the first cell of their row
     const long long xe = xp_mem ? col + 1 : row0;
     const long long ye = ym_mem ? col - c.sx : yp_mem ? col + c.sx : row0 + 1;
-    const int mo_x1 = (x == c.nx) ? c.pad[0] : 0, mo_x0 = (x == 1) ? c.pad[0] : 0;
-    const int mo_y1 = (j == c.ny) ? c.pad[1] : 0, mo_y0 = (j == 1) ? c.pad[1] : 0;
+    const int mo_x1 = (x == c.nx) ? c.pad_hi[0] : 0, mo_x0 = (x == 1) ? c.pad_lo[0] : 0;
+    const int mo_y1 = (j == c.ny) ? c.pad_hi[1] : 0, mo_y0 = (j == 1) ? c.pad_lo[1] : 0;
     auto fetch = [&](ZmAhead<F32> &a, int q) {
         // plane q's ce, gr and edge neighbours; plane q+1's cl and fs
         const long long pq = q * c.sxy;
@@ -707,8 +880,8 @@ k_solute_zm(View V, int cur, int nxt)
             const int g_old = use.g;
             const double cln[6] = { clxp, clxm, clyp, clym, clp, clm };
             const double fsn[6] = { fsxp, fsxm, fsyp, fsym, fsp, fsm };
-            const int mo[6] = { mo_x1, mo_x0, mo_y1, mo_y0, (k == c.nz) ? c.pad[2] : 0,
-                                (k == 1) ? c.pad[2] : 0 };
+            const int mo[6] = { mo_x1, mo_x0, mo_y1, mo_y0, (k == c.nz) ? c.pad_hi[2] : 0,
+                                (k == 1) ? c.pad_lo[2] : 0 };
             const double ce_new = diffuse_cell(c, ce_old, clv, fsv, cln, fsn, mo);
             neg |= (ce_new < 0);
             lost += st_ce<F32>(b, c, idx, ce_new);
@@ -803,6 +976,26 @@ word_pos(const Consts &c, long long wi, int *k, int *j, int *w)
     *k = (int) kk + 1;
 }
 
+// Solid bits of the x-1 (left) and x+1 (right) neighbours of the cells of
+// mask word w; the halo cells x = 0 and x = nx+1 come from the row's x-halo
+// bytes (solid only where another subblock of a coupled batch lies there).
+__device__ __forceinline__ void
+x_neighbour_bits(const DevBufs &b, const Consts &c, int k, int j, int w, unsigned long long own,
+                 unsigned long long *left, unsigned long long *right)
+{
+    *left = own << 1;
+    *right = own >> 1;
+    const long long row = (long long) k * (c.ny + 2) + j;
+    if (w > 0)
+        *left |= mask_word(b, c, k, j, w - 1) >> 63;
+    else
+        *left |= (unsigned long long) (G(b.xhalo)[2 * row] != 0);
+    if (w + 1 < c.w64)
+        *right |= mask_word(b, c, k, j, w + 1) << 63;
+    else
+        *right |= (unsigned long long) (G(b.xhalo)[2 * row + 1] != 0) << (c.nx - 1 - 64 * w);
+}
+
 __device__ __forceinline__ unsigned long long
 member_bits(const DevBufs &b, const Consts &c, long long wi, int *pk, int *pj, int *pw)
 {
@@ -812,12 +1005,8 @@ member_bits(const DevBufs &b, const Consts &c, long long wi, int *pk, int *pj, i
     *pj = j;
     *pw = w;
     unsigned long long own = mask_word(b, c, k, j, w);
-    unsigned long long left = own << 1;
-    unsigned long long right = own >> 1;
-    if (w > 0)
-        left |= mask_word(b, c, k, j, w - 1) >> 63;
-    if (w + 1 < c.w64)
-        right |= mask_word(b, c, k, j, w + 1) << 63;
+    unsigned long long left, right;
+    x_neighbour_bits(b, c, k, j, w, own, &left, &right);
     unsigned long long yz = mask_word(b, c, k, j - 1, w) | mask_word(b, c, k, j + 1, w) |
                             mask_word(b, c, k - 1, j, w) | mask_word(b, c, k + 1, j, w);
     // bit b of word w is interior x = 64 w + b + 1 <= nx
@@ -957,12 +1146,7 @@ k_list_write(View V)
         const unsigned long long own = mask_word(b, c, k, j, w);
         // bit b of each word: the neighbour of interior x = 64 w + b + 1 in the
         // capture order +x, +y, -x, -y, +z, -z (growth.c:604-610)
-        nb[0] = own >> 1;
-        nb[2] = own << 1;
-        if (w + 1 < c.w64)
-            nb[0] |= mask_word(b, c, k, j, w + 1) << 63;
-        if (w > 0)
-            nb[2] |= mask_word(b, c, k, j, w - 1) >> 63;
+        x_neighbour_bits(b, c, k, j, w, own, &nb[2], &nb[0]);
         nb[1] = mask_word(b, c, k, j + 1, w);
         nb[3] = mask_word(b, c, k, j - 1, w);
         nb[4] = mask_word(b, c, k + 1, j, w);
@@ -1306,6 +1490,25 @@ hipError_t launch_nucleate(const View &v, int mode, int cur, hipStream_t s)
         hipLaunchKernelGGL(k_nuc_detect, dim3((unsigned) v.tot_nuc), dim3(NUC_BS), 0, s, v,
                            (mode & 4) ? 1 : 0, cur);
     hipLaunchKernelGGL(k_nuc_activate, dim3(v.ndom), dim3(NUC_BS), 0, s, v, mode);
+    return hipGetLastError();
+}
+
+// Nucleation of a coupled batch: detection and event order per domain, then
+// activation across the domains in rank order.
+hipError_t launch_nucleate_coupled(const View &v, int cur, hipStream_t s)
+{
+    if (v.tot_nuc > 0)
+        hipLaunchKernelGGL(k_nuc_detect, dim3((unsigned) v.tot_nuc), dim3(NUC_BS), 0, s, v, 1, cur);
+    hipLaunchKernelGGL(k_nuc_activate, dim3(v.ndom), dim3(NUC_BS), 0, s, v, 1);
+    hipLaunchKernelGGL(k_nuc_activate_coupled, dim3(1), dim3(NUC_BS), 0, s, v);
+    return hipGetLastError();
+}
+
+// `max_face`: the largest face (cells) or halo-row word count of any domain
+hipError_t launch_halo_exchange(const View &v, int phase, int cb, long long max_face, hipStream_t s)
+{
+    const unsigned gx = (unsigned) ((max_face + 255) / 256);
+    hipLaunchKernelGGL(k_halo_exchange, dim3(gx, (unsigned) (6 * v.ndom)), dim3(256), 0, s, v, phase, cb);
     return hipGetLastError();
 }
 

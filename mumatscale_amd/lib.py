@@ -76,6 +76,7 @@ def lib():
     L.mmca_batch_domain.argtypes = [vp, C.c_int, C.POINTER(vp)]
     L.mmca_batch_step.argtypes = [vp, C.c_int, C.c_int, C.POINTER(abi.Report)]
     L.mmca_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.mmca_batch_couple.argtypes = [vp, i32p]
     L.mmca_batch_set_kernel_timing.argtypes = [vp, C.c_int]
     L.mmca_batch_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.mmca_batch_autotune.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg), C.c_int, C.c_int,
@@ -397,6 +398,12 @@ class Batch:
 
     def set_tuning(self, key, value):
         _check(lib().mmca_batch_set_tuning(self._h, key, value), "mmca_batch_set_tuning")
+
+    def couple(self, nsub):
+        """Make the domains the subblocks (ranks) of one global domain split
+        nsub[0] x nsub[1] x nsub[2] (mmca_batch_couple)."""
+        arr = (C.c_int32 * 3)(*[int(v) for v in nsub])
+        _check(lib().mmca_batch_couple(self._h, arr), "mmca_batch_couple")
 
     def set_kernel_timing(self, enable=True):
         _check(lib().mmca_batch_set_kernel_timing(self._h, 1 if enable else 0), "mmca_batch_set_kernel_timing")

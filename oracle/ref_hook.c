@@ -16,6 +16,11 @@
  * Outputs:
  *   <dir>/hashes.txt        one line per step: FNV-1a 64 of each padded field
  *   <dir>/state_NNNNNN.bin  full padded fields at the requested steps
+ *
+ * Runs on several MPI ranks write, instead, one line per step and rank to
+ * <dir>/ihash_rNN.txt: a word hash (ihash below) of each field's interior
+ * cells, so that a coupled emulation whose edge and corner halo cells are
+ * never written can be compared with each rank's subblock.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -139,11 +144,92 @@ hash_line(const char *dir, uint64_t ts)
     fclose(f);
 }
 
+/* Hash of 64-bit words w_i: sum over i of mix(w_i + i * C1) modulo 2^64,
+ * mix(x) = (x ^ (x >> 31)) * C2.  Order-dependent and easy to restate with
+ * vector arithmetic. */
+static uint64_t
+ihash_add(uint64_t h, uint64_t w, uint64_t i)
+{
+    uint64_t x = w + i * 0x9E3779B97F4A7C15ULL;
+    x ^= x >> 31;
+    x *= 0xBF58476D1CE4E5B9ULL;
+    return h + x;
+}
+
+/* interior cells of a padded field, raster order; `elem` bytes per cell
+ * (4: zero-extended to a word; 8: one word; 24: three words) */
+static uint64_t
+ihash_field(const void *base, size_t elem)
+{
+    const int nx = bp->gsdimx, ny = bp->gsdimy, nz = bp->gsdimz;
+    const unsigned char *p = (const unsigned char *) base;
+    uint64_t h = 0, i = 0;
+    for (int z = 1; z <= nz; z++)
+        for (int y = 1; y <= ny; y++)
+            for (int x = 1; x <= nx; x++)
+            {
+                size_t idx = (size_t) x + (size_t) y * (nx + 2) + (size_t) z * (nx + 2) * (ny + 2);
+                if (elem == 4)
+                {
+                    uint32_t v;
+                    memcpy(&v, p + idx * 4, 4);
+                    h = ihash_add(h, (uint64_t) v, i++);
+                }
+                else
+                    for (size_t e = 0; e < elem / 8; e++)
+                    {
+                        uint64_t v;
+                        memcpy(&v, p + idx * elem + 8 * e, 8);
+                        h = ihash_add(h, v, i++);
+                    }
+            }
+    return h;
+}
+
+static void
+ihash_line(const char *dir, uint64_t ts)
+{
+    char path[1024];
+    snprintf(path, sizeof(path), "%s/ihash_r%02d.txt", dir, iproc);
+    FILE *f = fopen(path, ts == 0 ? "w" : "a");
+    if (!f)
+        exit(3);
+    uint64_t hl = 0;
+    for (int i = 1; i <= lsp->gindex; i++)
+        hl = ihash_add(hl, (uint64_t) (uint32_t) lsp->diff_id[i], (uint64_t) (i - 1));
+    uint64_t hg = 0, i = 0;
+    for (int g = 1; g < bp->num_grains; g++)
+    {
+        for (int q = 0; q < 9; q++)
+        {
+            uint64_t v;
+            memcpy(&v, &grain_cache[g].rotmat[q / 3][q % 3], 8);
+            hg = ihash_add(hg, v, i++);
+        }
+        hg = ihash_add(hg, (uint64_t) grain_cache[g].nuc_timestep, i++);
+    }
+    fprintf(f, "%lu %d %d %016lx %016lx %016lx %016lx %016lx %016lx %016lx\n",
+            (unsigned long) ts, bp->num_grains, lsp->gindex,
+            (unsigned long) ihash_field(lsp->ce, 8), (unsigned long) ihash_field(lsp->cl, 8),
+            (unsigned long) ihash_field(lsp->fs, 8), (unsigned long) ihash_field(lsp->gr, 4),
+            (unsigned long) ihash_field(lsp->dc, 24), (unsigned long) hl, (unsigned long) hg);
+    fclose(f);
+}
+
 void
 __wrap_doiteration(void)
 {
     static int first = 1;
     const char *dir = getenv("MMCA_REF_DUMP_DIR");
+    if (dir && nproc > 1)
+    {
+        if (first)
+            ihash_line(dir, 0);
+        first = 0;
+        __real_doiteration();
+        ihash_line(dir, bp->timestep);
+        return;
+    }
     if (first && dir)
     {
         /* state after initialisation, before step 1 */

@@ -11,6 +11,15 @@ dump hook (oracle/ref_hook.c) writes:
   <case>/expected.json     exit status, last completed step, the reference's
                            "N grains ... Average" line
 
+Control files in tests/golden/cases_mr/ split the block into several
+subblocks; each runs on as many MPI ranks (MPICH's mpiexec from the image)
+and keeps, per rank and step, the hook's interior hashes:
+
+  <case>/ihash.txt.gz      "rank ts num_grains gindex" and word hashes of the
+                           interior ce, cl, fs, gr, dc, of the rank's
+                           interface list and of the (shared) grain table
+  <case>/expected.json     as above
+
 Run in the container that holds the reference sources:
     python tests/golden/make_golden.py
 """
@@ -27,6 +36,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 CASES = os.path.join(HERE, "cases")
+CASES_MR = os.path.join(HERE, "cases_mr")
+MPIEXEC = "/opt/conda/bin/mpiexec"
 REF_BIN = os.path.join(REPO, "oracle", "_ref", "mumatscale_ref")
 
 
@@ -70,6 +81,43 @@ def run_case(name, out_dir):
         return exp
 
 
+def nranks(ctrl):
+    for line in open(ctrl):
+        w = line.split()
+        if w and w[0] == "NSubblocks":
+            return int(w[1]) * int(w[2]) * int(w[3])
+    return 1
+
+
+def run_case_mr(name, out_dir):
+    ctrl = os.path.join(CASES_MR, name + ".in")
+    n = nranks(ctrl)
+    with tempfile.TemporaryDirectory() as tmp:
+        env = dict(os.environ, MMCA_REF_DUMP_DIR=tmp, OMP_NUM_THREADS="1")
+        proc = subprocess.run([MPIEXEC, "-hosts", "127.0.0.1", "-n", str(n), REF_BIN, "-c", ctrl], cwd=tmp,
+                              env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=3600)
+        text = proc.stdout.decode(errors="replace")
+        rows = []
+        for r in range(n):
+            for line in open(os.path.join(tmp, "ihash_r%02d.txt" % r)).read().splitlines():
+                rows.append("%d %s" % (r, line))
+        os.makedirs(out_dir, exist_ok=True)
+        with gzip.open(os.path.join(out_dir, "ihash.txt.gz"), "wt") as fh:
+            fh.write("\n".join(rows) + "\n")
+        avg = [l for l in text.splitlines() if "Average" in l]
+        err = [l.strip() for l in text.splitlines() if "ERROR" in l]
+        exp = {
+            "returncode": proc.returncode,
+            "ranks": n,
+            "last_step": max(int(r.split()[1]) for r in rows),
+            "error_line": err[0] if err else None,
+            "average_line": avg[0].strip() if avg else None,
+        }
+        with open(os.path.join(out_dir, "expected.json"), "w") as fh:
+            json.dump(exp, fh, indent=1)
+        return exp
+
+
 def main():
     subprocess.check_call(["make", "-s", "-f", os.path.join(REPO, "oracle", "ref.mk")], cwd=REPO)
     names = sorted(f[:-3] for f in os.listdir(CASES) if f.endswith(".in"))
@@ -80,6 +128,15 @@ def main():
         if os.path.isdir(out):
             shutil.rmtree(out)
         exp = run_case(n, out)
+        print(n, exp)
+    names = sorted(f[:-3] for f in os.listdir(CASES_MR) if f.endswith(".in"))
+    if len(sys.argv) > 1:
+        names = [n for n in names if n in sys.argv[1:]]
+    for n in names:
+        out = os.path.join(HERE, n)
+        if os.path.isdir(out):
+            shutil.rmtree(out)
+        exp = run_case_mr(n, out)
         print(n, exp)
 
 

@@ -45,3 +45,64 @@ def hash_line(fnv, ts, num_grains, gindex, fields, diff_id, grain_bytes):
     out.append("%016x" % fnv(np.asarray(diff_id, dtype=np.int32)))
     out.append("%016x" % (fnv(np.frombuffer(grain_bytes, np.uint8)) if grain_bytes else FNV_EMPTY))
     return out
+
+
+# --- multi-rank fixtures (tests/golden/cases_mr/, one subblock per MPI rank) ---
+
+CASES_MR = sorted(f[:-3] for f in os.listdir(os.path.join(GOLDEN, "cases_mr")) if f.endswith(".in"))
+IHASH_NAMES = ["ce", "cl", "fs", "gr", "dc", "diff_id", "grains"]
+_C1 = np.uint64(0x9E3779B97F4A7C15)
+_C2 = np.uint64(0xBF58476D1CE4E5B9)
+
+
+def ihash_words(w):
+    """The hook's word hash: sum over i of mix(w_i + i*C1) mod 2^64 with
+    mix(x) = (x ^ (x >> 31)) * C2 (oracle/ref_hook.c, ihash_add)."""
+    w = np.ascontiguousarray(w, dtype=np.uint64).reshape(-1)
+    i = np.arange(w.size, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = w + i * _C1
+        x ^= x >> np.uint64(31)
+        x *= _C2
+        return int(x.sum(dtype=np.uint64))
+
+
+def ihash_field(a):
+    """Interior cells of a padded field (nz+2, ny+2, nx+2[, 3]) in raster
+    order: 8-byte values as words, 4-byte values zero-extended."""
+    inner = np.ascontiguousarray(a[1:-1, 1:-1, 1:-1])
+    if inner.dtype.itemsize == 4:
+        return ihash_words(inner.view(np.uint32).astype(np.uint64))
+    return ihash_words(inner.view(np.uint64))
+
+
+def ihash_list(diff_id):
+    return ihash_words(np.asarray(diff_id, dtype=np.int64).astype(np.uint32).astype(np.uint64))
+
+
+def ihash_grains(grains, num_grains):
+    words = []
+    for g in grains[1:num_grains]:
+        rot = np.array([[g.rotmat[a][b] for b in range(3)] for a in range(3)], dtype=np.float64)
+        words.append(rot.reshape(-1).view(np.uint64))
+        words.append(np.array([g.nuc_timestep], dtype=np.uint64))
+    return ihash_words(np.concatenate(words)) if words else 0
+
+
+def load_case_mr(case):
+    """(cfg, rows, expected): rows[rank][ts] = [num_grains, gindex, hashes...]"""
+    cfg = read_ctrl(os.path.join(GOLDEN, "cases_mr", case + ".in"))
+    rows = {}
+    with gzip.open(os.path.join(GOLDEN, case, "ihash.txt.gz"), "rt") as fh:
+        for line in fh:
+            w = line.split()
+            if w:
+                rows.setdefault(int(w[0]), {})[int(w[1])] = [int(w[2]), int(w[3])] + [int(v, 16) for v in w[4:]]
+    with open(os.path.join(GOLDEN, case, "expected.json")) as fh:
+        expected = json.load(fh)
+    return cfg, rows, expected
+
+
+def rank_coords(cfg, r):
+    n = cfg.nsub
+    return (r % n[0], (r // n[0]) % n[1], r // (n[0] * n[1]))

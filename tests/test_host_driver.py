@@ -128,3 +128,38 @@ def test_write_ctrl_round_trips(path, tmp_path):
         assert [list(t) for t in back.volumes] == [list(t) for t in cfg.volumes]
         assert (back.finish_time, back.fs_finish, back.basefilename) == (cfg.finish_time, cfg.fs_finish,
                                                                           cfg.basefilename)
+
+
+def _run_reference_input(name, tmp_path, timeout=900):
+    path = os.path.join(GOLDEN, "reference_inputs", name)
+    r = subprocess.run([RUN, path], cwd=tmp_path, capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    avg = [l for l in r.stdout.splitlines() if "Average" in l]
+    assert len(avg) == 1
+    w = avg[0].split()
+    screen = [l.split() for l in r.stdout.splitlines() if len(l.split()) == 4 and l.split()[2].endswith("%")]
+    return int(w[0]), float(w[9]), screen
+
+
+@pytest.mark.gpu
+def test_multigrains_through_c_driver(tmp_path):
+    """The reference's MultiGrains regression (tests/testMultiGrains.py:25-37,
+    ca6mpi.in on 6 MPI ranks): 336 grains, average 7.14e-5 mm^3 within 1e-6.
+    The driver runs the 1 x 2 x 3 subblocks as 6 coupled ranks on one GPU."""
+    ngrains, avg, _ = _run_reference_input("ca6mpi.in", tmp_path)
+    assert ngrains == 336
+    assert abs(avg - 7.14e-5) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_amtrack_through_c_driver(tmp_path):
+    """The reference's AMtrack regression (tests/testAMtrack.py:25-56,
+    am_singletrack.in on 6 MPI ranks): 1889 grains, average 9.1e-7 mm^3
+    within 1e-8, last solid fraction 86 % within 0.2."""
+    ngrains, avg, screen = _run_reference_input("am_singletrack.in", tmp_path)
+    fs = float(screen[-1][2][:-1])
+    print("grains", ngrains, "average", avg, "fs", fs)
+    assert ngrains == 1889
+    assert abs(avg - 9.1e-7) <= 1e-8
+    assert abs(fs - 86.0) <= 0.2

@@ -551,6 +551,8 @@ int mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report)
 {
     if (!ctx || nsteps < 0)
         return MMCA_ERR_ARG;
+    if (!ctx->peers.empty())
+        return MMCA_ERR_STATE;   // a rank of a coupled batch steps with the batch
     HIPCHK(hipSetDevice(ctx->device));
     TRY(sync_counters(ctx));
     if (ctx->hc.status != MMCA_OK)
@@ -627,6 +629,8 @@ int mmca_set_record(mmca_ctx *ctx, uint8_t *codes, int64_t n_steps)
 {
     if (!ctx || n_steps < 0)
         return MMCA_ERR_ARG;
+    if (!ctx->peers.empty())
+        return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
     if (codes && n_steps > 0 && ctx->c.f32)
         return MMCA_ERR_ARG;     // the adjoint works on FP64 storage
@@ -762,6 +766,8 @@ int mmca_doiteration(mmca_ctx *ctx)
 {
     if (!ctx)
         return MMCA_ERR_ARG;
+    if (!ctx->peers.empty())
+        return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
     TRY(sync_counters(ctx));
     if (ctx->hc.status != MMCA_OK)
@@ -1014,6 +1020,8 @@ int mmca_rand(mmca_ctx *ctx, int32_t *out, int32_t n)
     return MMCA_OK;
 }
 
+}  // extern "C"
+
 namespace {
 
 // Restart image of one block: a header, the counters, the random stream at
@@ -1129,10 +1137,14 @@ int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc)
 
 }  // namespace
 
+extern "C" {
+
 int mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size)
 {
     if (!ctx || capacity < 0 || (buf && capacity == 0))
         return MMCA_ERR_ARG;
+    if (!ctx->peers.empty())
+        return MMCA_ERR_STATE;   // the image covers one block with its own grain table
     if (!ctx->nucleation_ready)
         return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
@@ -1175,6 +1187,8 @@ int mmca_load_state(mmca_ctx *ctx, const void *buf, int64_t size)
 {
     if (!ctx || !buf || size < (int64_t) sizeof(StateHeader))
         return MMCA_ERR_ARG;
+    if (!ctx->peers.empty())
+        return MMCA_ERR_STATE;
     if (!ctx->nucleation_ready)
         return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
@@ -1245,6 +1259,8 @@ int mmca_set_grains(mmca_ctx *ctx, const mmca_grain *grains, int32_t num_grains)
 {
     if (!ctx || !grains || num_grains < 1 || (int64_t) num_grains > ctx->p.max_total_grains + 1)
         return MMCA_ERR_ARG;
+    if (!ctx->peers.empty())
+        return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
     TRY(sync_counters(ctx));
     const size_t n = (size_t) num_grains;

@@ -84,3 +84,30 @@ def test_coupled_batch_matches_reference_ranks(case):
                 if taint[r] is not None and t >= taint[r] and bad == ["dc"]:
                     bad = []
                 assert not bad, (case, t, r, bad)
+
+
+def test_coupling_rejects_what_it_cannot_emulate():
+    """Wrong layouts and periodic faces across subblocks are refused; a
+    coupled rank cannot be stepped, restarted or given grains on its own."""
+    cfg, _, _ = load_case_mr("m1_directional_z2")
+    params = [cfg.params(whole_domain=False, sb_coords=rank_coords(cfg, r)) for r in range(2)]
+    nucs = [cfg.nucleation(rank=r) for r in range(2)]
+    with Batch(params, nucs) as bt:
+        with pytest.raises(abi.MmcaError):
+            bt.couple([2, 1, 1])                  # sb_coords do not match an x split
+        with pytest.raises(abi.MmcaError):
+            bt.couple([1, 1, 3])                  # three ranks for two domains
+    wrap = [cfg.params(whole_domain=False, sb_coords=rank_coords(cfg, r)) for r in range(2)]
+    for p in wrap:
+        p.pad_bndy[2] = 0
+    with Batch(wrap, nucs) as bt:
+        with pytest.raises(abi.MmcaError):
+            bt.couple(cfg.nsub)
+    with Batch(params, nucs) as bt:
+        bt.couple(cfg.nsub)
+        with pytest.raises(abi.MmcaError):
+            bt.domains[0].step(1)
+        with pytest.raises(abi.MmcaError):
+            bt.domains[1].save_state()
+        reps = bt.step(3)
+        assert [r.timestep for r in reps] == [3, 3]

@@ -46,6 +46,8 @@ mmca_ctrl_defaults(mmca_run_cfg *c)
     c->part_coef = 0.3;
     c->am_layers = 1;
     c->am_tracks = 1;
+    c->vis_format = MMCA_VIS_NONE;
+    c->randomize_grains = 1;
 }
 
 void
@@ -206,6 +208,27 @@ apply(mmca_run_cfg *c, const char *key, const char *rest, const char *path)
         return need(sscanf(rest, "%d", &c->screenpfreq), 1, key, path);
     else if (!strcasecmp(key, "DataWriteFreq"))
         return need(sscanf(rest, "%d", &c->data_write_freq), 1, key, path);
+    else if (!strcasecmp(key, "DataWriteStart"))
+        return need(sscanf(rest, "%d", &c->data_write_start), 1, key, path);
+    else if (!strcasecmp(key, "RandomizeGrains"))
+        return need(sscanf(rest, "%d", &c->randomize_grains), 1, key, path);
+    else if (!strcasecmp(key, "DataFormat"))
+    {
+        char fmt[64] = { 0 };
+        if (need(sscanf(rest, "%63s", fmt), 1, key, path))
+            return -1;
+        if (!strcasecmp(fmt, "XDMF"))
+            c->vis_format = MMCA_VIS_XDMF;
+        else if (!strcasecmp(fmt, "VTK"))
+            c->vis_format = MMCA_VIS_VTK;
+        else
+        {
+            /* read_ctrl.c:601-620 takes VTK or XDMF only (NONE is the default) */
+            fprintf(stderr, "%s: unsupported data file type: %s\n", path, fmt);
+            return -1;
+        }
+        return 0;
+    }
     else if (!strcasecmp(key, "GeoFileName") || !strcasecmp(key, "MatFileName") ||
              !strcasecmp(key, "AlloyPropsFile0"))
     {

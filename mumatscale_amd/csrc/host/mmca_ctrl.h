@@ -9,13 +9,19 @@
 
 #include "mmca.h"
 
+#define MMCA_VIS_NONE 0
+#define MMCA_VIS_VTK  1
+#define MMCA_VIS_XDMF 2
+
 typedef struct mmca_run_cfg {
     char     basefilename[256];   /* BaseFileName                         */
     int      nsub[3];             /* NSubblocks                           */
     int      sbdim[3];            /* NCellsPerSB                          */
     int      pad[3];              /* FaceCtrl per axis                    */
     double   cell_size, courant_limit, dt, finish_time, fs_finish;
-    int      screenpfreq, data_write_freq;
+    int      screenpfreq, data_write_freq, data_write_start;
+    int      vis_format;          /* DataFormat: MMCA_VIS_NONE / _VTK / _XDMF */
+    int      randomize_grains;    /* RandomizeGrains                      */
     double   t_init, cool_rate, liquidus_temp, solidus_temp;
     uint32_t random_seed;
     double   bulk_density, bulk_mean, bulk_sigma;

@@ -9,6 +9,11 @@
  *   computeFS    loop.c:173-182   solid_volume / totalNonMoldVolume
  *   output_grains grain.c:752-833 <BaseFileName>_grains.csv and the
  *                                 "N grains. Min ... Average" line
+ *   writeData    loop.c:116-127   DataFormat XDMF: at timestep 0, every
+ *                                 DataWriteFreq steps from DataWriteStart on
+ *                                 and after the last step (mmca_xdmf.h; the
+ *                                 writer lives in libmmca_xdmf.so, loaded
+ *                                 only when such output is asked for)
  *
  * With NSubblocks x y z the domain runs as the reference runs it on x*y*z
  * MPI ranks: one subblock per rank, each with its own random stream, halos
@@ -20,14 +25,55 @@
  * Usage: mmca_run <ctrl-file> [--device N] [--max-steps N] [--fp32]
  *                 [--no-ce-check] [--one-block] [--dump-ctrl]
  */
+#include <dlfcn.h>
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <sys/time.h>
+#include <unistd.h>
 
 #include "mmca.h"
 #include "mmca_ctrl.h"
+#include "mmca_xdmf.h"
+
+/* The XDMF writer and libhdf5 behind it, from libmmca_xdmf.so beside this
+ * program; runs without data output do not need HDF5 at all. */
+typedef struct xdmf_lib {
+    void *so;
+    mmca_xdmf *(*open)(const mmca_run_cfg *, const int *, const int *);
+    int (*write)(mmca_xdmf *, mmca_ctx *const *, const mmca_params *, uint64_t);
+    void (*close)(mmca_xdmf *);
+    mmca_xdmf *x;
+} xdmf_lib;
+
+static int
+xdmf_load(xdmf_lib *w)
+{
+    char path[PATH_MAX];
+    ssize_t n = readlink("/proc/self/exe", path, sizeof(path) - 32);
+    if (n <= 0)
+        return -1;
+    path[n] = 0;
+    char *slash = strrchr(path, '/');
+    strcpy(slash ? slash + 1 : path, "libmmca_xdmf.so");
+    w->so = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!w->so)
+    {
+        fprintf(stderr, "DataFormat XDMF needs %s: %s\n", path, dlerror());
+        return -1;
+    }
+    *(void **) &w->open = dlsym(w->so, "mmca_xdmf_open");
+    *(void **) &w->write = dlsym(w->so, "mmca_xdmf_write");
+    *(void **) &w->close = dlsym(w->so, "mmca_xdmf_close");
+    if (!w->open || !w->write || !w->close)
+    {
+        fprintf(stderr, "%s lacks the writer's entry points\n", path);
+        return -1;
+    }
+    return 0;
+}
 
 static double
 wall_now(void)
@@ -62,6 +108,9 @@ dump_ctrl(const mmca_run_cfg *c, const mmca_params *p, const mmca_nucleation_cfg
     printf("basefilename %s\n", c->basefilename);
     printf("nsub %d %d %d\n", c->nsub[0], c->nsub[1], c->nsub[2]);
     printf("finish_time %.17g\nfs_finish %.17g\nscreenpfreq %d\n", c->finish_time, c->fs_finish, c->screenpfreq);
+    printf("data_write_freq %d\ndata_write_start %d\ndata_format %s\nrandomize_grains %d\n", c->data_write_freq,
+           c->data_write_start, c->vis_format == MMCA_VIS_XDMF ? "XDMF" : c->vis_format == MMCA_VIS_VTK ? "VTK" : "NONE",
+           c->randomize_grains);
     printf("nx %d\nny %d\nnz %d\n", p->nx, p->ny, p->nz);
     printf("pad_bndy %d %d %d\n", p->pad_bndy[0], p->pad_bndy[1], p->pad_bndy[2]);
     printf("gdim %d %d %d\n", p->gdim[0], p->gdim[1], p->gdim[2]);
@@ -262,6 +311,20 @@ main(int argc, char **argv)
     else if (check(mmca_create(&ps[0], device, &doms[0]), "mmca_create") ||
              check(mmca_init_nucleation(doms[0], &ns[0]), "mmca_init_nucleation"))
         return 1;
+    /* writeData's output (loop.c:116-127); VTK is not written by this driver */
+    xdmf_lib out = { 0 };
+    const int write_freq = cfg.vis_format == MMCA_VIS_XDMF ? cfg.data_write_freq : 0;
+    if (cfg.data_write_freq > 0 && cfg.vis_format == MMCA_VIS_VTK)
+        fprintf(stderr, "DataFormat VTK is not written by this driver; no data files.\n");
+    if (write_freq > 0)
+    {
+        const int one[3] = { 1, 1, 1 };
+        if (xdmf_load(&out))
+            return 1;
+        out.x = out.open(&cfg, nr > 1 ? cfg.nsub : one, nr > 1 ? cfg.sbdim : p.gdim);
+        if (!out.x)
+            return 1;
+    }
     printf("Initial Subblocks assigned.  Starting calculations.\n");
     const double total_volume = (p.gdim[0] * p.gdim[1] * p.gdim[2]) * pow(cfg.cell_size, 3);
     const double cell_vol = pow(cfg.cell_size, 3.0);
@@ -271,11 +334,22 @@ main(int argc, char **argv)
     double fs = 0.0;
     const uint64_t fin = cfg.finish_time ? finish_step(1, cfg.dt, cfg.finish_time) : 0;
     int status = MMCA_OK;
-    for (;;)
+    if (write_freq > 0 && out.write(out.x, doms, ps, 0))
+        status = MMCA_ERR_ARG;
+    while (status == MMCA_OK)
     {
         uint64_t next = UINT64_MAX;
         if (cfg.screenpfreq > 0)
             next = (t / cfg.screenpfreq + 1) * cfg.screenpfreq;
+        if (write_freq > 0)
+        {
+            /* next output step: a multiple of DataWriteFreq at or after DataWriteStart */
+            uint64_t w = (t / write_freq + 1) * write_freq;
+            if (cfg.data_write_start > 0 && w < (uint64_t) cfg.data_write_start)
+                w = ((uint64_t) cfg.data_write_start + write_freq - 1) / write_freq * write_freq;
+            if (w < next)
+                next = w;
+        }
         if (fin && (fin > t ? fin : t + 1) < next)
             next = fin > t ? fin : t + 1;
         if (max_steps >= 0 && (uint64_t) max_steps < next)
@@ -307,6 +381,12 @@ main(int argc, char **argv)
                    reps[0].num_grains - 1);
             fflush(stdout);
         }
+        if (write_freq > 0 && (long long) t >= cfg.data_write_start && t % write_freq == 0 &&
+            out.write(out.x, doms, ps, t))
+        {
+            status = MMCA_ERR_ARG;
+            break;
+        }
         int done = 0;
         if (cfg.finish_time && t * cfg.dt >= cfg.finish_time)
         {
@@ -323,6 +403,11 @@ main(int argc, char **argv)
         if (done)
             break;
     }
+    /* the final solution, unless this step was written already */
+    if (status == MMCA_OK && write_freq > 0 && t % write_freq != 0 && out.write(out.x, doms, ps, t))
+        status = MMCA_ERR_ARG;
+    if (out.x)
+        out.close(out.x);
     int rc = status == MMCA_OK ? output_grains(doms, ps, nr, &cfg) : 1;
     if (bt)
         mmca_batch_destroy(bt);

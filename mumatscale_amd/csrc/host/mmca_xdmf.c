@@ -1,0 +1,406 @@
+/*
+ * XDMF + HDF5 output of the C host driver (the reference's io_xdmf.c
+ * layout).  The fields come from the device through mmca_io_fields
+ * (cache_io_data, loop.c:74-113); the grain renumbering draws from rank 0's
+ * random stream through mmca_rand, so later orientations stay those of a
+ * reference run that writes the same files.
+ */
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include <hdf5.h>
+
+#include "mmca_xdmf.h"
+
+struct mmca_xdmf {
+    const mmca_run_cfg *cfg;
+    int nsub[3], sbdim[3];     /* subblocks per axis, cells per subblock */
+    int nranks;
+    FILE *ts_fp;               /* time series, open for the whole run */
+    size_t *renumber;          /* getRenumberArray (io_xdmf.c:344-375) */
+    size_t renumber_sz;
+};
+
+/* Cart coordinates of a rank (x fastest), the reference's (cx, cy, cz) */
+static void
+rank_coords(const mmca_xdmf *x, int r, int *cx, int *cy, int *cz)
+{
+    *cx = r % x->nsub[0];
+    *cy = (r / x->nsub[0]) % x->nsub[1];
+    *cz = r / (x->nsub[0] * x->nsub[1]);
+}
+
+/* BaseFileName without its directories (get_local_rank_file, io_xdmf.c:82-95) */
+static const char *
+local_base(const mmca_run_cfg *c)
+{
+    const char *p = strrchr(c->basefilename, '/');
+    return p ? p + 1 : c->basefilename;
+}
+
+mmca_xdmf *
+mmca_xdmf_open(const mmca_run_cfg *cfg, const int nsub[3], const int sbdim[3])
+{
+    mmca_xdmf *x = calloc(1, sizeof(*x));
+    if (!x)
+        return NULL;
+    x->cfg = cfg;
+    for (int d = 0; d < 3; d++)
+    {
+        x->nsub[d] = nsub[d];
+        x->sbdim[d] = sbdim[d];
+    }
+    x->nranks = nsub[0] * nsub[1] * nsub[2];
+    for (int r = 0; r < x->nranks; r++)
+    {
+        char dir[600];
+        snprintf(dir, sizeof(dir), "%s_r%d", cfg->basefilename, r);
+        if (mkdir(dir, 0755) != 0 && errno != EEXIST)
+        {
+            fprintf(stderr, "Failed to create directory %s:  %d\n", dir, errno);
+            free(x);
+            return NULL;
+        }
+    }
+    return x;
+}
+
+/* getRenumberArray: a new random permutation of the ids whenever the grain
+ * count has grown (grain 0 stays 0), from rank 0's stream */
+static int
+renumber_array(mmca_xdmf *x, mmca_ctx *dom0, int ng)
+{
+    if ((size_t) ng > x->renumber_sz)
+    {
+        free(x->renumber);
+        x->renumber = malloc(sizeof(size_t) * (size_t) ng);
+        if (!x->renumber)
+            return -1;
+        x->renumber_sz = (size_t) ng;
+        for (int i = 0; i < ng; i++)
+            x->renumber[i] = (size_t) i;
+        for (int i = 1; i < ng; i++)
+        {
+            int picidx;
+            do
+            {
+                int32_t r;
+                if (mmca_rand(dom0, &r, 1) != MMCA_OK)
+                    return -1;
+                picidx = r % ng;
+            }
+            while (picidx == 0);
+            size_t tmp = x->renumber[picidx];
+            x->renumber[picidx] = x->renumber[i];
+            x->renumber[i] = tmp;
+        }
+    }
+    if (!x->renumber)
+    {
+        x->renumber = malloc(sizeof(size_t));
+        if (!x->renumber)
+            return -1;
+        x->renumber[0] = 0;
+    }
+    return 0;
+}
+
+/* writeXDMFGrids (io_xdmf.c:141-304) */
+static void
+write_grids(mmca_xdmf *x, FILE *fp, uint64_t ts, const char *in)
+{
+    const mmca_run_cfg *c = x->cfg;
+    const int nx = x->sbdim[0], ny = x->sbdim[1], nz = x->sbdim[2];
+    const char *b = local_base(c);
+    fprintf(fp, "%s<Grid Name=\"Full Volume at time %g\" GridType=\"Collection\" CollectionType=\"Spatial\">\n", in,
+            ts * c->dt);
+    fprintf(fp, "%s\t<Time Value=\"%g\" />\n", in, ts * c->dt);
+    for (int i = 0; i < x->nranks; i++)
+    {
+        int cx, cy, cz;
+        rank_coords(x, i, &cx, &cy, &cz);
+        char file[700], sub[64];
+        snprintf(file, sizeof(file), "%s_r%d/%05lu.h5", b, i, (unsigned long) ts);
+        snprintf(sub, sizeof(sub), "/SubBlock_%dx%dx%d", cx, cy, cz);
+        fprintf(fp, "%s\t<Grid Name=\"SubBlock %dx%dx%d (%d)\" GridType=\"Uniform\">\n", in, cx, cy, cz, i);
+        fprintf(fp, "%s\t\t<Topology TopologyType=\"3DCoRectMesh\" Dimensions=\"%u %u %u\" />\n", in, nz + 1, ny + 1,
+                nx + 1);
+        fprintf(fp, "%s\t\t<Geometry GeometryType=\"ORIGIN_DXDYDZ\">\n", in);
+        fprintf(fp, "%s\t\t\t<DataItem Format=\"XML\" Dimensions=\"3\">\n", in);
+        /* getSBRealBounds (functions.c:153-173): lower corner of the subblock */
+        const double rx = 0.0 + (cx * nx + 0.5) * c->cell_size - c->cell_size / 2.0;
+        const double ry = 0.0 + (cy * ny + 0.5) * c->cell_size - c->cell_size / 2.0;
+        const double rz = 0.0 + (cz * nz + 0.5) * c->cell_size - c->cell_size / 2.0;
+        fprintf(fp, "%s\t\t\t\t%e %e %e\n", in, rz, ry, rx);
+        fprintf(fp, "%s\t\t\t</DataItem>\n", in);
+        fprintf(fp, "%s\t\t\t<DataItem Format=\"XML\" Dimensions=\"3\">\n", in);
+        fprintf(fp, "%s\t\t\t\t%e %e %e\n", in, c->cell_size, c->cell_size, c->cell_size);
+        fprintf(fp, "%s\t\t\t</DataItem>\n", in);
+        fprintf(fp, "%s\t\t</Geometry>\n", in);
+        /* the reference writes Precision="8" (sizeof(double)) for its float data */
+        fprintf(fp, "%s\t\t<Attribute Name=\"Temperature\" Center=\"Cell\">\n", in);
+        fprintf(fp, "%s\t\t\t<DataItem Format=\"HDF\" NumberType=\"Float\" Precision=\"%lu\" Dimensions=\"%u %u %u\">\n",
+                in, (unsigned long) sizeof(double), nz, ny, nx);
+        fprintf(fp, "%s\t\t\t\tSERIAL:%s:%s/Temperature\n", in, file, sub);
+        fprintf(fp, "%s\t\t\t</DataItem>\n", in);
+        fprintf(fp, "%s\t\t</Attribute>\n", in);
+        fprintf(fp, "%s\t\t<Attribute Name=\"Solute Composition\" Center=\"Cell\">\n", in);
+        fprintf(fp, "%s\t\t\t<DataItem Format=\"HDF\" NumberType=\"Float\" Precision=\"%lu\" Dimensions=\"%u %u %u\">\n",
+                in, (unsigned long) sizeof(double), nz, ny, nx);
+        fprintf(fp, "%s\t\t\t\tSERIAL:%s:%s/CE\n", in, file, sub);
+        fprintf(fp, "%s\t\t\t</DataItem>\n", in);
+        fprintf(fp, "%s\t\t</Attribute>\n", in);
+        fprintf(fp, "%s\t\t<Attribute Name=\"Grain\" Center=\"Cell\">\n", in);
+        if (c->randomize_grains)
+        {
+            fprintf(fp, "%s\t\t\t<DataItem ItemType=\"Function\" NumberType=\"UInt\" Dimensions=\"%u %u %u\" "
+                        "Function=\"$0[$1]\">\n", in, nz, ny, nx);
+            fprintf(fp, "%s\t\t\t\t<DataItem Reference=\"XML\" Dimensions=\"%u %u %u\">\n", in, nz, ny, nx);
+            fprintf(fp, "%s\t\t\t\t\t/Xdmf/Domain/DataItem[@Name=\"Grain Renumber\"]\n", in);
+            fprintf(fp, "%s\t\t\t\t</DataItem>\n", in);
+        }
+        fprintf(fp, "%s\t\t\t\t<DataItem Format=\"HDF\" NumberType=\"UInt\" Dimensions=\"%u %u %u\">\n", in, nz, ny,
+                nx);
+        fprintf(fp, "%s\t\t\t\t\tSERIAL:%s:%s/Grain\n", in, file, sub);
+        fprintf(fp, "%s\t\t\t\t</DataItem>\n", in);
+        if (c->randomize_grains)
+            fprintf(fp, "%s\t\t\t</DataItem>\n", in);
+        fprintf(fp, "%s\t\t</Attribute>\n", in);
+        fprintf(fp, "%s\t\t<Attribute Name=\"Grain Angle\" Center=\"Cell\" AttributeType=\"Vector\">\n", in);
+        fprintf(fp, "%s\t\t\t<DataItem ItemType=\"Function\" NumberType=\"Float\" Dimensions=\"%u %u %u 3\" "
+                    "Function=\"$1[$0], $2[$0], $3[$0]\">\n", in, nz, ny, nx);
+        fprintf(fp, "%s\t\t\t\t<DataItem Format=\"HDF\" NumberType=\"UInt\" Dimensions=\"%u %u %u\">\n", in, nz, ny,
+                nx);
+        fprintf(fp, "%s\t\t\t\t\tSERIAL:%s:%s/Grain\n", in, file, sub);
+        fprintf(fp, "%s\t\t\t\t</DataItem>\n", in);
+        const char lbl[3] = { 'X', 'Y', 'Z' };
+        for (int d = 0; d < 3; d++)
+        {
+            fprintf(fp, "%s\t\t\t\t<DataItem Reference=\"XML\" Dimensions=\"%u %u %u\">\n", in, nz, ny, nx);
+            fprintf(fp, "%s\t\t\t\t\t/Xdmf/Domain/DataItem[@Name=\"Grain Angle %c\"]\n", in, lbl[d]);
+            fprintf(fp, "%s\t\t\t\t</DataItem>\n", in);
+        }
+        fprintf(fp, "%s\t\t\t</DataItem>\n", in);
+        fprintf(fp, "%s\t\t</Attribute>\n", in);
+        fprintf(fp, "%s\t\t<Attribute Name=\"Fraction Solid\" Center=\"Cell\">\n", in);
+        fprintf(fp, "%s\t\t\t<DataItem Format=\"HDF\" NumberType=\"Float\" Precision=\"%lu\" Dimensions=\"%u %u %u\">\n",
+                in, (unsigned long) sizeof(double), nz, ny, nx);
+        fprintf(fp, "%s\t\t\t\tSERIAL:%s:%s/FracSolid\n", in, file, sub);
+        fprintf(fp, "%s\t\t\t</DataItem>\n", in);
+        fprintf(fp, "%s\t\t</Attribute>\n", in);
+        fprintf(fp, "%s\t</Grid>\n", in);
+    }
+    fprintf(fp, "%s</Grid>\n", in);
+}
+
+/* writeGlobalData (io_xdmf.c:377-405) */
+static void
+write_global_data(mmca_xdmf *x, FILE *fp, const char *in, int ng)
+{
+    const char *b = local_base(x->cfg);
+    if (x->cfg->randomize_grains)
+    {
+        fprintf(fp, "%s<DataItem Name=\"Grain Renumber\" Format=\"HDF\" Dimensions=\"%d\" NumberType=\"UInt\">\n", in,
+                ng);
+        fprintf(fp, "%s\tSERIAL:%s_gbl.h5:/GrainRenumber", in, b);
+        fprintf(fp, "\n%s</DataItem>\n", in);
+    }
+    const char lbl[3] = { 'X', 'Y', 'Z' };
+    for (int d = 0; d < 3; d++)
+    {
+        fprintf(fp, "%s<DataItem Name=\"Grain Angle %c\" Format=\"HDF\" Dimensions=\"%d\" NumberType=\"Float\" "
+                    "Precision=\"8\">\n", in, lbl[d], ng);
+        fprintf(fp, "%s\tSERIAL:%s_gbl.h5:/GrainAngle%c\n", in, b, lbl[d]);
+        fprintf(fp, "%s</DataItem>\n", in);
+    }
+}
+
+/* writeXDMFTimeSeries (io_xdmf.c:307-339, 482-498): the entry is written,
+ * then the closing tags, and the file position is put back before them so
+ * that the next entry replaces them */
+static int
+write_time_series(mmca_xdmf *x, uint64_t ts, int ng)
+{
+    if (!x->ts_fp)
+    {
+        char fn[600];
+        snprintf(fn, sizeof(fn), "%s_ts.xmf", x->cfg->basefilename);
+        x->ts_fp = fopen(fn, "w");
+        if (!x->ts_fp)
+        {
+            fprintf(stderr, "Unable to open file %s for writing.\n", fn);
+            return -1;
+        }
+        FILE *f = x->ts_fp;
+        fprintf(f, "<?xml version=\"1.0\" ?>\n");
+        fprintf(f, "<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n");
+        fprintf(f, "<Xdmf xmlns:xi=\"http://www.w3.org/2001/XInclude\" Version=\"2.0\">\n");
+        fprintf(f, "\t<Domain>\n");
+        fprintf(f, "\t\t<Information Name=\"Description\">\n");
+        fprintf(f, "\t\t\tSimulation time series data\n");
+        fprintf(f, "\t\t</Information>\n");
+        fprintf(f, "\t\t<Information Name=\"Simulation Date\">\n");
+        time_t now = time(NULL);
+        fprintf(f, "\t\t\t%s", ctime(&now));
+        fprintf(f, "\t\t</Information>\n");
+        fprintf(f, "\t\t<Grid Name=\"Time Series\" GridType=\"Collection\" CollectionType=\"Temporal\">\n");
+    }
+    FILE *f = x->ts_fp;
+    write_grids(x, f, ts, "\t\t\t");
+    fpos_t pos;
+    fgetpos(f, &pos);
+    fprintf(f, "\t\t</Grid>\n");
+    write_global_data(x, f, "\t\t", ng);
+    fprintf(f, "\t</Domain>\n");
+    fprintf(f, "</Xdmf>\n");
+    fflush(f);
+    fsetpos(f, &pos);
+    return 0;
+}
+
+/* dataset creation as io_xdmf.c:416-421 and 580-595: chunked, never filled,
+ * shuffled, deflate level 9 */
+static hid_t
+chunked(int rank, const hsize_t *dims)
+{
+    hid_t pl = H5Pcreate(H5P_DATASET_CREATE);
+    H5Pset_chunk(pl, rank, dims);
+    H5Pset_fill_time(pl, H5D_FILL_TIME_NEVER);
+    H5Pset_shuffle(pl);
+    H5Pset_deflate(pl, 9);
+    return pl;
+}
+
+/* writeGlobalHeavyData (io_xdmf.c:407-468) */
+static int
+write_global_heavy(mmca_xdmf *x, mmca_ctx *dom0, int ng)
+{
+    char fn[600];
+    snprintf(fn, sizeof(fn), "%s_gbl.h5", x->cfg->basefilename);
+    hid_t f = H5Fcreate(fn, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0)
+    {
+        fprintf(stderr, "Unable to open HDF5 file %s\n", fn);
+        return -1;
+    }
+    hsize_t n = (hsize_t) ng;
+    hid_t space = H5Screate_simple(1, &n, NULL);
+    hid_t pl = chunked(1, &n);
+    if (x->cfg->randomize_grains)
+    {
+        if (renumber_array(x, dom0, ng))
+            return -1;
+        hid_t ds = H5Dcreate(f, "/GrainRenumber", H5T_NATIVE_UINT, space, H5P_DEFAULT, pl, H5P_DEFAULT);
+        H5Dwrite(ds, sizeof(size_t) == 8 ? H5T_NATIVE_UINT64 : H5T_NATIVE_UINT32, H5S_ALL, H5S_ALL, H5P_DEFAULT,
+                 x->renumber);
+        H5Dclose(ds);
+    }
+    mmca_grain *g = calloc((size_t) ng, sizeof(mmca_grain));
+    float *a = malloc(sizeof(float) * (size_t) ng);
+    if (!g || !a || mmca_get_grains(dom0, g, ng, NULL) != MMCA_OK)
+        return -1;
+    const char lbl[3] = { 'X', 'Y', 'Z' };
+    for (int d = 0; d < 3; d++)
+    {
+        for (int i = 0; i < ng; i++)
+            a[i] = (float) g[i].rotang[d];
+        char name[32];
+        snprintf(name, sizeof(name), "/GrainAngle%c", lbl[d]);
+        hid_t ds = H5Dcreate(f, name, H5T_NATIVE_FLOAT, space, H5P_DEFAULT, pl, H5P_DEFAULT);
+        H5Dwrite(ds, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, a);
+        H5Dclose(ds);
+    }
+    free(g);
+    free(a);
+    H5Pclose(pl);
+    H5Sclose(space);
+    H5Fclose(f);
+    return 0;
+}
+
+/* xdmf_writeSubblocks (io_xdmf.c:559-673) for rank r */
+static int
+write_subblock(mmca_xdmf *x, mmca_ctx *dom, const mmca_params *p, int r, uint64_t ts)
+{
+    const size_t P = (size_t) (p->nx + 2) * (p->ny + 2) * (p->nz + 2);
+    float *fs = malloc(sizeof(float) * P), *ce = malloc(sizeof(float) * P), *t = malloc(sizeof(float) * P);
+    int32_t *gr = malloc(sizeof(int32_t) * P);
+    if (!fs || !ce || !t || !gr)
+    {
+        fprintf(stderr, "out of host memory\n");
+        return -1;
+    }
+    if (mmca_io_fields(dom, fs, ce, t, gr) != MMCA_OK)
+        return -1;
+    char fn[700];
+    snprintf(fn, sizeof(fn), "%s_r%d/%05lu.h5", x->cfg->basefilename, r, (unsigned long) ts);
+    hid_t f = H5Fcreate(fn, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0)
+    {
+        fprintf(stderr, "Unable to open HDF5 file %s\n", fn);
+        return -1;
+    }
+    const hsize_t start[3] = { 1, 1, 1 };
+    const hsize_t padded[3] = { (hsize_t) p->nz + 2, (hsize_t) p->ny + 2, (hsize_t) p->nx + 2 };
+    const hsize_t count[3] = { (hsize_t) p->nz, (hsize_t) p->ny, (hsize_t) p->nx };
+    hid_t space = H5Screate_simple(3, count, NULL);
+    hid_t link = H5Pcreate(H5P_LINK_CREATE);
+    H5Pset_create_intermediate_group(link, 1);
+    hid_t pl = chunked(3, count);
+    hid_t mem = H5Screate_simple(3, padded, NULL);
+    H5Sselect_hyperslab(mem, H5S_SELECT_SET, start, NULL, count, NULL);
+    int cx, cy, cz;
+    rank_coords(x, r, &cx, &cy, &cz);
+    const char *names[4] = { "Temperature", "CE", "Grain", "FracSolid" };
+    const void *data[4] = { t, ce, gr, fs };
+    for (int v = 0; v < 4; v++)
+    {
+        char path[128];
+        snprintf(path, sizeof(path), "/SubBlock_%dx%dx%d/%s", cx, cy, cz, names[v]);
+        const hid_t type = (v == 2) ? H5T_NATIVE_INT : H5T_NATIVE_FLOAT;
+        hid_t ds = H5Dcreate(f, path, type, space, link, pl, H5P_DEFAULT);
+        H5Dwrite(ds, type, mem, H5S_ALL, H5P_DEFAULT, data[v]);
+        H5Dclose(ds);
+    }
+    H5Fclose(f);
+    H5Pclose(pl);
+    H5Pclose(link);
+    H5Sclose(space);
+    H5Sclose(mem);
+    free(fs);
+    free(ce);
+    free(t);
+    free(gr);
+    return 0;
+}
+
+int
+mmca_xdmf_write(mmca_xdmf *x, mmca_ctx *const *doms, const mmca_params *ps, uint64_t ts)
+{
+    int32_t ng = 0;
+    if (mmca_get_grains(doms[0], NULL, 0, &ng) != MMCA_OK)
+        return -1;
+    /* writeMain on rank 0 (the time-series entry, then the global data),
+     * then every rank's subblock file */
+    if (write_time_series(x, ts, ng) || write_global_heavy(x, doms[0], ng))
+        return -1;
+    for (int r = 0; r < x->nranks; r++)
+        if (write_subblock(x, doms[r], &ps[r], r, ts))
+            return -1;
+    return 0;
+}
+
+void
+mmca_xdmf_close(mmca_xdmf *x)
+{
+    if (!x)
+        return;
+    if (x->ts_fp)
+        fclose(x->ts_fp);
+    free(x->renumber);
+    free(x);
+}

@@ -1,0 +1,32 @@
+/*
+ * XDMF + HDF5 output of the C host driver, in the layout of the
+ * reference's writer (io_xdmf.c): per rank and output step
+ * <BaseFileName>_r<rank>/<step %05lu>.h5 with /SubBlock_XxYxZ/{Temperature,
+ * CE, Grain, FracSolid}, the global file <BaseFileName>_gbl.h5 with the grain
+ * renumbering and angles, and the time series <BaseFileName>_ts.xmf.
+ */
+#ifndef MMCA_XDMF_H
+#define MMCA_XDMF_H
+
+#include <stdint.h>
+
+#include "mmca.h"
+#include "mmca_ctrl.h"
+
+typedef struct mmca_xdmf mmca_xdmf;
+
+/* xdmf_prepareIO (io_xdmf.c:533-541): one output directory per rank, for
+ * nsub[0]*nsub[1]*nsub[2] ranks of sbdim cells each (rank r at Cart
+ * coordinates x = r % nsub[0], y = r / nsub[0] % nsub[1], z = the rest).
+ * Returns NULL with a message on stderr on failure. */
+mmca_xdmf *mmca_xdmf_open(const mmca_run_cfg *cfg, const int nsub[3], const int sbdim[3]);
+
+/* writeData (loop.c:116-127) at timestep ts: the ranks' fields through
+ * mmca_io_fields, then rank 0's time-series entry and global data (its
+ * grain renumbering draws from rank 0's random stream, mmca_rand), then
+ * each rank's HDF5 file.  doms[r] / ps[r]: the subblock of rank r. */
+int mmca_xdmf_write(mmca_xdmf *x, mmca_ctx *const *doms, const mmca_params *ps, uint64_t ts);
+
+void mmca_xdmf_close(mmca_xdmf *x);
+
+#endif

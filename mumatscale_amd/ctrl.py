@@ -57,6 +57,9 @@ class Config:
         self.dt = float(C.c_float(0.1).value)
         self.screenpfreq = 1
         self.data_write_freq = 0
+        self.data_write_start = 0
+        self.data_format = "NONE"        # DataFormat: NONE, VTK or XDMF
+        self.randomize_grains = 1
         self.bulk_density = 0.0
         self.bulk_mean = 0.0
         self.bulk_sigma = 0.0
@@ -178,6 +181,15 @@ class Config:
             self.screenpfreq = i()[0]
         elif key == "datawritefreq":
             self.data_write_freq = i()[0]
+        elif key == "datawritestart":
+            self.data_write_start = i()[0]
+        elif key == "dataformat":
+            fmt = rest.split()[0].upper()
+            if fmt not in ("VTK", "XDMF"):
+                raise ValueError("Unsupported data file type: %s" % rest.split()[0])
+            self.data_format = fmt
+        elif key == "randomizegrains":
+            self.randomize_grains = i()[0]
         elif key in ("geofilename", "matfilename", "alloypropsfile0"):
             inc = rest.split()[0]
             self.read(inc if os.path.isabs(inc) else os.path.join(base, inc))
@@ -326,6 +338,8 @@ def write_ctrl(cfg, path, nsub=None):
         "FinishTime %s" % r(cfg.finish_time),
         "FsFinish %s" % r(cfg.fs_finish),
         "ScreenPFreq %d" % cfg.screenpfreq,
+        "DataWriteFreq %d" % cfg.data_write_freq,
+        "RandomizeGrains %d" % cfg.randomize_grains,
         "LiquidusTemp %s" % r(cfg.liquidus_temp),
         "SolidusTemp %s" % r(cfg.solidus_temp),
         "Cinit %s" % r(cfg.c_init),
@@ -362,6 +376,11 @@ def write_ctrl(cfg, path, nsub=None):
         "NoOfLayers %d" % cfg.am_layers,
         "NoOfTracks %d" % cfg.am_tracks,
     ]
+    if cfg.data_format != "NONE":
+        lines.append("DataFormat %s" % cfg.data_format)
+    if cfg.data_write_start:
+        # the reference also takes this value as its checkpoint frequency
+        lines.append("DataWriteStart %d" % cfg.data_write_start)
     for f in cfg.fixed:
         lines.append("FixedNuc %s %s %s %s %s %s %s" % tuple(r(v) for v in f))
     for v in cfg.volumes:

@@ -20,6 +20,16 @@ and keeps, per rank and step, the hook's interior hashes:
                            interface list and of the (shared) grain table
   <case>/expected.json     as above
 
+Control files in tests/golden/cases_xdmf/ ask for the reference's XDMF
+output (DataFormat XDMF); they run on their NSubblocks ranks, with the hook's
+per-rank hashes left out, and keep a digest of the files written
+(tests/xdmf_digest.py):
+
+  <case>/digest.json.gz    the time series text (without its date line), the
+                           structure of every HDF5 file and a hash of each
+                           dataset
+  <case>/expected.json     as above
+
 Run in the container that holds the reference sources:
     python tests/golden/make_golden.py
 """
@@ -33,10 +43,14 @@ import tempfile
 
 import numpy as np
 
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import xdmf_digest  # noqa: E402
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 CASES = os.path.join(HERE, "cases")
 CASES_MR = os.path.join(HERE, "cases_mr")
+CASES_XDMF = os.path.join(HERE, "cases_xdmf")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 REF_BIN = os.path.join(REPO, "oracle", "_ref", "mumatscale_ref")
 
@@ -118,6 +132,44 @@ def run_case_mr(name, out_dir):
         return exp
 
 
+def base_name(ctrl):
+    for line in open(ctrl):
+        w = line.split()
+        if w and w[0] == "BaseFileName":
+            return w[1]
+    raise ValueError(ctrl)
+
+
+def run_case_xdmf(name, out_dir):
+    ctrl = os.path.join(CASES_XDMF, name + ".in")
+    n, base = nranks(ctrl), base_name(ctrl)
+    with tempfile.TemporaryDirectory() as tmp:
+        if os.path.dirname(base):
+            os.makedirs(os.path.join(tmp, os.path.dirname(base)))
+        env = dict(os.environ, MMCA_REF_DUMP_DIR=os.path.join(tmp, "hook"), OMP_NUM_THREADS="1")
+        os.makedirs(env["MMCA_REF_DUMP_DIR"])
+        proc = subprocess.run([MPIEXEC, "-hosts", "127.0.0.1", "-n", str(n), REF_BIN, "-c", ctrl], cwd=tmp,
+                              env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=3600)
+        text = proc.stdout.decode(errors="replace")
+        digest = xdmf_digest.digest_output(tmp, base)
+        os.makedirs(out_dir, exist_ok=True)
+        with gzip.open(os.path.join(out_dir, "digest.json.gz"), "wt") as fh:
+            json.dump(digest, fh, indent=0, sort_keys=True)
+        avg = [l for l in text.splitlines() if "Average" in l]
+        err = [l.strip() for l in text.splitlines() if "ERROR" in l]
+        steps = [int(os.path.basename(f)[:-3]) for f in digest["files"] if "_r0/" in f]
+        exp = {
+            "returncode": proc.returncode,
+            "ranks": n,
+            "last_step": max(steps),
+            "error_line": err[0] if err else None,
+            "average_line": avg[0].strip() if avg else None,
+        }
+        with open(os.path.join(out_dir, "expected.json"), "w") as fh:
+            json.dump(exp, fh, indent=1)
+        return exp
+
+
 def main():
     subprocess.check_call(["make", "-s", "-f", os.path.join(REPO, "oracle", "ref.mk")], cwd=REPO)
     names = sorted(f[:-3] for f in os.listdir(CASES) if f.endswith(".in"))
@@ -137,6 +189,15 @@ def main():
         if os.path.isdir(out):
             shutil.rmtree(out)
         exp = run_case_mr(n, out)
+        print(n, exp)
+    names = sorted(f[:-3] for f in os.listdir(CASES_XDMF) if f.endswith(".in"))
+    if len(sys.argv) > 1:
+        names = [n for n in names if n in sys.argv[1:]]
+    for n in names:
+        out = os.path.join(HERE, n)
+        if os.path.isdir(out):
+            shutil.rmtree(out)
+        exp = run_case_xdmf(n, out)
         print(n, exp)
 
 

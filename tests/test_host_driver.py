@@ -16,6 +16,8 @@ from mumatscale_amd.ctrl import read_ctrl
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RUN = os.path.join(ROOT, "mumatscale_amd", "mmca_run")
 CTRL_FILES = sorted(glob.glob(os.path.join(GOLDEN, "cases", "*.in")) +
+                    glob.glob(os.path.join(GOLDEN, "cases_mr", "*.in")) +
+                    glob.glob(os.path.join(GOLDEN, "cases_xdmf", "*.in")) +
                     glob.glob(os.path.join(GOLDEN, "reference_inputs", "*.in")))
 
 
@@ -45,6 +47,10 @@ def test_ctrl_reader_matches_python(path):
     assert float(d["finish_time"][0][0]) == cfg.finish_time
     assert float(d["fs_finish"][0][0]) == cfg.fs_finish
     assert int(d["screenpfreq"][0][0]) == cfg.screenpfreq
+    assert int(d["data_write_freq"][0][0]) == cfg.data_write_freq
+    assert int(d["data_write_start"][0][0]) == cfg.data_write_start
+    assert d["data_format"][0][0] == cfg.data_format
+    assert int(d["randomize_grains"][0][0]) == cfg.randomize_grains
     for k in ("nx", "ny", "nz", "temp_model", "am_layers", "am_tracks", "gnsbx", "max_total_grains"):
         assert int(d[k][0][0]) == getattr(p, k), k
     for k in ("cell_size", "dt", "courant_limit", "c_init", "liquidus_temp", "solidus_temp", "part_coef",

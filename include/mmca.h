@@ -305,9 +305,11 @@ int  mmca_batch_step(mmca_batch *batch, int nsteps, int want_sums, mmca_report *
  * of gr and fs after partitioning (ExchangeFacesForVar), and activates new
  * grains across the ranks in rank order into one shared grain table
  * (activateNewGrains, grain.c:841-916), while every rank keeps its own
- * interface list and dc_tmp slots.  Padded faces only (FaceCtrl 1 on every
- * axis split in more than one subblock).  Step the batch only through
- * mmca_batch_step afterwards.
+ * interface list and dc_tmp slots.  A wrapped face (FaceCtrl 0) of the
+ * global domain exchanges with the subblock at the other end of the axis
+ * (sbID_from_sbcoords, functions.c:272-311); with one subblock along that
+ * axis it keeps its halo, as one block does.  All domains must have the same
+ * FaceCtrl.  Step the batch only through mmca_batch_step afterwards.
  */
 int  mmca_batch_couple(mmca_batch *batch, const int32_t nsub[3]);
 int  mmca_batch_set_tuning(mmca_batch *batch, int key, int value);

@@ -1535,8 +1535,8 @@ int mmca_batch_couple(mmca_batch *bt, const int32_t nsub[3])
             const int na = a == 0 ? p.nx : a == 1 ? p.ny : p.nz;
             if (na != n3[a] || p.sb_coords[a] != rc[a] || p.gdim[a] != nsub[a] * n3[a])
                 return MMCA_ERR_ARG;
-            if (!p.pad_bndy[a] && nsub[a] > 1)
-                return MMCA_ERR_ARG;     // periodic faces across subblocks are not emulated
+            if (!p.pad_bndy[a] != !p0.pad_bndy[a])
+                return MMCA_ERR_ARG;     // one FaceCtrl for the whole domain
         }
         if (p.precision != p0.precision || p.max_total_grains != p0.max_total_grains ||
             c->hc.num_grains != 1 || c->hc.timestep != bt->dom[0]->hc.timestep || !c->nucleation_ready)

@@ -461,7 +461,16 @@ k_halo_exchange(View V, int phase, int cb)
         cn[a] = c.off[a] / n3[a];
     cn[ax] += hi ? 1 : -1;
     if (cn[ax] < 0 || cn[ax] >= c.nsub[ax])
-        return;                                   // a face of the global domain
+    {
+        // a face of the global domain: a mold wall, or a wrapped face whose
+        // neighbour is the subblock at the other end of the axis
+        // (sbID_from_sbcoords, functions.c:272-311); with one subblock along
+        // the axis that is the subblock itself, which exchanges nothing
+        // (face_util.c:128, 217) and keeps its halo
+        if ((hi ? c.pad_hi[ax] : c.pad_lo[ax]) || c.nsub[ax] < 2)
+            return;
+        cn[ax] = (cn[ax] + c.nsub[ax]) % c.nsub[ax];
+    }
     const int r = (cn[2] * c.nsub[1] + cn[1]) * c.nsub[0] + cn[0];   // MPI Cart rank (z, y, x)
     const DevBufs nb = V.dom[r].b;
     // the two axes along the face

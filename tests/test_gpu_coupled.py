@@ -87,7 +87,7 @@ def test_coupled_batch_matches_reference_ranks(case):
 
 
 def test_coupling_rejects_what_it_cannot_emulate():
-    """Wrong layouts and periodic faces across subblocks are refused; a
+    """Wrong layouts and ranks with different FaceCtrl are refused; a
     coupled rank cannot be stepped, restarted or given grains on its own."""
     cfg, _, _ = load_case_mr("m1_directional_z2")
     params = [cfg.params(whole_domain=False, sb_coords=rank_coords(cfg, r)) for r in range(2)]
@@ -97,10 +97,9 @@ def test_coupling_rejects_what_it_cannot_emulate():
             bt.couple([2, 1, 1])                  # sb_coords do not match an x split
         with pytest.raises(abi.MmcaError):
             bt.couple([1, 1, 3])                  # three ranks for two domains
-    wrap = [cfg.params(whole_domain=False, sb_coords=rank_coords(cfg, r)) for r in range(2)]
-    for p in wrap:
-        p.pad_bndy[2] = 0
-    with Batch(wrap, nucs) as bt:
+    mixed = [cfg.params(whole_domain=False, sb_coords=rank_coords(cfg, r)) for r in range(2)]
+    mixed[1].pad_bndy[2] = 0                      # FaceCtrl differs between the ranks
+    with Batch(mixed, nucs) as bt:
         with pytest.raises(abi.MmcaError):
             bt.couple(cfg.nsub)
     with Batch(params, nucs) as bt:

@@ -312,6 +312,17 @@ int  mmca_batch_step(mmca_batch *batch, int nsteps, int want_sums, mmca_report *
  * FaceCtrl.  Step the batch only through mmca_batch_step afterwards.
  */
 int  mmca_batch_couple(mmca_batch *batch, const int32_t nsub[3]);
+/*
+ * Restart image of a whole batch, as mmca_save_state / mmca_load_state are
+ * for one block (checkpoint.c:955-1238 replaced; the reference's checkpoint
+ * stores neither the random streams nor dc_tmp).  For a coupled batch the
+ * image holds every rank's fields, list, dc_tmp slots and stream, and the
+ * shared grain table once.  Load into a batch created with the same
+ * parameters and nucleation set-up and, for a coupled image, coupled with
+ * the same nsub; the run then continues bit for bit.
+ */
+int  mmca_batch_save_state(mmca_batch *batch, void *buf, int64_t capacity, int64_t *size);
+int  mmca_batch_load_state(mmca_batch *batch, const void *buf, int64_t size);
 int  mmca_batch_set_tuning(mmca_batch *batch, int key, int value);
 int  mmca_batch_set_kernel_timing(mmca_batch *batch, int enable);
 int  mmca_batch_kernel_time_ms(mmca_batch *batch, double *avg_ms, int64_t *launches);

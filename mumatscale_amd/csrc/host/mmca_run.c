@@ -14,6 +14,11 @@
  *                                 and after the last step (mmca_xdmf.h; the
  *                                 writer lives in libmmca_xdmf.so, loaded
  *                                 only when such output is asked for)
+ *   writeCheckpoint loop.c:129-145 every checkpointfreq steps (DataWriteStart,
+ *                                 as the reference reads it, or
+ *                                 --checkpoint-freq): the restart image of
+ *                                 the whole run in cp_main.mmca; --restart
+ *                                 continues from it bit for bit
  *
  * With NSubblocks x y z the domain runs as the reference runs it on x*y*z
  * MPI ranks: one subblock per rank, each with its own random stream, halos
@@ -23,7 +28,8 @@
  * screen or finish checks are issued as one step call.
  *
  * Usage: mmca_run <ctrl-file> [--device N] [--max-steps N] [--fp32]
- *                 [--no-ce-check] [--one-block] [--dump-ctrl]
+ *                 [--no-ce-check] [--one-block] [--checkpoint-freq N]
+ *                 [--restart] [--dump-ctrl]
  */
 #include <dlfcn.h>
 #include <limits.h>
@@ -45,6 +51,8 @@ typedef struct xdmf_lib {
     mmca_xdmf *(*open)(const mmca_run_cfg *, const int *, const int *);
     int (*write)(mmca_xdmf *, mmca_ctx *const *, const mmca_params *, uint64_t);
     void (*close)(mmca_xdmf *);
+    int64_t (*get_renumber)(const mmca_xdmf *, uint64_t *, int64_t);
+    int (*set_renumber)(mmca_xdmf *, const uint64_t *, int64_t);
     mmca_xdmf *x;
 } xdmf_lib;
 
@@ -67,7 +75,9 @@ xdmf_load(xdmf_lib *w)
     *(void **) &w->open = dlsym(w->so, "mmca_xdmf_open");
     *(void **) &w->write = dlsym(w->so, "mmca_xdmf_write");
     *(void **) &w->close = dlsym(w->so, "mmca_xdmf_close");
-    if (!w->open || !w->write || !w->close)
+    *(void **) &w->get_renumber = dlsym(w->so, "mmca_xdmf_get_renumber");
+    *(void **) &w->set_renumber = dlsym(w->so, "mmca_xdmf_set_renumber");
+    if (!w->open || !w->write || !w->close || !w->get_renumber || !w->set_renumber)
     {
         fprintf(stderr, "%s lacks the writer's entry points\n", path);
         return -1;
@@ -100,6 +110,88 @@ finish_step(uint64_t t, double dt, double finish_time)
     while (s * dt < finish_time)
         s++;
     return s;
+}
+
+/*
+ * writeCheckpoint / mainRestart (loop.c:129-145, checkpoint.c:867-1238) with
+ * this build's restart image: the whole run (every rank, its random stream
+ * and dc_tmp slots, the grain table) plus the XDMF writer's grain
+ * renumbering, so that a restarted run continues bit for bit and writes the
+ * same files.  Written as cp_main_<step>.mmca and renamed to cp_main.mmca,
+ * as clean_checkpoint (checkpoint.c:165-177) renames its file.
+ *
+ *   "MMCARUN1", int64 image bytes, int64 renumber entries, image, entries
+ */
+static const char CP_MAGIC[8] = { 'M', 'M', 'C', 'A', 'R', 'U', 'N', '1' };
+
+static int
+save_checkpoint(mmca_batch *bt, mmca_ctx *ctx, const xdmf_lib *out, uint64_t t)
+{
+    int64_t n = 0;
+    if (check(bt ? mmca_batch_save_state(bt, NULL, 0, &n) : mmca_save_state(ctx, NULL, 0, &n), "save state"))
+        return -1;
+    const int64_t nren = out->x ? out->get_renumber(out->x, NULL, 0) : 0;
+    char *buf = malloc((size_t) n);
+    uint64_t *ren = malloc(sizeof(uint64_t) * (size_t) (nren > 0 ? nren : 1));
+    if (!buf || !ren)
+    {
+        fprintf(stderr, "out of host memory\n");
+        return -1;
+    }
+    if (check(bt ? mmca_batch_save_state(bt, buf, n, &n) : mmca_save_state(ctx, buf, n, &n), "save state"))
+        return -1;
+    if (out->x)
+        out->get_renumber(out->x, ren, nren);
+    char tmp[64];
+    snprintf(tmp, sizeof(tmp), "cp_main_%05lu.mmca", (unsigned long) t);
+    FILE *f = fopen(tmp, "wb");
+    int ok = f && fwrite(CP_MAGIC, 8, 1, f) == 1 && fwrite(&n, 8, 1, f) == 1 && fwrite(&nren, 8, 1, f) == 1 &&
+             fwrite(buf, 1, (size_t) n, f) == (size_t) n &&
+             (nren == 0 || fwrite(ren, sizeof(uint64_t), (size_t) nren, f) == (size_t) nren);
+    if (f && fclose(f) != 0)
+        ok = 0;
+    free(buf);
+    free(ren);
+    if (!ok || rename(tmp, "cp_main.mmca") != 0)
+    {
+        fprintf(stderr, "Unable to write checkpoint %s\n", tmp);
+        return -1;
+    }
+    printf("Checkpoint at timestep %lu \n", (unsigned long) t);
+    return 0;
+}
+
+static int
+load_checkpoint(mmca_batch *bt, mmca_ctx *ctx, xdmf_lib *out)
+{
+    FILE *f = fopen("cp_main.mmca", "rb");
+    char magic[8];
+    int64_t n = 0, nren = 0;
+    if (!f || fread(magic, 8, 1, f) != 1 || memcmp(magic, CP_MAGIC, 8) != 0 || fread(&n, 8, 1, f) != 1 ||
+        fread(&nren, 8, 1, f) != 1 || n <= 0 || nren < 0)
+    {
+        fprintf(stderr, "cp_main.mmca is missing or not a checkpoint of this driver\n");
+        if (f)
+            fclose(f);
+        return -1;
+    }
+    char *buf = malloc((size_t) n);
+    uint64_t *ren = malloc(sizeof(uint64_t) * (size_t) (nren > 0 ? nren : 1));
+    int ok = buf && ren && fread(buf, 1, (size_t) n, f) == (size_t) n &&
+             (nren == 0 || fread(ren, sizeof(uint64_t), (size_t) nren, f) == (size_t) nren);
+    fclose(f);
+    if (!ok)
+    {
+        fprintf(stderr, "cp_main.mmca is truncated\n");
+        return -1;
+    }
+    if (check(bt ? mmca_batch_load_state(bt, buf, n) : mmca_load_state(ctx, buf, n), "load state"))
+        return -1;
+    if (out->x && out->set_renumber(out->x, ren, nren))
+        return -1;
+    free(buf);
+    free(ren);
+    return 0;
 }
 
 static void
@@ -232,8 +324,8 @@ int
 main(int argc, char **argv)
 {
     const char *path = NULL;
-    int device = 0, fp32 = 0, ce_check = 1, dump = 0, one_block = 0;
-    long long max_steps = -1;
+    int device = 0, fp32 = 0, ce_check = 1, dump = 0, one_block = 0, restart = 0;
+    long long max_steps = -1, cp_freq = -1;
     for (int i = 1; i < argc; i++)
     {
         if (!strcmp(argv[i], "--device") && i + 1 < argc)
@@ -248,12 +340,16 @@ main(int argc, char **argv)
             one_block = 1;
         else if (!strcmp(argv[i], "--dump-ctrl"))
             dump = 1;
+        else if (!strcmp(argv[i], "--checkpoint-freq") && i + 1 < argc)
+            cp_freq = atoll(argv[++i]);
+        else if (!strcmp(argv[i], "--restart"))
+            restart = 1;
         else if (argv[i][0] != '-' && !path)
             path = argv[i];
         else
         {
             fprintf(stderr, "usage: %s <ctrl-file> [--device N] [--max-steps N] [--fp32] [--no-ce-check] "
-                            "[--one-block] [--dump-ctrl]\n", argv[0]);
+                            "[--one-block] [--checkpoint-freq N] [--restart] [--dump-ctrl]\n", argv[0]);
             return 2;
         }
     }
@@ -325,16 +421,24 @@ main(int argc, char **argv)
         if (!out.x)
             return 1;
     }
+    /* checkpointfreq: the reference reads DataWriteStart into it (read_ctrl.c:594-599) */
+    const long long checkpoint_freq = cp_freq >= 0 ? cp_freq : cfg.data_write_start;
+    uint64_t t = 0;
+    if (restart)
+    {
+        if (load_checkpoint(bt, doms[0], &out) || check(mmca_get_timestep(doms[0], &t), "mmca_get_timestep"))
+            return 1;
+        printf("Restarting from timestep %lu\n", (unsigned long) t);
+    }
     printf("Initial Subblocks assigned.  Starting calculations.\n");
     const double total_volume = (p.gdim[0] * p.gdim[1] * p.gdim[2]) * pow(cfg.cell_size, 3);
     const double cell_vol = pow(cfg.cell_size, 3.0);
     if (cfg.screenpfreq > 0)
         printf("%10g  %10g  %8.4lf%% %8d\n", wall_now() - t_start, 0.0, 0.0, 0);
-    uint64_t t = 0;
     double fs = 0.0;
-    const uint64_t fin = cfg.finish_time ? finish_step(1, cfg.dt, cfg.finish_time) : 0;
+    const uint64_t fin = cfg.finish_time ? finish_step(t + 1, cfg.dt, cfg.finish_time) : 0;
     int status = MMCA_OK;
-    if (write_freq > 0 && out.write(out.x, doms, ps, 0))
+    if (write_freq > 0 && !restart && out.write(out.x, doms, ps, 0))
         status = MMCA_ERR_ARG;
     while (status == MMCA_OK)
     {
@@ -350,6 +454,8 @@ main(int argc, char **argv)
             if (w < next)
                 next = w;
         }
+        if (checkpoint_freq > 0 && (t / checkpoint_freq + 1) * checkpoint_freq < next)
+            next = (t / checkpoint_freq + 1) * checkpoint_freq;
         if (fin && (fin > t ? fin : t + 1) < next)
             next = fin > t ? fin : t + 1;
         if (max_steps >= 0 && (uint64_t) max_steps < next)
@@ -383,6 +489,11 @@ main(int argc, char **argv)
         }
         if (write_freq > 0 && (long long) t >= cfg.data_write_start && t % write_freq == 0 &&
             out.write(out.x, doms, ps, t))
+        {
+            status = MMCA_ERR_ARG;
+            break;
+        }
+        if (checkpoint_freq > 0 && t % checkpoint_freq == 0 && save_checkpoint(bt, doms[0], &out, t))
         {
             status = MMCA_ERR_ARG;
             break;

@@ -404,3 +404,29 @@ mmca_xdmf_close(mmca_xdmf *x)
     free(x->renumber);
     free(x);
 }
+
+int64_t
+mmca_xdmf_get_renumber(const mmca_xdmf *x, uint64_t *out, int64_t max)
+{
+    const int64_t n = x->renumber ? (int64_t) x->renumber_sz : 0;
+    for (int64_t i = 0; i < n && i < max; i++)
+        out[i] = x->renumber[i];
+    return n;
+}
+
+int
+mmca_xdmf_set_renumber(mmca_xdmf *x, const uint64_t *in, int64_t n)
+{
+    free(x->renumber);
+    x->renumber = NULL;
+    x->renumber_sz = 0;
+    if (n <= 0)
+        return 0;
+    x->renumber = malloc(sizeof(size_t) * (size_t) n);
+    if (!x->renumber)
+        return -1;
+    for (int64_t i = 0; i < n; i++)
+        x->renumber[i] = (size_t) in[i];
+    x->renumber_sz = (size_t) n;
+    return 0;
+}

@@ -29,4 +29,10 @@ int mmca_xdmf_write(mmca_xdmf *x, mmca_ctx *const *doms, const mmca_params *ps, 
 
 void mmca_xdmf_close(mmca_xdmf *x);
 
+/* The grain renumbering kept between writes (getRenumberArray's static
+ * array), for restart files: get copies at most max entries into out and
+ * returns the array's length; set replaces it. */
+int64_t mmca_xdmf_get_renumber(const mmca_xdmf *x, uint64_t *out, int64_t max);
+int mmca_xdmf_set_renumber(mmca_xdmf *x, const uint64_t *in, int64_t n);
+
 #endif

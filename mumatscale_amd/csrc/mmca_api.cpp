@@ -1038,6 +1038,16 @@ struct StateHeader {
 
 constexpr char STATE_MAGIC[8] = { 'M', 'M', 'C', 'A', 'S', 'T', 'A', 'T' };
 
+// Restart image of a batch: this header, then the image of every domain in
+// order; a coupled batch stores its shared grain table once, in rank 0's.
+struct BatchStateHeader {
+    char magic[8];
+    int32_t version, ndom, coupled, cur;
+    int32_t nsub[3], pad_;
+};
+
+constexpr char BATCH_MAGIC[8] = { 'M', 'M', 'C', 'A', 'B', 'T', 'C', 'H' };
+
 // the stream right after the orientations consumed so far
 GlibcRand stream_at_consumption(const mmca_ctx *ctx)
 {
@@ -1090,7 +1100,12 @@ struct StateIO {
     }
 };
 
-int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc)
+// Which grain table an image carries: its own (one block), none (a coupled
+// rank after the first, whose table is rank 0's), or rank 0's of a coupled
+// batch, with the creating rank of every grain.
+enum TablePart { TABLE_OWN = 0, TABLE_SHARED = 1, TABLE_COUPLED_OWNER = 2 };
+
+int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc, TablePart table)
 {
     DevBufs &b = ctx->b;
     const size_t P = (size_t) h.P;
@@ -1121,10 +1136,14 @@ int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc)
     TRY(io.dev(b.dty, nslot * sizeof(double)));
     TRY(io.dev(b.dtz, nslot * sizeof(double)));
     const size_t ng = (size_t) hc.num_grains;
-    TRY(io.dev(b.g_rot, 9 * ng * sizeof(double)));
-    TRY(io.dev(b.g_cell, ng * sizeof(int)));
-    TRY(io.dev(b.g_step, ng * sizeof(unsigned long long)));
-    TRY(io.dev(b.g_src, ng * sizeof(int)));
+    if (table != TABLE_SHARED) {
+        TRY(io.dev(b.g_rot, 9 * ng * sizeof(double)));
+        TRY(io.dev(b.g_cell, ng * sizeof(int)));
+        TRY(io.dev(b.g_step, ng * sizeof(unsigned long long)));
+        TRY(io.dev(b.g_src, ng * sizeof(int)));
+    }
+    if (table == TABLE_COUPLED_OWNER)
+        TRY(io.dev(b.g_dom, ng * sizeof(int)));
     TRY(io.dev(b.site_ls, (size_t) h.nsites * sizeof(int)));
     if (io.load)
         ctx->rand_angles.assign((size_t) (3 * h.n_angles), 0.0);
@@ -1135,19 +1154,9 @@ int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc)
     return MMCA_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size)
+// Appends the image of one block (header and body) to io.
+int save_block(mmca_ctx *ctx, StateIO &io, TablePart table)
 {
-    if (!ctx || capacity < 0 || (buf && capacity == 0))
-        return MMCA_ERR_ARG;
-    if (!ctx->peers.empty())
-        return MMCA_ERR_STATE;   // the image covers one block with its own grain table
-    if (!ctx->nucleation_ready)
-        return MMCA_ERR_STATE;
-    HIPCHK(hipSetDevice(ctx->device));
     TRY(sync_counters(ctx));
     StateHeader h;
     memset(&h, 0, sizeof(h));
@@ -1168,16 +1177,77 @@ int mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size)
     if ((int64_t) ctx->rand_angles.size() < 3 * h.n_angles)
         return MMCA_ERR_STATE;
     Counters hc = ctx->hc;
+    io.ctx = ctx;
+    io.s = ctx->stream;
+    TRY(io.host(&h, sizeof(h)));
+    TRY(state_body(ctx, io, h, hc, table));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MMCA_OK;
+}
+
+// Restores the image of one block starting at io.pos; its header and counters
+// must fit the block's parameters.  *out_hc receives the image's counters.
+int load_block(mmca_ctx *ctx, StateIO &io, TablePart table, StateHeader *out_h, Counters *out_hc)
+{
+    TRY(sync_counters(ctx));
+    if (io.cap - io.pos < (int64_t) (sizeof(StateHeader) + sizeof(Counters)))
+        return MMCA_ERR_ARG;
+    StateHeader h;
+    memcpy(&h, io.in + io.pos, sizeof(h));
+    if (memcmp(h.magic, STATE_MAGIC, 8) != 0 || h.version != 1 || h.P != ctx->c.P || h.nx != ctx->c.nx ||
+        h.ny != ctx->c.ny || h.nz != ctx->c.nz || h.nsites != ctx->b.nsites ||
+        h.max_total_grains != ctx->p.max_total_grains || h.oriented != ctx->oriented ||
+        h.precision != (ctx->c.f32 ? MMCA_PREC_FP32DEV : MMCA_PREC_FP64) || h.cur < 0 || h.cur > 1 ||
+        (ctx->c.f32 && h.nledger != ctx->b.nledger) || h.n_angles < 0 || h.n_angles > ctx->rand_capacity ||
+        h.n_ext < 0 || (table != TABLE_OWN && h.n_ext != 0))
+        return MMCA_ERR_ARG;
+    // the counters come first after the header: check the sizes they imply
+    Counters hc;
+    memcpy(&hc, io.in + io.pos + sizeof(h), sizeof(Counters));
+    if (hc.num_grains < 1 || (int64_t) hc.num_grains > ctx->p.max_total_grains + 1 || hc.gindex < 0 ||
+        (int64_t) hc.gindex > (int64_t) ctx->c.nx * ctx->c.ny * ctx->c.nz || hc.dc_tmp_size < 0 ||
+        (int64_t) hc.dc_tmp_size > (int64_t) ctx->c.nx * ctx->c.ny * ctx->c.nz + 100 ||
+        hc.random_used != h.n_angles)
+        return MMCA_ERR_ARG;
+    io.ctx = ctx;
+    io.s = ctx->stream;
+    io.pos += sizeof(h);
+    TRY(state_body(ctx, io, h, hc, table));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->cur = h.cur;
+    ctx->q0 = hc.random_used;
+    ctx->rand_filled = hc.random_used;
+    ctx->rand_dirty = false;
+    ctx->have_oce = false;
+    ctx->view_dirty = true;
+    ctx->hc = hc;
+    TRY(push_counters(ctx));
+    if (out_h)
+        *out_h = h;
+    if (out_hc)
+        *out_hc = hc;
+    return MMCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size)
+{
+    if (!ctx || capacity < 0 || (buf && capacity == 0))
+        return MMCA_ERR_ARG;
+    if (!ctx->peers.empty())
+        return MMCA_ERR_STATE;   // a coupled rank: mmca_batch_save_state covers the whole batch
+    if (!ctx->nucleation_ready)
+        return MMCA_ERR_STATE;
+    HIPCHK(hipSetDevice(ctx->device));
     StateIO io;
     io.buf = (char *) buf;
     io.in = nullptr;
     io.cap = capacity;
     io.load = false;
-    io.ctx = ctx;
-    io.s = ctx->stream;
-    TRY(io.host(&h, sizeof(h)));
-    TRY(state_body(ctx, io, h, hc));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    TRY(save_block(ctx, io, TABLE_OWN));
     if (size)
         *size = io.pos;
     return MMCA_OK;
@@ -1192,43 +1262,12 @@ int mmca_load_state(mmca_ctx *ctx, const void *buf, int64_t size)
     if (!ctx->nucleation_ready)
         return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
-    TRY(sync_counters(ctx));
-    StateHeader h;
-    memcpy(&h, buf, sizeof(h));
-    if (memcmp(h.magic, STATE_MAGIC, 8) != 0 || h.version != 1 || h.P != ctx->c.P || h.nx != ctx->c.nx ||
-        h.ny != ctx->c.ny || h.nz != ctx->c.nz || h.nsites != ctx->b.nsites ||
-        h.max_total_grains != ctx->p.max_total_grains || h.oriented != ctx->oriented ||
-        h.precision != (ctx->c.f32 ? MMCA_PREC_FP32DEV : MMCA_PREC_FP64) || h.cur < 0 || h.cur > 1 ||
-        (ctx->c.f32 && h.nledger != ctx->b.nledger) || h.n_angles < 0 || h.n_angles > ctx->rand_capacity ||
-        h.n_ext < 0)
-        return MMCA_ERR_ARG;
-    // the counters come first after the header: check the sizes they imply
-    Counters hc;
-    memcpy(&hc, (const char *) buf + sizeof(h), std::min<size_t>(sizeof(Counters), (size_t) size - sizeof(h)));
-    if (hc.num_grains < 1 || (int64_t) hc.num_grains > ctx->p.max_total_grains + 1 || hc.gindex < 0 ||
-        (int64_t) hc.gindex > (int64_t) ctx->c.nx * ctx->c.ny * ctx->c.nz || hc.dc_tmp_size < 0 ||
-        (int64_t) hc.dc_tmp_size > (int64_t) ctx->c.nx * ctx->c.ny * ctx->c.nz + 100 ||
-        hc.random_used != h.n_angles)
-        return MMCA_ERR_ARG;
     StateIO io;
     io.buf = nullptr;
     io.in = (const char *) buf;
     io.cap = size;
     io.load = true;
-    io.ctx = ctx;
-    io.s = ctx->stream;
-    io.pos = sizeof(h);
-    TRY(state_body(ctx, io, h, hc));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    ctx->cur = h.cur;
-    ctx->q0 = hc.random_used;
-    ctx->rand_filled = hc.random_used;
-    ctx->rand_dirty = false;
-    ctx->have_oce = false;
-    ctx->view_dirty = true;
-    ctx->hc = hc;
-    TRY(push_counters(ctx));
-    return MMCA_OK;
+    return load_block(ctx, io, TABLE_OWN, nullptr, nullptr);
 }
 
 int mmca_io_fields(mmca_ctx *ctx, float *fs_io, float *ce_io, float *temperature_io, int32_t *gr_io)
@@ -1569,6 +1608,80 @@ int mmca_batch_couple(mmca_batch *bt, const int32_t nsub[3])
     }
     bt->max_face = mf;
     bt->coupled = true;
+    bt->view_dirty = true;
+    return MMCA_OK;
+}
+
+int mmca_batch_save_state(mmca_batch *bt, void *buf, int64_t capacity, int64_t *size)
+{
+    if (!bt || capacity < 0 || (buf && capacity == 0))
+        return MMCA_ERR_ARG;
+    for (mmca_ctx *c : bt->dom)
+        if (!c->nucleation_ready)
+            return MMCA_ERR_STATE;
+    HIPCHK(hipSetDevice(bt->device));
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    BatchStateHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, BATCH_MAGIC, 8);
+    h.version = 1;
+    h.ndom = (int32_t) bt->dom.size();
+    h.coupled = bt->coupled ? 1 : 0;
+    h.cur = bt->cur;
+    for (int a = 0; a < 3; a++)
+        h.nsub[a] = bt->coupled ? bt->dom[0]->c.nsub[a] : 0;
+    StateIO io;
+    io.buf = (char *) buf;
+    io.in = nullptr;
+    io.cap = capacity;
+    io.load = false;
+    TRY(io.host(&h, sizeof(h)));
+    for (size_t d = 0; d < bt->dom.size(); d++) {
+        const TablePart t = !bt->coupled ? TABLE_OWN : d == 0 ? TABLE_COUPLED_OWNER : TABLE_SHARED;
+        TRY(save_block(bt->dom[d], io, t));
+    }
+    if (size)
+        *size = io.pos;
+    return MMCA_OK;
+}
+
+int mmca_batch_load_state(mmca_batch *bt, const void *buf, int64_t size)
+{
+    if (!bt || !buf || size < (int64_t) sizeof(BatchStateHeader))
+        return MMCA_ERR_ARG;
+    for (mmca_ctx *c : bt->dom)
+        if (!c->nucleation_ready)
+            return MMCA_ERR_STATE;
+    BatchStateHeader h;
+    memcpy(&h, buf, sizeof(h));
+    if (memcmp(h.magic, BATCH_MAGIC, 8) != 0 || h.version != 1 || h.ndom != (int32_t) bt->dom.size() ||
+        h.coupled != (bt->coupled ? 1 : 0) || h.cur < 0 || h.cur > 1)
+        return MMCA_ERR_ARG;
+    for (int a = 0; a < 3; a++)
+        if (h.nsub[a] != (bt->coupled ? bt->dom[0]->c.nsub[a] : 0))
+            return MMCA_ERR_ARG;
+    HIPCHK(hipSetDevice(bt->device));
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    StateIO io;
+    io.buf = nullptr;
+    io.in = (const char *) buf;
+    io.cap = size;
+    io.load = true;
+    io.pos = sizeof(h);
+    Counters first;
+    for (size_t d = 0; d < bt->dom.size(); d++) {
+        const TablePart t = !bt->coupled ? TABLE_OWN : d == 0 ? TABLE_COUPLED_OWNER : TABLE_SHARED;
+        StateHeader bh;
+        Counters hc;
+        TRY(load_block(bt->dom[d], io, t, &bh, &hc));
+        if (d == 0)
+            first = hc;
+        // the domains of a batch step together; coupled ranks share one grain count
+        if (bh.cur != h.cur || hc.timestep != first.timestep ||
+            (bt->coupled && hc.num_grains != first.num_grains))
+            return MMCA_ERR_ARG;
+    }
+    bt->cur = h.cur;
     bt->view_dirty = true;
     return MMCA_OK;
 }

@@ -77,6 +77,8 @@ def lib():
     L.mmca_batch_step.argtypes = [vp, C.c_int, C.c_int, C.POINTER(abi.Report)]
     L.mmca_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.mmca_batch_couple.argtypes = [vp, i32p]
+    L.mmca_batch_save_state.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.mmca_batch_load_state.argtypes = [vp, vp, C.c_int64]
     L.mmca_batch_set_kernel_timing.argtypes = [vp, C.c_int]
     L.mmca_batch_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.mmca_batch_autotune.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.NucleationCfg), C.c_int, C.c_int,
@@ -404,6 +406,18 @@ class Batch:
         nsub[0] x nsub[1] x nsub[2] (mmca_batch_couple)."""
         arr = (C.c_int32 * 3)(*[int(v) for v in nsub])
         _check(lib().mmca_batch_couple(self._h, arr), "mmca_batch_couple")
+
+    def save_state(self):
+        """Restart image of the whole batch (bytes), coupled ranks included."""
+        n = C.c_int64(0)
+        _check(lib().mmca_batch_save_state(self._h, None, 0, C.byref(n)), "mmca_batch_save_state")
+        buf = C.create_string_buffer(n.value)
+        _check(lib().mmca_batch_save_state(self._h, buf, n.value, C.byref(n)), "mmca_batch_save_state")
+        return buf.raw[:n.value]
+
+    def load_state(self, image):
+        buf = C.create_string_buffer(image, len(image))
+        _check(lib().mmca_batch_load_state(self._h, buf, len(image)), "mmca_batch_load_state")
 
     def set_kernel_timing(self, enable=True):
         _check(lib().mmca_batch_set_kernel_timing(self._h, 1 if enable else 0), "mmca_batch_set_kernel_timing")

@@ -110,3 +110,63 @@ def test_coupling_rejects_what_it_cannot_emulate():
             bt.domains[1].save_state()
         reps = bt.step(3)
         assert [r.timestep for r in reps] == [3, 3]
+
+
+@pytest.mark.parametrize("case,save_at", [("m3_multigrain_1x2x3", 150), ("m6_wrap_remelt_x2y2", 400)])
+def test_coupled_restart_continues_bit_for_bit(case, save_at):
+    """A coupled batch saved mid-run (mmca_batch_save_state) and loaded into a
+    fresh coupled batch continues as the reference's uninterrupted ranks do:
+    every rank's hashes equal the reference's at later steps."""
+    cfg, rows, expected = load_case_mr(case)
+    n = expected["ranks"]
+    last = min(expected["last_step"], save_at + 300)
+    params = [cfg.params(check_negative_ce=True, whole_domain=False, sb_coords=rank_coords(cfg, r))
+              for r in range(n)]
+    nucs = [cfg.nucleation(rank=r) for r in range(n)]
+    taint = [dc_taint_step(rows[r]) for r in range(n)]
+    with Batch(params, nucs) as bt:
+        bt.couple(cfg.nsub)
+        bt.step(save_at)
+        image = bt.save_state()
+    with Batch(params, nucs) as bt:
+        bt.couple(cfg.nsub)
+        bt.load_state(image)
+        assert bt.save_state() == image              # the image restores everything it holds
+        done = save_at
+        for t in (save_at + 1, save_at + 20, last):
+            print("case", case, "restart at", save_at, "step", t, flush=True)
+            reps = bt.step(t - done)
+            done = t
+            assert all(r.timestep == t for r in reps)
+            grains = bt.domains[0].grains()
+            ngr = bt.domains[0].num_grains
+            for r, dom in enumerate(bt.domains):
+                ref = rows[r][t]
+                got = rank_state(dom, ngr, grains)
+                bad = [name for name, a, b in zip(IHASH_NAMES, got, ref[2:]) if a != b]
+                if taint[r] is not None and t >= taint[r] and bad == ["dc"]:
+                    bad = []
+                assert (ngr, len(dom.interface_list())) == (ref[0], ref[1]), (case, t, r)
+                assert not bad, (case, t, r, bad)
+
+
+def test_batch_image_refuses_other_layouts():
+    """A coupled image loads only into a batch coupled the same way."""
+    cfg, _, _ = load_case_mr("m1_directional_z2")
+    params = [cfg.params(whole_domain=False, sb_coords=rank_coords(cfg, r)) for r in range(2)]
+    nucs = [cfg.nucleation(rank=r) for r in range(2)]
+    with Batch(params, nucs) as bt:
+        bt.couple(cfg.nsub)
+        bt.step(5)
+        image = bt.save_state()
+        with pytest.raises(abi.MmcaError):
+            bt.domains[0].save_state()               # one rank alone has no image
+    with Batch(params, nucs) as bt:                 # not coupled
+        with pytest.raises(abi.MmcaError):
+            bt.load_state(image)
+    with Batch(params, nucs) as bt:
+        bt.couple(cfg.nsub)
+        with pytest.raises(abi.MmcaError):
+            bt.load_state(image[:-8])                # truncated
+        bt.load_state(image)
+        assert [r.timestep for r in bt.step(1)] == [6, 6]

@@ -51,6 +51,7 @@ typedef struct xdmf_lib {
     mmca_xdmf *(*open)(const mmca_run_cfg *, const int *, const int *);
     int (*write)(mmca_xdmf *, mmca_ctx *const *, const mmca_params *, uint64_t);
     void (*close)(mmca_xdmf *);
+    int (*flush)(mmca_xdmf *);
     int64_t (*get_renumber)(const mmca_xdmf *, uint64_t *, int64_t);
     int (*set_renumber)(mmca_xdmf *, const uint64_t *, int64_t);
     mmca_xdmf *x;
@@ -75,9 +76,10 @@ xdmf_load(xdmf_lib *w)
     *(void **) &w->open = dlsym(w->so, "mmca_xdmf_open");
     *(void **) &w->write = dlsym(w->so, "mmca_xdmf_write");
     *(void **) &w->close = dlsym(w->so, "mmca_xdmf_close");
+    *(void **) &w->flush = dlsym(w->so, "mmca_xdmf_flush");
     *(void **) &w->get_renumber = dlsym(w->so, "mmca_xdmf_get_renumber");
     *(void **) &w->set_renumber = dlsym(w->so, "mmca_xdmf_set_renumber");
-    if (!w->open || !w->write || !w->close || !w->get_renumber || !w->set_renumber)
+    if (!w->open || !w->write || !w->close || !w->flush || !w->get_renumber || !w->set_renumber)
     {
         fprintf(stderr, "%s lacks the writer's entry points\n", path);
         return -1;
@@ -518,7 +520,11 @@ main(int argc, char **argv)
     if (status == MMCA_OK && write_freq > 0 && t % write_freq != 0 && out.write(out.x, doms, ps, t))
         status = MMCA_ERR_ARG;
     if (out.x)
+    {
+        if (out.flush(out.x) && status == MMCA_OK)
+            status = MMCA_ERR_ARG;
         out.close(out.x);
+    }
     int rc = status == MMCA_OK ? output_grains(doms, ps, nr, &cfg) : 1;
     if (bt)
         mmca_batch_destroy(bt);

@@ -6,13 +6,17 @@
  * reference run that writes the same files.
  */
 #include <errno.h>
+#include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
 #include <time.h>
+#include <unistd.h>
 
 #include <hdf5.h>
+#include <zlib.h>
 
 #include "mmca_xdmf.h"
 
@@ -23,6 +27,16 @@ struct mmca_xdmf {
     FILE *ts_fp;               /* time series, open for the whole run */
     size_t *renumber;          /* getRenumberArray (io_xdmf.c:344-375) */
     size_t renumber_sz;
+    void *io[4];               /* host copies of all ranks: Temperature, CE, Grain, FracSolid */
+    size_t io_cells;           /* padded cells each of them holds */
+    float *ang;                /* grain angles X, Y, Z of the pending write, ng each */
+    size_t ang_cap;
+    /* the pending write, finished by a host thread while the run goes on */
+    const mmca_params *ps;
+    uint64_t ts;
+    int ng;
+    pthread_t th;
+    int running, rc;
 };
 
 /* Cart coordinates of a rank (x fastest), the reference's (cx, cy, cz) */
@@ -275,9 +289,23 @@ chunked(int rank, const hsize_t *dims)
     return pl;
 }
 
-/* writeGlobalHeavyData (io_xdmf.c:407-468) */
+/* one dataset of a fresh file: created with the reference's properties and
+ * written from memory space `mem` (H5S_ALL: the whole dataset) */
 static int
-write_global_heavy(mmca_xdmf *x, mmca_ctx *dom0, int ng)
+put(hid_t f, const char *path, hid_t ftype, hid_t mtype, hid_t space, hid_t mem, hid_t link, hid_t pl,
+    const void *data)
+{
+    hid_t ds = H5Dcreate(f, path, ftype, space, link, pl, H5P_DEFAULT);
+    if (ds < 0)
+        return -1;
+    const herr_t st = H5Dwrite(ds, mtype, mem, H5S_ALL, H5P_DEFAULT, data);
+    return (H5Dclose(ds) < 0 || st < 0) ? -1 : 0;
+}
+
+/* writeGlobalHeavyData (io_xdmf.c:407-468), from the pending write's
+ * renumbering and angles */
+static int
+write_global_heavy(mmca_xdmf *x, int ng)
 {
     char fn[600];
     snprintf(fn, sizeof(fn), "%s_gbl.h5", x->cfg->basefilename);
@@ -290,107 +318,307 @@ write_global_heavy(mmca_xdmf *x, mmca_ctx *dom0, int ng)
     hsize_t n = (hsize_t) ng;
     hid_t space = H5Screate_simple(1, &n, NULL);
     hid_t pl = chunked(1, &n);
+    int rc = 0;
     if (x->cfg->randomize_grains)
-    {
-        if (renumber_array(x, dom0, ng))
-            return -1;
-        hid_t ds = H5Dcreate(f, "/GrainRenumber", H5T_NATIVE_UINT, space, H5P_DEFAULT, pl, H5P_DEFAULT);
-        H5Dwrite(ds, sizeof(size_t) == 8 ? H5T_NATIVE_UINT64 : H5T_NATIVE_UINT32, H5S_ALL, H5S_ALL, H5P_DEFAULT,
-                 x->renumber);
-        H5Dclose(ds);
-    }
-    mmca_grain *g = calloc((size_t) ng, sizeof(mmca_grain));
-    float *a = malloc(sizeof(float) * (size_t) ng);
-    if (!g || !a || mmca_get_grains(dom0, g, ng, NULL) != MMCA_OK)
-        return -1;
+        rc = put(f, "/GrainRenumber", H5T_NATIVE_UINT, sizeof(size_t) == 8 ? H5T_NATIVE_UINT64 : H5T_NATIVE_UINT32,
+                 space, H5S_ALL, H5P_DEFAULT, pl, x->renumber);
     const char lbl[3] = { 'X', 'Y', 'Z' };
-    for (int d = 0; d < 3; d++)
+    for (int d = 0; d < 3 && rc == 0; d++)
     {
-        for (int i = 0; i < ng; i++)
-            a[i] = (float) g[i].rotang[d];
         char name[32];
         snprintf(name, sizeof(name), "/GrainAngle%c", lbl[d]);
-        hid_t ds = H5Dcreate(f, name, H5T_NATIVE_FLOAT, space, H5P_DEFAULT, pl, H5P_DEFAULT);
-        H5Dwrite(ds, H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, a);
-        H5Dclose(ds);
+        rc = put(f, name, H5T_NATIVE_FLOAT, H5T_NATIVE_FLOAT, space, H5S_ALL, H5P_DEFAULT, pl,
+                 x->ang + (size_t) d * (size_t) ng);
     }
-    free(g);
-    free(a);
     H5Pclose(pl);
     H5Sclose(space);
-    H5Fclose(f);
+    if (H5Fclose(f) < 0)
+        rc = -1;
+    if (rc)
+        fprintf(stderr, "Unable to write HDF5 file %s\n", fn);
+    return rc;
+}
+
+/*
+ * One dataset of a subblock file, filtered off the HDF5 call: the interior of
+ * a padded FP32/int32 host copy, shuffled and deflated exactly as the
+ * dataset's filter pipeline does it (H5Z shuffle: byte b of element i goes to
+ * b * n + i; H5Z deflate: compress2 at level 9 into ceil(1.001 n) + 12
+ * bytes, and an optional filter that fails leaves its bit set in the chunk's
+ * filter mask), so that H5Dwrite_chunk stores the bytes H5Dwrite would.  The
+ * datasets of all ranks are compressed at once on host threads, as the
+ * reference's ranks compress their subblocks at the same time.
+ */
+typedef struct ztask {
+    const unsigned char *padded;   /* 4-byte elements over the padded box */
+    int nx, ny, nz;
+    unsigned char *out;            /* the stored chunk */
+    size_t out_n;
+    uint32_t mask;                 /* filters skipped (bit 1: deflate) */
+    int failed;
+} ztask;
+
+typedef struct zpool {
+    ztask *t;
+    int n;
+    int next;
+} zpool;
+
+static void
+ztask_run(ztask *z)
+{
+    const size_t nx = (size_t) z->nx, ny = (size_t) z->ny, nz = (size_t) z->nz;
+    const size_t n = nx * ny * nz, nb = 4 * n;
+    unsigned char *shuf = malloc(nb ? nb : 1);
+    uLongf cap = (uLongf) ceil((double) nb * (double) 1.001f) + 12;
+    z->out = malloc(cap);
+    if (!shuf || !z->out)
+    {
+        free(shuf);
+        z->failed = 1;
+        return;
+    }
+    /* interior rows of the padded box, x fastest, straight into shuffled order */
+    size_t i = 0;
+    for (size_t k = 1; k <= nz; k++)
+        for (size_t j = 1; j <= ny; j++)
+        {
+            const unsigned char *row = z->padded + 4 * ((k * (ny + 2) + j) * (nx + 2) + 1);
+            for (size_t x = 0; x < nx; x++, i++)
+                for (int b = 0; b < 4; b++)
+                    shuf[(size_t) b * n + i] = row[4 * x + b];
+        }
+    const int st = compress2(z->out, &cap, shuf, nb, 9);
+    if (st == Z_OK)
+    {
+        z->out_n = cap;
+        z->mask = 0;
+        free(shuf);
+    }
+    else if (st == Z_BUF_ERROR)
+    {
+        free(z->out);              /* deflate is optional: the shuffled bytes are stored */
+        z->out = shuf;
+        z->out_n = nb;
+        z->mask = 1u << 1;
+    }
+    else
+    {
+        free(shuf);
+        z->failed = 1;
+    }
+}
+
+static void *
+zpool_worker(void *arg)
+{
+    zpool *p = arg;
+    for (;;)
+    {
+        const int i = __atomic_fetch_add(&p->next, 1, __ATOMIC_RELAXED);
+        if (i >= p->n)
+            return NULL;
+        ztask_run(&p->t[i]);
+    }
+}
+
+static int
+io_threads(int tasks)
+{
+    const char *e = getenv("MMCA_IO_THREADS");
+    long n = e ? atol(e) : sysconf(_SC_NPROCESSORS_ONLN);
+    if (n < 1)
+        n = 1;
+    if (n > 64)
+        n = 64;
+    return (int) (n < tasks ? n : tasks);
+}
+
+/* cache_io_data (loop.c:74-113) of every rank into the host copies (field
+ * order of the files: Temperature, CE, Grain, FracSolid) */
+static int
+fetch_fields(mmca_xdmf *x, mmca_ctx *const *doms, const mmca_params *ps)
+{
+    const int nr = x->nranks;
+    const size_t P = (size_t) (ps[0].nx + 2) * (ps[0].ny + 2) * (ps[0].nz + 2);
+    if (P * (size_t) nr > x->io_cells)
+    {
+        for (int v = 0; v < 4; v++)
+        {
+            free(x->io[v]);
+            x->io[v] = malloc(4 * P * (size_t) nr);
+            if (!x->io[v])
+            {
+                fprintf(stderr, "out of host memory\n");
+                x->io_cells = 0;
+                return -1;
+            }
+        }
+        x->io_cells = P * (size_t) nr;
+    }
+    for (int r = 0; r < nr; r++)
+    {
+        float *t = (float *) x->io[0] + P * r, *ce = (float *) x->io[1] + P * r;
+        int32_t *gr = (int32_t *) x->io[2] + P * r;
+        float *fs = (float *) x->io[3] + P * r;
+        if (mmca_io_fields(doms[r], fs, ce, t, gr) != MMCA_OK)
+            return -1;
+    }
     return 0;
 }
 
-/* xdmf_writeSubblocks (io_xdmf.c:559-673) for rank r */
+/* xdmf_writeSubblocks (io_xdmf.c:559-673) for every rank, from the host
+ * copies: one file each with its four datasets, created with the
+ * reference's properties */
 static int
-write_subblock(mmca_xdmf *x, mmca_ctx *dom, const mmca_params *p, int r, uint64_t ts)
+write_subblocks(mmca_xdmf *x, const mmca_params *ps, uint64_t ts)
 {
-    const size_t P = (size_t) (p->nx + 2) * (p->ny + 2) * (p->nz + 2);
-    float *fs = malloc(sizeof(float) * P), *ce = malloc(sizeof(float) * P), *t = malloc(sizeof(float) * P);
-    int32_t *gr = malloc(sizeof(int32_t) * P);
-    if (!fs || !ce || !t || !gr)
-    {
+    const int nr = x->nranks;
+    if (nr < 1)
+        return -1;
+    const size_t P = (size_t) (ps[0].nx + 2) * (ps[0].ny + 2) * (ps[0].nz + 2);
+    const int nt = 4 * nr;
+    ztask *tk = calloc((size_t) nt, sizeof(ztask));
+    if (!tk)
+        return -1;
+    for (int r = 0; r < nr; r++)
+        for (int v = 0; v < 4; v++)
+        {
+            ztask *z = &tk[4 * r + v];
+            z->padded = (const unsigned char *) x->io[v] + 4 * P * (size_t) r;
+            z->nx = ps[r].nx;
+            z->ny = ps[r].ny;
+            z->nz = ps[r].nz;
+        }
+    zpool pool = { tk, nt, 0 };
+    const int nth = io_threads(nt);
+    pthread_t th[64];
+    int started = 0;
+    for (; started < nth - 1; started++)
+        if (pthread_create(&th[started], NULL, zpool_worker, &pool) != 0)
+            break;
+    zpool_worker(&pool);
+    for (int i = 0; i < started; i++)
+        pthread_join(th[i], NULL);
+    int rc = 0;
+    for (int i = 0; i < nt; i++)
+        if (tk[i].failed)
+            rc = -1;
+    if (rc)
         fprintf(stderr, "out of host memory\n");
-        return -1;
-    }
-    if (mmca_io_fields(dom, fs, ce, t, gr) != MMCA_OK)
-        return -1;
-    char fn[700];
-    snprintf(fn, sizeof(fn), "%s_r%d/%05lu.h5", x->cfg->basefilename, r, (unsigned long) ts);
-    hid_t f = H5Fcreate(fn, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
-    if (f < 0)
+    for (int r = 0; r < nr && rc == 0; r++)
     {
-        fprintf(stderr, "Unable to open HDF5 file %s\n", fn);
-        return -1;
+        char fn[700];
+        snprintf(fn, sizeof(fn), "%s_r%d/%05lu.h5", x->cfg->basefilename, r, (unsigned long) ts);
+        hid_t f = H5Fcreate(fn, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+        if (f < 0)
+        {
+            fprintf(stderr, "Unable to open HDF5 file %s\n", fn);
+            rc = -1;
+            break;
+        }
+        const hsize_t count[3] = { (hsize_t) ps[r].nz, (hsize_t) ps[r].ny, (hsize_t) ps[r].nx };
+        const hsize_t origin[3] = { 0, 0, 0 };
+        hid_t space = H5Screate_simple(3, count, NULL);
+        hid_t link = H5Pcreate(H5P_LINK_CREATE);
+        H5Pset_create_intermediate_group(link, 1);
+        hid_t pl = chunked(3, count);
+        int cx, cy, cz;
+        rank_coords(x, r, &cx, &cy, &cz);
+        const char *names[4] = { "Temperature", "CE", "Grain", "FracSolid" };
+        for (int v = 0; v < 4 && rc == 0; v++)
+        {
+            char path[128];
+            snprintf(path, sizeof(path), "/SubBlock_%dx%dx%d/%s", cx, cy, cz, names[v]);
+            const ztask *z = &tk[4 * r + v];
+            hid_t ds = H5Dcreate(f, path, v == 2 ? H5T_NATIVE_INT : H5T_NATIVE_FLOAT, space, link, pl, H5P_DEFAULT);
+            if (ds < 0 || H5Dwrite_chunk(ds, H5P_DEFAULT, z->mask, origin, z->out_n, z->out) < 0)
+                rc = -1;
+            if (ds >= 0 && H5Dclose(ds) < 0)
+                rc = -1;
+        }
+        H5Pclose(pl);
+        H5Pclose(link);
+        H5Sclose(space);
+        if (H5Fclose(f) < 0)
+            rc = -1;
+        if (rc)
+            fprintf(stderr, "Unable to write HDF5 file %s\n", fn);
     }
-    const hsize_t start[3] = { 1, 1, 1 };
-    const hsize_t padded[3] = { (hsize_t) p->nz + 2, (hsize_t) p->ny + 2, (hsize_t) p->nx + 2 };
-    const hsize_t count[3] = { (hsize_t) p->nz, (hsize_t) p->ny, (hsize_t) p->nx };
-    hid_t space = H5Screate_simple(3, count, NULL);
-    hid_t link = H5Pcreate(H5P_LINK_CREATE);
-    H5Pset_create_intermediate_group(link, 1);
-    hid_t pl = chunked(3, count);
-    hid_t mem = H5Screate_simple(3, padded, NULL);
-    H5Sselect_hyperslab(mem, H5S_SELECT_SET, start, NULL, count, NULL);
-    int cx, cy, cz;
-    rank_coords(x, r, &cx, &cy, &cz);
-    const char *names[4] = { "Temperature", "CE", "Grain", "FracSolid" };
-    const void *data[4] = { t, ce, gr, fs };
-    for (int v = 0; v < 4; v++)
+    for (int i = 0; i < nt; i++)
+        free(tk[i].out);
+    free(tk);
+    return rc;
+}
+
+/* the part of a write that needs neither the device nor the random stream:
+ * writeMain's time-series entry and global file, then the subblock files */
+static void *
+finish_write(void *arg)
+{
+    mmca_xdmf *x = arg;
+    x->rc = (write_time_series(x, x->ts, x->ng) || write_global_heavy(x, x->ng) || write_subblocks(x, x->ps, x->ts))
+                ? -1 : 0;
+    return NULL;
+}
+
+int
+mmca_xdmf_flush(mmca_xdmf *x)
+{
+    if (x->running)
     {
-        char path[128];
-        snprintf(path, sizeof(path), "/SubBlock_%dx%dx%d/%s", cx, cy, cz, names[v]);
-        const hid_t type = (v == 2) ? H5T_NATIVE_INT : H5T_NATIVE_FLOAT;
-        hid_t ds = H5Dcreate(f, path, type, space, link, pl, H5P_DEFAULT);
-        H5Dwrite(ds, type, mem, H5S_ALL, H5P_DEFAULT, data[v]);
-        H5Dclose(ds);
+        pthread_join(x->th, NULL);
+        x->running = 0;
     }
-    H5Fclose(f);
-    H5Pclose(pl);
-    H5Pclose(link);
-    H5Sclose(space);
-    H5Sclose(mem);
-    free(fs);
-    free(ce);
-    free(t);
-    free(gr);
-    return 0;
+    return x->rc;
 }
 
 int
 mmca_xdmf_write(mmca_xdmf *x, mmca_ctx *const *doms, const mmca_params *ps, uint64_t ts)
 {
+    if (mmca_xdmf_flush(x))
+        return -1;
     int32_t ng = 0;
-    if (mmca_get_grains(doms[0], NULL, 0, &ng) != MMCA_OK)
+    if (mmca_get_grains(doms[0], NULL, 0, &ng) != MMCA_OK || ng < 1)
         return -1;
-    /* writeMain on rank 0 (the time-series entry, then the global data),
-     * then every rank's subblock file */
-    if (write_time_series(x, ts, ng) || write_global_heavy(x, doms[0], ng))
+    /* at the write step: the renumbering's draws from rank 0's stream, the
+     * grain angles and the fields */
+    if (x->cfg->randomize_grains && renumber_array(x, doms[0], ng))
         return -1;
-    for (int r = 0; r < x->nranks; r++)
-        if (write_subblock(x, doms[r], &ps[r], r, ts))
-            return -1;
+    if ((size_t) ng > x->ang_cap)
+    {
+        free(x->ang);
+        x->ang = malloc(3 * sizeof(float) * (size_t) ng);
+        x->ang_cap = x->ang ? (size_t) ng : 0;
+    }
+    mmca_grain *g = calloc((size_t) ng, sizeof(mmca_grain));
+    if (!x->ang || !g)
+    {
+        free(g);
+        fprintf(stderr, "out of host memory\n");
+        return -1;
+    }
+    if (mmca_get_grains(doms[0], g, ng, NULL) != MMCA_OK)
+    {
+        free(g);
+        return -1;
+    }
+    for (int d = 0; d < 3; d++)
+        for (int i = 0; i < ng; i++)
+            x->ang[(size_t) d * (size_t) ng + i] = (float) g[i].rotang[d];
+    free(g);
+    if (fetch_fields(x, doms, ps))
+        return -1;
+    x->ps = ps;
+    x->ts = ts;
+    x->ng = ng;
+    x->rc = 0;
+    if (pthread_create(&x->th, NULL, finish_write, x) != 0)
+    {
+        finish_write(x);
+        return x->rc;
+    }
+    x->running = 1;
     return 0;
 }
 
@@ -399,9 +627,13 @@ mmca_xdmf_close(mmca_xdmf *x)
 {
     if (!x)
         return;
+    mmca_xdmf_flush(x);
     if (x->ts_fp)
         fclose(x->ts_fp);
     free(x->renumber);
+    for (int v = 0; v < 4; v++)
+        free(x->io[v]);
+    free(x->ang);
     free(x);
 }
 

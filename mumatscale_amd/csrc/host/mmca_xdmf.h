@@ -21,11 +21,18 @@ typedef struct mmca_xdmf mmca_xdmf;
  * Returns NULL with a message on stderr on failure. */
 mmca_xdmf *mmca_xdmf_open(const mmca_run_cfg *cfg, const int nsub[3], const int sbdim[3]);
 
-/* writeData (loop.c:116-127) at timestep ts: the ranks' fields through
- * mmca_io_fields, then rank 0's time-series entry and global data (its
- * grain renumbering draws from rank 0's random stream, mmca_rand), then
- * each rank's HDF5 file.  doms[r] / ps[r]: the subblock of rank r. */
+/* writeData (loop.c:116-127) at timestep ts.  Before it returns: the grain
+ * renumbering's draws from rank 0's random stream (mmca_rand), the grain
+ * angles and the ranks' fields (mmca_io_fields).  A host thread then writes
+ * rank 0's time-series entry and global file and every rank's HDF5 file
+ * while the caller steps on; the next write, mmca_xdmf_flush and
+ * mmca_xdmf_close wait for it.  doms[r] / ps[r]: the subblock of rank r (ps
+ * must stay valid until then).  Returns 0, or -1 with a message on stderr
+ * (also for a failure of the previous write's files). */
 int mmca_xdmf_write(mmca_xdmf *x, mmca_ctx *const *doms, const mmca_params *ps, uint64_t ts);
+
+/* Waits for the pending write; 0 if its files are complete. */
+int mmca_xdmf_flush(mmca_xdmf *x);
 
 void mmca_xdmf_close(mmca_xdmf *x);
 

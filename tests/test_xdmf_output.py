@@ -46,6 +46,22 @@ def write_steps(case):
     return sorted(int(f[-8:-3]) for f in digest["files"] if f.startswith(base + "_r0/"))
 
 
+def test_writer_library_exports_its_entry_points():
+    """libmmca_xdmf.so (loaded by mmca_run for DataFormat XDMF) is built and
+    exports what mmca_xdmf.h declares."""
+    import ctypes
+    import re
+    so = os.path.join(ROOT, "mumatscale_amd", "libmmca_xdmf.so")
+    if not os.path.exists(so):
+        pytest.fail("libmmca_xdmf.so is not built: run __graft_entry__.build()")
+    hdr = open(os.path.join(ROOT, "mumatscale_amd", "csrc", "host", "mmca_xdmf.h")).read()
+    names = re.findall(r"(mmca_xdmf_\w+)\s*\(", hdr)
+    assert len(set(names)) == 6
+    L = ctypes.CDLL(so)
+    for n in names:
+        assert hasattr(L, n), n
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_fixture_is_a_complete_output(case):
     """Every rank has a file for every written step, the time series names

@@ -170,3 +170,36 @@ def test_batch_image_refuses_other_layouts():
             bt.load_state(image[:-8])                # truncated
         bt.load_state(image)
         assert [r.timestep for r in bt.step(1)] == [6, 6]
+
+
+def test_coupled_fp32dev_against_fp64():
+    """Coupled ranks with FP32-deviation storage: the face exchange carries the
+    FP32 cl, and against the FP64 coupled run the global solute balance
+    (stored + ledger, summed over the ranks) closes to FP64 rounding, the grain
+    counts are equal and the solute and solid-fraction errors stay small."""
+    cfg, _, expected = load_case_mr("m3_multigrain_1x2x3")
+    n = expected["ranks"]
+    runs = {}
+    for prec in (abi.PREC_FP64, abi.PREC_FP32DEV):
+        params = [cfg.params(whole_domain=False, sb_coords=rank_coords(cfg, r), precision=prec)
+                  for r in range(n)]
+        nucs = [cfg.nucleation(rank=r) for r in range(n)]
+        with Batch(params, nucs) as bt:
+            bt.couple(cfg.nsub)
+            reps = bt.step(700, want_sums=True)
+            ce = np.concatenate([d.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1].ravel() for d in bt.domains])
+            runs[prec] = (reps, ce, bt.domains[0].num_grains)
+    (ra, ce_a, ng_a), (rb, ce_b, ng_b) = runs[abi.PREC_FP64], runs[abi.PREC_FP32DEV]
+    cells = ce_a.size
+    total0 = cells * cfg.c_init
+    stored = sum(r.solute_sum for r in rb)
+    ledger = sum(r.solute_ledger for r in rb)
+    l2 = np.linalg.norm(ce_b - ce_a) / np.linalg.norm(ce_a)
+    fs_err = abs(sum(r.solid_sum for r in rb) - sum(r.solid_sum for r in ra)) / cells
+    print("coupled fp32: grains %d/%d, L2 %.3e, fs err %.3e, ledger drift %.3e" %
+          (ng_a, ng_b, l2, fs_err, (stored + ledger - total0) / total0))
+    assert abs((stored + ledger - total0) / total0) <= 1e-12
+    assert abs((sum(r.solute_sum for r in ra) - total0) / total0) <= 1e-12
+    assert ng_a == ng_b
+    assert l2 <= 1e-3
+    assert fs_err <= 1e-3

@@ -9,11 +9,12 @@
  *   computeFS    loop.c:173-182   solid_volume / totalNonMoldVolume
  *   output_grains grain.c:752-833 <BaseFileName>_grains.csv and the
  *                                 "N grains. Min ... Average" line
- *   writeData    loop.c:116-127   DataFormat XDMF: at timestep 0, every
- *                                 DataWriteFreq steps from DataWriteStart on
- *                                 and after the last step (mmca_xdmf.h; the
- *                                 writer lives in libmmca_xdmf.so, loaded
- *                                 only when such output is asked for)
+ *   writeData    loop.c:116-127   DataFormat XDMF or VTK: at timestep 0,
+ *                                 every DataWriteFreq steps from
+ *                                 DataWriteStart on and after the last step
+ *                                 (mmca_xdmf.h, mmca_vtk.h; the writers live
+ *                                 in libmmca_io.so, loaded only when such
+ *                                 output is asked for)
  *   writeCheckpoint loop.c:129-145 every checkpointfreq steps (DataWriteStart,
  *                                 as the reference reads it, or
  *                                 --checkpoint-freq): the restart image of
@@ -42,11 +43,13 @@
 
 #include "mmca.h"
 #include "mmca_ctrl.h"
+#include "mmca_vtk.h"
 #include "mmca_xdmf.h"
 
-/* The XDMF writer and libhdf5 behind it, from libmmca_xdmf.so beside this
- * program; runs without data output do not need HDF5 at all. */
-typedef struct xdmf_lib {
+/* The output writers (XDMF with libhdf5 behind it, VTK with zlib), from
+ * libmmca_io.so beside this program; runs without data output do not need
+ * HDF5 at all. */
+typedef struct io_lib {
     void *so;
     mmca_xdmf *(*open)(const mmca_run_cfg *, const int *, const int *);
     int (*write)(mmca_xdmf *, mmca_ctx *const *, const mmca_params *, uint64_t);
@@ -54,11 +57,15 @@ typedef struct xdmf_lib {
     int (*flush)(mmca_xdmf *);
     int64_t (*get_renumber)(const mmca_xdmf *, uint64_t *, int64_t);
     int (*set_renumber)(mmca_xdmf *, const uint64_t *, int64_t);
-    mmca_xdmf *x;
-} xdmf_lib;
+    mmca_vtk *(*vopen)(const mmca_run_cfg *, const int *, const int *);
+    int (*vwrite)(mmca_vtk *, mmca_ctx *const *, const mmca_params *, uint64_t);
+    void (*vclose)(mmca_vtk *);
+    mmca_xdmf *x;                  /* DataFormat XDMF */
+    mmca_vtk *v;                   /* DataFormat VTK */
+} io_lib;
 
 static int
-xdmf_load(xdmf_lib *w)
+io_load(io_lib *w, int vtk)
 {
     char path[PATH_MAX];
     ssize_t n = readlink("/proc/self/exe", path, sizeof(path) - 32);
@@ -66,25 +73,44 @@ xdmf_load(xdmf_lib *w)
         return -1;
     path[n] = 0;
     char *slash = strrchr(path, '/');
-    strcpy(slash ? slash + 1 : path, "libmmca_xdmf.so");
+    strcpy(slash ? slash + 1 : path, "libmmca_io.so");
     w->so = dlopen(path, RTLD_NOW | RTLD_LOCAL);
     if (!w->so)
     {
-        fprintf(stderr, "DataFormat XDMF needs %s: %s\n", path, dlerror());
+        fprintf(stderr, "DataFormat %s needs %s: %s\n", vtk ? "VTK" : "XDMF", path, dlerror());
         return -1;
     }
-    *(void **) &w->open = dlsym(w->so, "mmca_xdmf_open");
-    *(void **) &w->write = dlsym(w->so, "mmca_xdmf_write");
-    *(void **) &w->close = dlsym(w->so, "mmca_xdmf_close");
-    *(void **) &w->flush = dlsym(w->so, "mmca_xdmf_flush");
-    *(void **) &w->get_renumber = dlsym(w->so, "mmca_xdmf_get_renumber");
-    *(void **) &w->set_renumber = dlsym(w->so, "mmca_xdmf_set_renumber");
-    if (!w->open || !w->write || !w->close || !w->flush || !w->get_renumber || !w->set_renumber)
+    int ok;
+    if (vtk)
+    {
+        *(void **) &w->vopen = dlsym(w->so, "mmca_vtk_open");
+        *(void **) &w->vwrite = dlsym(w->so, "mmca_vtk_write");
+        *(void **) &w->vclose = dlsym(w->so, "mmca_vtk_close");
+        ok = w->vopen && w->vwrite && w->vclose;
+    }
+    else
+    {
+        *(void **) &w->open = dlsym(w->so, "mmca_xdmf_open");
+        *(void **) &w->write = dlsym(w->so, "mmca_xdmf_write");
+        *(void **) &w->close = dlsym(w->so, "mmca_xdmf_close");
+        *(void **) &w->flush = dlsym(w->so, "mmca_xdmf_flush");
+        *(void **) &w->get_renumber = dlsym(w->so, "mmca_xdmf_get_renumber");
+        *(void **) &w->set_renumber = dlsym(w->so, "mmca_xdmf_set_renumber");
+        ok = w->open && w->write && w->close && w->flush && w->get_renumber && w->set_renumber;
+    }
+    if (!ok)
     {
         fprintf(stderr, "%s lacks the writer's entry points\n", path);
         return -1;
     }
     return 0;
+}
+
+/* writeData (loop.c:116-127) in the configured format */
+static int
+data_write(io_lib *w, mmca_ctx *const *doms, const mmca_params *ps, uint64_t t)
+{
+    return w->x ? w->write(w->x, doms, ps, t) : w->vwrite(w->v, doms, ps, t);
 }
 
 static double
@@ -127,7 +153,7 @@ finish_step(uint64_t t, double dt, double finish_time)
 static const char CP_MAGIC[8] = { 'M', 'M', 'C', 'A', 'R', 'U', 'N', '1' };
 
 static int
-save_checkpoint(mmca_batch *bt, mmca_ctx *ctx, const xdmf_lib *out, uint64_t t)
+save_checkpoint(mmca_batch *bt, mmca_ctx *ctx, const io_lib *out, uint64_t t)
 {
     int64_t n = 0;
     if (check(bt ? mmca_batch_save_state(bt, NULL, 0, &n) : mmca_save_state(ctx, NULL, 0, &n), "save state"))
@@ -164,7 +190,7 @@ save_checkpoint(mmca_batch *bt, mmca_ctx *ctx, const xdmf_lib *out, uint64_t t)
 }
 
 static int
-load_checkpoint(mmca_batch *bt, mmca_ctx *ctx, xdmf_lib *out)
+load_checkpoint(mmca_batch *bt, mmca_ctx *ctx, io_lib *out)
 {
     FILE *f = fopen("cp_main.mmca", "rb");
     char magic[8];
@@ -409,18 +435,20 @@ main(int argc, char **argv)
     else if (check(mmca_create(&ps[0], device, &doms[0]), "mmca_create") ||
              check(mmca_init_nucleation(doms[0], &ns[0]), "mmca_init_nucleation"))
         return 1;
-    /* writeData's output (loop.c:116-127); VTK is not written by this driver */
-    xdmf_lib out = { 0 };
-    const int write_freq = cfg.vis_format == MMCA_VIS_XDMF ? cfg.data_write_freq : 0;
-    if (cfg.data_write_freq > 0 && cfg.vis_format == MMCA_VIS_VTK)
-        fprintf(stderr, "DataFormat VTK is not written by this driver; no data files.\n");
+    /* writeData's output (loop.c:116-127): XDMF or VTK files, none by default */
+    io_lib out = { 0 };
+    const int write_freq = cfg.vis_format != MMCA_VIS_NONE ? cfg.data_write_freq : 0;
     if (write_freq > 0)
     {
         const int one[3] = { 1, 1, 1 };
-        if (xdmf_load(&out))
+        const int vtk = cfg.vis_format == MMCA_VIS_VTK;
+        if (io_load(&out, vtk))
             return 1;
-        out.x = out.open(&cfg, nr > 1 ? cfg.nsub : one, nr > 1 ? cfg.sbdim : p.gdim);
-        if (!out.x)
+        if (vtk)
+            out.v = out.vopen(&cfg, nr > 1 ? cfg.nsub : one, nr > 1 ? cfg.sbdim : p.gdim);
+        else
+            out.x = out.open(&cfg, nr > 1 ? cfg.nsub : one, nr > 1 ? cfg.sbdim : p.gdim);
+        if (!out.x && !out.v)
             return 1;
     }
     /* checkpointfreq: the reference reads DataWriteStart into it (read_ctrl.c:594-599) */
@@ -440,7 +468,7 @@ main(int argc, char **argv)
     double fs = 0.0;
     const uint64_t fin = cfg.finish_time ? finish_step(t + 1, cfg.dt, cfg.finish_time) : 0;
     int status = MMCA_OK;
-    if (write_freq > 0 && !restart && out.write(out.x, doms, ps, 0))
+    if (write_freq > 0 && !restart && data_write(&out, doms, ps, 0))
         status = MMCA_ERR_ARG;
     while (status == MMCA_OK)
     {
@@ -490,7 +518,7 @@ main(int argc, char **argv)
             fflush(stdout);
         }
         if (write_freq > 0 && (long long) t >= cfg.data_write_start && t % write_freq == 0 &&
-            out.write(out.x, doms, ps, t))
+            data_write(&out, doms, ps, t))
         {
             status = MMCA_ERR_ARG;
             break;
@@ -517,7 +545,7 @@ main(int argc, char **argv)
             break;
     }
     /* the final solution, unless this step was written already */
-    if (status == MMCA_OK && write_freq > 0 && t % write_freq != 0 && out.write(out.x, doms, ps, t))
+    if (status == MMCA_OK && write_freq > 0 && t % write_freq != 0 && data_write(&out, doms, ps, t))
         status = MMCA_ERR_ARG;
     if (out.x)
     {
@@ -525,6 +553,8 @@ main(int argc, char **argv)
             status = MMCA_ERR_ARG;
         out.close(out.x);
     }
+    if (out.v)
+        out.vclose(out.v);
     int rc = status == MMCA_OK ? output_grains(doms, ps, nr, &cfg) : 1;
     if (bt)
         mmca_batch_destroy(bt);

@@ -30,6 +30,11 @@ per-rank hashes left out, and keep a digest of the files written
                            dataset
   <case>/expected.json     as above
 
+Control files in tests/golden/cases_vtk/ ask for DataFormat VTK:
+
+  <case>/vtk.json          a hash of every .pvtu and .vtu file written
+  <case>/expected.json     as above
+
 Run in the container that holds the reference sources:
     python tests/golden/make_golden.py
 """
@@ -51,6 +56,7 @@ REPO = os.path.dirname(os.path.dirname(HERE))
 CASES = os.path.join(HERE, "cases")
 CASES_MR = os.path.join(HERE, "cases_mr")
 CASES_XDMF = os.path.join(HERE, "cases_xdmf")
+CASES_VTK = os.path.join(HERE, "cases_vtk")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 REF_BIN = os.path.join(REPO, "oracle", "_ref", "mumatscale_ref")
 
@@ -170,6 +176,36 @@ def run_case_xdmf(name, out_dir):
         return exp
 
 
+def run_case_vtk(name, out_dir):
+    ctrl = os.path.join(CASES_VTK, name + ".in")
+    n, base = nranks(ctrl), base_name(ctrl)
+    with tempfile.TemporaryDirectory() as tmp:
+        if os.path.dirname(base):
+            os.makedirs(os.path.join(tmp, os.path.dirname(base)))
+        env = dict(os.environ, MMCA_REF_DUMP_DIR=os.path.join(tmp, "hook"), OMP_NUM_THREADS="1")
+        os.makedirs(env["MMCA_REF_DUMP_DIR"])
+        proc = subprocess.run([MPIEXEC, "-hosts", "127.0.0.1", "-n", str(n), REF_BIN, "-c", ctrl], cwd=tmp,
+                              env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=3600)
+        text = proc.stdout.decode(errors="replace")
+        files = xdmf_digest.vtk_digest(tmp, base)
+        os.makedirs(out_dir, exist_ok=True)
+        with open(os.path.join(out_dir, "vtk.json"), "w") as fh:
+            json.dump(files, fh, indent=0, sort_keys=True)
+        avg = [l for l in text.splitlines() if "Average" in l]
+        err = [l.strip() for l in text.splitlines() if "ERROR" in l]
+        steps = [int(f.rsplit("_", 1)[1][:-5]) for f in files if f.endswith(".pvtu")]
+        exp = {
+            "returncode": proc.returncode,
+            "ranks": n,
+            "last_step": max(steps),
+            "error_line": err[0] if err else None,
+            "average_line": avg[0].strip() if avg else None,
+        }
+        with open(os.path.join(out_dir, "expected.json"), "w") as fh:
+            json.dump(exp, fh, indent=1)
+        return exp
+
+
 def main():
     subprocess.check_call(["make", "-s", "-f", os.path.join(REPO, "oracle", "ref.mk")], cwd=REPO)
     names = sorted(f[:-3] for f in os.listdir(CASES) if f.endswith(".in"))
@@ -198,6 +234,15 @@ def main():
         if os.path.isdir(out):
             shutil.rmtree(out)
         exp = run_case_xdmf(n, out)
+        print(n, exp)
+    names = sorted(f[:-3] for f in os.listdir(CASES_VTK) if f.endswith(".in"))
+    if len(sys.argv) > 1:
+        names = [n for n in names if n in sys.argv[1:]]
+    for n in names:
+        out = os.path.join(HERE, n)
+        if os.path.isdir(out):
+            shutil.rmtree(out)
+        exp = run_case_vtk(n, out)
         print(n, exp)
 
 

@@ -47,16 +47,18 @@ def write_steps(case):
 
 
 def test_writer_library_exports_its_entry_points():
-    """libmmca_xdmf.so (loaded by mmca_run for DataFormat XDMF) is built and
-    exports what mmca_xdmf.h declares."""
+    """libmmca_io.so (loaded by mmca_run for DataFormat XDMF or VTK) is built
+    and exports what mmca_xdmf.h and mmca_vtk.h declare."""
     import ctypes
     import re
-    so = os.path.join(ROOT, "mumatscale_amd", "libmmca_xdmf.so")
+    so = os.path.join(ROOT, "mumatscale_amd", "libmmca_io.so")
     if not os.path.exists(so):
-        pytest.fail("libmmca_xdmf.so is not built: run __graft_entry__.build()")
-    hdr = open(os.path.join(ROOT, "mumatscale_amd", "csrc", "host", "mmca_xdmf.h")).read()
-    names = re.findall(r"(mmca_xdmf_\w+)\s*\(", hdr)
-    assert len(set(names)) == 6
+        pytest.fail("libmmca_io.so is not built: run __graft_entry__.build()")
+    names = []
+    for h in ("mmca_xdmf.h", "mmca_vtk.h"):
+        hdr = open(os.path.join(ROOT, "mumatscale_amd", "csrc", "host", h)).read()
+        names += re.findall(r"(mmca_(?:xdmf|vtk)_\w+)\s*\(", hdr)
+    assert len(set(names)) == 9
     L = ctypes.CDLL(so)
     for n in names:
         assert hasattr(L, n), n

@@ -6,7 +6,10 @@ HDF5 files are read through the h5dump tool of the image's HDF5 (h5py is not
 installed): `h5dump -p -H` gives the structure (datasets, types, shapes,
 chunking, filters, fill time and the stored size after compression) and
 `h5dump -b LE` the raw data of every dataset, which is hashed per dataset.
-The time series is kept as text without its wall-clock date line."""
+The time series is kept as text without its wall-clock date line.
+
+VTK output (io_vtk.c) holds no date, so its digest is a hash of every
+<base>_<step>.pvtu and <base>_<rank>/<step>.vtu file."""
 import glob
 import hashlib
 import os
@@ -111,3 +114,10 @@ def compare(got, want):
             if g["data"].get(ds) != w["data"].get(ds):
                 bad.append("%s:%s data differs" % (f, ds))
     return bad
+
+
+def vtk_digest(root, base):
+    """{relative path: sha256[:16]} of a run's VTK files under root."""
+    prefix = os.path.join(root, base)
+    paths = sorted(glob.glob(prefix + "_*.pvtu") + glob.glob(prefix + "_[0-9][0-9][0-9][0-9]/*.vtu"))
+    return {os.path.relpath(p, root): hashlib.sha256(open(p, "rb").read()).hexdigest()[:16] for p in paths}

@@ -169,3 +169,14 @@ def test_amtrack_through_c_driver(tmp_path):
     assert ngrains == 1889
     assert abs(avg - 9.1e-7) <= 1e-8
     assert abs(fs - 86.0) <= 0.2
+
+
+@pytest.mark.gpu
+def test_courant_abort_through_c_driver(tmp_path):
+    """A dt beyond the Courant limit stops the run at the first step with the
+    reference's message (growth.c:43-50) and a failing exit status."""
+    path = os.path.join(GOLDEN, "cases", "c5_courant.in")
+    r = subprocess.run([RUN, path], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode != 0
+    assert "step 1: Instability by Courant criterion" in r.stderr

@@ -53,6 +53,8 @@ def test_oracle_matches_reference_every_step(case):
             assert st == abi.ERR_NEGATIVE_CE
         elif "grain.c:365" in expected["error_line"]:
             assert st == abi.ERR_GRAIN_CAPACITY
+        elif "Courant" in expected["error_line"]:
+            assert st == abi.ERR_COURANT
         else:
             pytest.fail("unexpected reference error " + expected["error_line"])
         assert s.failed_step == expected["last_step"] + 1

@@ -65,3 +65,16 @@ def test_orientation_draws_match_oracle_grains(case):
     assert k >= 1
     n_fixed_grains = sum(1 for f in fixed if f >= 0)
     assert k >= len(rot) - n_fixed_grains * 60
+
+
+def test_threshold_draw_that_cannot_end_is_an_error():
+    """genRandomThreshold (grain.c:141-154) redraws while t < 1e-5 or
+    |t - centre| > 6 sigma; with centre 0 and sigma 0 no draw is accepted and
+    the reference loops forever.  The set-up stops after a bounded number of
+    draws with MMCA_ERR_NUC_THRESHOLD instead."""
+    cfg, _, _, _ = load_case("c1_directional")
+    cfg.bulk_mean = 0.0
+    cfg.bulk_sigma = 0.0
+    with pytest.raises(abi.MmcaError) as e:
+        lib.nucleation_sites(cfg.params(), cfg.nucleation())
+    assert e.value.code == abi.ERR_NUC_THRESHOLD

@@ -23,7 +23,7 @@ and keeps, per rank and step, the hook's interior hashes:
 Control files in tests/golden/cases_xdmf/ ask for the reference's XDMF
 output (DataFormat XDMF); they run on their NSubblocks ranks, with the hook's
 per-rank hashes left out, and keep a digest of the files written
-(tests/xdmf_digest.py):
+(tests/output_digest.py):
 
   <case>/digest.json.gz    the time series text (without its date line), the
                            structure of every HDF5 file and a hash of each
@@ -49,7 +49,7 @@ import tempfile
 import numpy as np
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-import xdmf_digest  # noqa: E402
+import output_digest  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
@@ -157,7 +157,7 @@ def run_case_xdmf(name, out_dir):
         proc = subprocess.run([MPIEXEC, "-hosts", "127.0.0.1", "-n", str(n), REF_BIN, "-c", ctrl], cwd=tmp,
                               env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=3600)
         text = proc.stdout.decode(errors="replace")
-        digest = xdmf_digest.digest_output(tmp, base)
+        digest = output_digest.digest_output(tmp, base)
         os.makedirs(out_dir, exist_ok=True)
         with gzip.open(os.path.join(out_dir, "digest.json.gz"), "wt") as fh:
             json.dump(digest, fh, indent=0, sort_keys=True)
@@ -187,7 +187,7 @@ def run_case_vtk(name, out_dir):
         proc = subprocess.run([MPIEXEC, "-hosts", "127.0.0.1", "-n", str(n), REF_BIN, "-c", ctrl], cwd=tmp,
                               env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=3600)
         text = proc.stdout.decode(errors="replace")
-        files = xdmf_digest.vtk_digest(tmp, base)
+        files = output_digest.vtk_digest(tmp, base)
         os.makedirs(out_dir, exist_ok=True)
         with open(os.path.join(out_dir, "vtk.json"), "w") as fh:
             json.dump(files, fh, indent=0, sort_keys=True)

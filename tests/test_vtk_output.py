@@ -13,7 +13,7 @@ import subprocess
 
 import pytest
 
-import xdmf_digest
+import output_digest
 from golden_cases import GOLDEN
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -56,6 +56,6 @@ def test_vtk_output_matches_reference(case, tmp_path):
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     assert expected["average_line"] in r.stdout
-    got = xdmf_digest.vtk_digest(str(tmp_path), base)
+    got = output_digest.vtk_digest(str(tmp_path), base)
     bad = sorted(f for f in set(got) | set(want) if got.get(f) != want.get(f))
     assert not bad, bad[:20]

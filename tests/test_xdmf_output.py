@@ -1,7 +1,7 @@
 """XDMF + HDF5 output of the C host driver (DataFormat XDMF) against the
 reference's own files for the same control file (tests/golden/cases_xdmf/,
 fixtures from tests/golden/make_golden.py, compared through the digests of
-tests/xdmf_digest.py).
+tests/output_digest.py).
 
 Every HDF5 file must have the reference's structure (datasets, types, shapes,
 chunking, shuffle + deflate 9, fill time, and so the same stored size) and
@@ -23,7 +23,7 @@ import subprocess
 
 import pytest
 
-import xdmf_digest
+import output_digest
 from golden_cases import GOLDEN
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -94,7 +94,7 @@ def test_fixture_is_a_complete_output(case):
 def test_xdmf_output_matches_reference(case, tmp_path):
     if not os.path.exists(RUN):
         pytest.fail("mmca_run is not built: run __graft_entry__.build()")
-    if not xdmf_digest.available():
+    if not output_digest.available():
         pytest.skip("h5dump is not installed")
     ctrl, base, want, expected = load(case)
     if os.path.dirname(base):
@@ -108,8 +108,8 @@ def test_xdmf_output_matches_reference(case, tmp_path):
     assert r.returncode == 0
     if expected["average_line"]:
         assert expected["average_line"] in r.stdout
-    got = xdmf_digest.digest_output(str(tmp_path), base)
-    bad = xdmf_digest.compare(got, want)
+    got = output_digest.digest_output(str(tmp_path), base)
+    bad = output_digest.compare(got, want)
     assert not bad, bad[:20]
 
 
@@ -123,7 +123,7 @@ def test_restarted_run_writes_the_same_files(case, stop, tmp_path):
     reference's does after a restart, and holds the later entries only."""
     if not os.path.exists(RUN):
         pytest.fail("mmca_run is not built: run __graft_entry__.build()")
-    if not xdmf_digest.available():
+    if not output_digest.available():
         pytest.skip("h5dump is not installed")
     ctrl, base, want, expected = load(case)
     if os.path.dirname(base):
@@ -137,9 +137,9 @@ def test_restarted_run_writes_the_same_files(case, stop, tmp_path):
         assert r.returncode == 0
     assert "Restarting from timestep %d" % stop in r.stdout
     assert expected["average_line"] in r.stdout
-    got = xdmf_digest.digest_output(str(tmp_path), base)
+    got = output_digest.digest_output(str(tmp_path), base)
     later = [t for t in write_steps(case) if t > stop]
-    bad = xdmf_digest.compare(dict(got, xmf=""), dict(want, xmf=""))   # every HDF5 file
+    bad = output_digest.compare(dict(got, xmf=""), dict(want, xmf=""))   # every HDF5 file
     assert not bad, bad[:20]
     head = got["xmf"].splitlines()
     n0 = next(i for i, l in enumerate(head) if 'Name="Time Series"' in l) + 1

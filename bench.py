@@ -162,6 +162,45 @@ def run_fp32_variant(cfg, steps, warmup, shape):
     return res
 
 
+def run_coupled(dims, steps, warmup, shape, nsub=(2, 2, 2)):
+    """The headline domain run as the reference's MPI ranks on this GPU (a
+    coupled batch: face-halo exchanges and cross-rank grain ids each step),
+    against the same domain as one block over the same steps.  The ranks
+    sample their own sites and streams, so the two runs are different
+    simulations of the same size."""
+    from mumatscale_amd.lib import Batch, Subblock
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    one = directional_config(nx, ny, nz)
+    with Subblock(one.params(check_negative_ce=False), one.nucleation()) as sb:
+        apply_shape(sb, shape)
+        sb.step(warmup)
+        t0 = time.perf_counter()
+        sb.step(steps)
+        wall_one = time.perf_counter() - t0
+    cfg = directional_config(nx, ny, nz)
+    cfg.nsub = list(nsub)
+    cfg.sbdim = [nx // nsub[0], ny // nsub[1], nz // nsub[2]]
+    nr = nsub[0] * nsub[1] * nsub[2]
+    coords = [(r % nsub[0], r // nsub[0] % nsub[1], r // (nsub[0] * nsub[1])) for r in range(nr)]
+    params = [cfg.params(check_negative_ce=False, whole_domain=False, sb_coords=coords[r]) for r in range(nr)]
+    nucs = [cfg.nucleation(rank=r) for r in range(nr)]
+    with Batch(params, nucs) as bt:
+        bt.couple(nsub)
+        bt.set_tuning(abi.TUNE_ZM_PLANES, shape["zm"])
+        bt.set_tuning(abi.TUNE_ZM_ROWS, shape["rows"])
+        bt.set_tuning(abi.TUNE_XCD_REMAP, shape["xcd"])
+        bt.step(warmup)
+        t0 = time.perf_counter()
+        reps = bt.step(steps)
+        wall = time.perf_counter() - t0
+    return {"ranks": nr, "nsub": list(nsub), "subblock": cfg.sbdim, "steps": steps,
+            "value": round(n * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+            "one_block_value": round(n * steps / wall_one / 1e6, 1),
+            "one_block_ms_per_step": round(wall_one / steps * 1e3, 4),
+            "grains": reps[0].num_grains - 1}
+
+
 def run_adjoint(n=128, nseeds=16, steps=4000):
     """Gradients of the final-ce L2 mismatch with respect to D_l, D_s, k and
     the initial ce field along a recorded trajectory of the seeded-grain
@@ -471,6 +510,8 @@ def main():
     if not args.no_extra:
         extra["fp32dev"] = run_fp32_variant(cfg, args.extra_steps, args.warmup, shape)
         log("fp32dev:", extra["fp32dev"])
+        extra["coupled_ranks"] = run_coupled((nx, ny, nz), args.extra_steps, args.warmup, shape)
+        log("coupled ranks:", extra["coupled_ranks"])
         extra["batched_ensemble"] = run_ensemble(3000, 200)
         log("ensemble:", extra["batched_ensemble"])
         extra["adjoint"] = run_adjoint()

@@ -468,9 +468,16 @@ def main():
     ap.add_argument("--am-steps", type=int, default=60)
     ap.add_argument("--capture-footprint", action="store_true",
                     help="print the capture kernel's read floor at steps 250-1000 and exit")
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="run only the 2x2x2 coupled-ranks leg (and its one-block comparison), print it and exit")
     args = ap.parse_args()
     if args.capture_footprint:
         capture_footprint()
+        return
+    if args.coupled_only:
+        dims = tuple(int(v) for v in args.size.split("x"))
+        shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd}
+        print(json.dumps(run_coupled(dims, args.extra_steps, args.warmup, shape)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"

@@ -455,39 +455,49 @@ k_halo_exchange(View V, int phase, int cb)
     const Consts c = V.dom[d].c;
     if (c.nsub[0] == 0)
         return;
-    const int n3[3] = { c.nx, c.ny, c.nz };
-    int cn[3];
-    for (int a = 0; a < 3; ++a)
-        cn[a] = c.off[a] / n3[a];
-    cn[ax] += hi ? 1 : -1;
-    if (cn[ax] < 0 || cn[ax] >= c.nsub[ax])
+    // Everything indexed by the face's axis is picked with selects, not array
+    // subscripts: a runtime subscript would put the descriptor in private
+    // (scratch) memory for every thread.
+    const int ns0 = c.nsub[0], ns1 = c.nsub[1], ns2 = c.nsub[2];
+    int cx = c.off[0] / c.nx, cy = c.off[1] / c.ny, cz = c.off[2] / c.nz;
+    const int na = ax == 0 ? c.nx : (ax == 1 ? c.ny : c.nz);     // cells along the axis
+    const int nsa = ax == 0 ? ns0 : (ax == 1 ? ns1 : ns2);
+    int ca = (ax == 0 ? cx : (ax == 1 ? cy : cz)) + (hi ? 1 : -1);
+    if (ca < 0 || ca >= nsa)
     {
         // a face of the global domain: a mold wall, or a wrapped face whose
         // neighbour is the subblock at the other end of the axis
         // (sbID_from_sbcoords, functions.c:272-311); with one subblock along
         // the axis that is the subblock itself, which exchanges nothing
         // (face_util.c:128, 217) and keeps its halo
-        if ((hi ? c.pad_hi[ax] : c.pad_lo[ax]) || c.nsub[ax] < 2)
+        const int wall = hi ? (ax == 0 ? c.pad_hi[0] : (ax == 1 ? c.pad_hi[1] : c.pad_hi[2]))
+                            : (ax == 0 ? c.pad_lo[0] : (ax == 1 ? c.pad_lo[1] : c.pad_lo[2]));
+        if (wall || nsa < 2)
             return;
-        cn[ax] = (cn[ax] + c.nsub[ax]) % c.nsub[ax];
+        ca = (ca + nsa) % nsa;
     }
-    const int r = (cn[2] * c.nsub[1] + cn[1]) * c.nsub[0] + cn[0];   // MPI Cart rank (z, y, x)
+    if (ax == 0)
+        cx = ca;
+    else if (ax == 1)
+        cy = ca;
+    else
+        cz = ca;
+    const int r = (cz * ns1 + cy) * ns0 + cx;     // MPI Cart rank (z, y, x)
     const DevBufs nb = V.dom[r].b;
-    // the two axes along the face
-    const int u = (ax == 0) ? 1 : 0, v = (ax == 2) ? 1 : 2;
-    const long long nface = (long long) n3[u] * n3[v];
+    // the two axes along the face, u (faster) and v, with their strides
+    const int nu = ax == 0 ? c.ny : c.nx, nv = ax == 2 ? c.ny : c.nz;
+    const long long su = ax == 0 ? c.sx : 1, sv = ax == 2 ? c.sx : c.sxy;
+    const long long sa = ax == 0 ? 1 : (ax == 1 ? c.sx : c.sxy);
+    const long long nface = (long long) nu * nv;
     const long long t = (long long) blockIdx.x * blockDim.x + threadIdx.x;
-    const int hal = hi ? n3[ax] + 1 : 0;          // halo layer of this subblock
-    const int src = hi ? 1 : n3[ax];              // the neighbour's interior layer
+    const int hal = hi ? na + 1 : 0;              // halo layer of this subblock
+    const int src = hi ? 1 : na;                  // the neighbour's interior layer
     if (t < nface)
     {
-        int q[3];
-        q[u] = (int) (t % n3[u]) + 1;
-        q[v] = (int) (t / n3[u]) + 1;
-        q[ax] = hal;
-        const long long dst = q[0] + (long long) q[1] * c.sx + (long long) q[2] * c.sxy;
-        q[ax] = src;
-        const long long from = q[0] + (long long) q[1] * c.sx + (long long) q[2] * c.sxy;
+        const int qu = (int) (t % nu) + 1, qv = (int) (t / nu) + 1;
+        const long long base = qu * su + qv * sv;
+        const long long dst = base + hal * sa;
+        const long long from = base + src * sa;
         if (phase == 0)
         {
             if (c.f32)
@@ -505,7 +515,7 @@ k_halo_exchange(View V, int phase, int cb)
             fs_buf(b, cb)[dst] = fs_buf(nb, cb)[from];
             if (ax == 0)
             {
-                const long long row = (long long) q[2] * (c.ny + 2) + q[1];
+                const long long row = (long long) qv * (c.ny + 2) + qu;   // x face: u = y, v = z
                 G(b.xhalo)[2 * row + hi] = (unsigned char) (g > 0);
             }
         }

@@ -145,3 +145,35 @@ def test_restarted_run_writes_the_same_files(case, stop, tmp_path):
     n0 = next(i for i, l in enumerate(head) if 'Name="Time Series"' in l) + 1
     assert head[n0:] == want["xmf"].splitlines()[-(len(head) - n0):]
     assert got["xmf"].count("<Grid Name=\"Full Volume") == len(later)
+
+
+def test_digest_reads_dataset_sizes_from_the_header():
+    """The digest splits h5dump's binary dump by the datasets its header
+    lists, in order; the parser must find nested groups, types and shapes."""
+    header = "\n".join([
+        'HDF5 "f.h5" {',
+        'GROUP "/" {',
+        '   DATASET "GrainAngleX" {',
+        '      DATATYPE  H5T_IEEE_F32LE',
+        '      DATASPACE  SIMPLE { ( 29 ) / ( 29 ) }',
+        '      STORAGE_LAYOUT {',
+        '         CHUNKED ( 29 )',
+        '      }',
+        '   }',
+        '   GROUP "SubBlock_0x1x2" {',
+        '      DATASET "Grain" {',
+        '         DATATYPE  H5T_STD_I32LE',
+        '         DATASPACE  SIMPLE { ( 12, 12, 24 ) / ( 12, 12, 24 ) }',
+        '         FILTERS {',
+        '            COMPRESSION DEFLATE { LEVEL 9 }',
+        '         }',
+        '      }',
+        '   }',
+        '   DATASET "GrainRenumber" {',
+        '      DATATYPE  H5T_STD_U32LE',
+        '      DATASPACE  SIMPLE { ( 29 ) / ( 29 ) }',
+        '   }',
+        '}',
+        '}'])
+    assert output_digest._datasets(header) == [("/GrainAngleX", 4 * 29), ("/SubBlock_0x1x2/Grain", 4 * 12 * 12 * 24),
+                                               ("/GrainRenumber", 4 * 29)]

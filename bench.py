@@ -6,13 +6,17 @@ T = T_liq - 5 K + G (z - V t) with G = 1e4 K/m and V = 1e-3 m/s, surface
 nucleation 1e10 m^-2 (mean 1 K, sigma 0.2 K) on every padded face, bulk
 nucleation 1e13 m^-3 (mean 5 K, sigma 0.5 K), random orientations from seed
 1, c0 = 4 wt%, k = 0.14, m = -2.6 K/wt%, T_liq = 933 K, D_l = 3e-9, D_s = 1e-12
-m^2/s, dt = 0.9 dx^2 / (6 D_l) = 5e-5 s, FSGrow 0.66, plain FP64.
+m^2/s, FSGrow 0.66, plain FP64, dt = 0.45 dx^2 / (6 D_l) = 2.5e-5 s: half
+the explicit diffusion limit.  At the limit itself (5e-5 s) the partitioning
+update is unstable with k = 0.14 and the reference CPU build stops at the
+first negative ce (growth.c:163-165, step ~25); the headline runs the stable
+dt with that check on, and the bench line reports the literal-limit run (the
+step at which ce turns negative, and its throughput without the check) as
+`literal_dt`.
 
 A "step" is one doiteration() (timestep++, temperature, nucleation, grain
 activation, diffusion, partitioning, growth, interface list, capture) over
-the whole block, with the state resident in HBM.  At this dt the reference
-CPU build stops at the first negative ce (growth.c:163-165, step ~25); the
-timed run uses the reference GPU build's behaviour, which has no such check.
+the whole block, with the state resident in HBM.
 
 Prints one JSON line on stdout.  --gpus N with N > 1 is refused: this build
 runs on one GPU.
@@ -32,13 +36,33 @@ from mumatscale_amd.workloads import directional_config, ensemble_configs  # noq
 METRIC = "Mcell-updates/s & % HBM roofline; batched/FP32 vs plain FP64 at stated error"
 HBM_PEAK_GBS = 8000.0
 B_ALG_FP64 = 56.0      # {ce, cl, fs, gr} read + written per cell-update
+B_ALG_FP32 = 40.0      # ce, cl as FP32 deviations; fs FP64; gr int32
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_solute_zm_directional_256x256x512_fp64.md")
+TRAFFIC_PROFILES = {"fp64": os.path.join(ROOT, "profiles", "k_solute_zm_directional_256x256x512_fp64.md"),
+                    "fp32": os.path.join(ROOT, "profiles", "k_solute_zm_directional_256x256x512_fp32dev.md")}
+
+
+def bandwidth_probe(cells, reps=20):
+    """Achievable HBM bandwidth on this GPU for the solute kernel's field
+    mixes (mmca_bandwidth_probe: streaming kernels with no arithmetic), at
+    `cells` cells per launch: GB/s of each mix and its bytes per cell."""
+    import ctypes as C
+    from mumatscale_amd.lib import lib
+    out = {}
+    for mix, name in ((0, "copy_f64"), (1, "solute_mix_fp64"), (2, "solute_mix_fp32dev")):
+        ms, bpc = C.c_double(0), C.c_double(0)
+        st = lib().mmca_bandwidth_probe(0, mix, cells, reps, C.byref(ms), C.byref(bpc))
+        if st != abi.OK:
+            return None
+        out[name] = {"GB/s": round(cells * bpc.value / (ms.value * 1e-3) / 1e9, 1),
+                     "bytes_per_cell": bpc.value, "ms": round(ms.value, 4)}
+    out["cells"] = cells
+    return out
 
 
 def apply_shape(sb, shape):
@@ -46,21 +70,24 @@ def apply_shape(sb, shape):
     sb.set_tuning(abi.TUNE_ZM_PLANES, shape["zm"])
     sb.set_tuning(abi.TUNE_ZM_ROWS, shape["rows"])
     sb.set_tuning(abi.TUNE_XCD_REMAP, shape["xcd"])
+    sb.set_tuning(abi.TUNE_ZM_LDS, shape.get("lds", 0))
+    sb.set_tuning(abi.TUNE_CAPTURE_CHUNK, shape.get("chunk", 0))
 
 
-def measured_traffic(size, shape):
+def measured_traffic(size, shape, prec="fp64"):
     """HBM bytes per launch of the solute kernel from the committed
     FETCH_SIZE/WRITE_SIZE summary, when this run has the shape it was
     measured on (None otherwise)."""
     import re
-    if not os.path.exists(TRAFFIC_PROFILE):
+    path = TRAFFIC_PROFILES[prec]
+    if not os.path.exists(path):
         return None
-    with open(TRAFFIC_PROFILE) as fh:
+    with open(path) as fh:
         text = fh.read()
-    rec = re.search(r"shape = (\S+) zm_planes (\d+) zm_rows (\d+) xcd_remap (\d+)", text)
+    rec = re.search(r"shape = (\S+) zm_planes (\d+) zm_rows (\d+) xcd_remap (\d+)(?: zm_lds (\d+))?", text)
     m = re.search(r"traffic_bytes_per_launch = ([0-9.eE+]+)", text)
-    want = (size, str(shape["zm"]), str(shape["rows"]), str(shape["xcd"]))
-    if not rec or not m or rec.groups() != want:
+    want = (size, str(shape["zm"]), str(shape["rows"]), str(shape["xcd"]), str(shape.get("lds", 0)))
+    if not rec or not m or rec.groups()[:4] + (rec.group(5) or "0",) != want:
         return None
     return float(m.group(1))
 
@@ -84,14 +111,17 @@ def run_ensemble(steps, baseline_steps):
     log("ensemble: autotuner picked planes=%d rows=%d (%.3f ms) in %.1f s" % (planes, rows, tms, time.time() - t0))
     reps = {}
     walls = {}
+    kms = {}
     for prec, params in (("fp32", p32), ("fp64", p64)):
         with Batch(params, nuc) as bt:
             bt.set_tuning(abi.TUNE_ZM_PLANES, planes)
             bt.set_tuning(abi.TUNE_ZM_ROWS, rows)
             bt.set_tuning(abi.TUNE_GRAPH, 1)
+            bt.set_kernel_timing(True)
             t0 = time.perf_counter()
             reps[prec] = bt.step(steps, want_sums=True)
             walls[prec] = time.perf_counter() - t0
+            kms[prec], _ = bt.kernel_time_ms()
     ncell = [c.sbdim[0] * c.sbdim[1] * c.sbdim[2] for c in cfgs]
     drift = max(abs(r.solute_sum + r.solute_ledger - n * c.c_init) / (n * c.c_init)
                 for r, c, n in zip(reps["fp32"], cfgs, ncell))
@@ -111,6 +141,10 @@ def run_ensemble(steps, baseline_steps):
             "dtype": "f32-deviation storage, f64 arithmetic",
             "value": round(batched, 1), "unit": "Mcell-updates/s", "ms_per_step": round(walls["fp32"] / steps * 1e3, 4),
             "tile": {"zm_planes": planes, "zm_rows": rows}, "hip_graph": True,
+            "k_solute_ms": round(kms["fp32"], 4),
+            "roofline_frac": round(cells * B_ALG_FP32 / (kms["fp32"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            if kms["fp32"] > 0 else None,
+            "step_over_kernel": round(walls["fp32"] / steps * 1e3 / kms["fp32"], 3) if kms["fp32"] > 0 else None,
             "one_at_a_time_fp64": round(one, 1), "one_at_a_time_steps": baseline_steps,
             "speedup_vs_one_at_a_time_fp64": round(batched / one, 2),
             "batched_fp64": round(cells * steps / walls["fp64"] / 1e6, 1),
@@ -122,44 +156,62 @@ def run_ensemble(steps, baseline_steps):
 
 
 def run_fp32_variant(cfg, steps, warmup, shape):
-    """The headline workload with FP32-deviation storage: throughput at the
-    headline dt, and its error against FP64 after the same steps at half
-    that dt (at the explicit limit the partitioning update is unstable and
-    two precisions diverge like two chaotic runs)."""
+    """The headline workload with FP32-deviation storage: throughput, the
+    solute kernel's roofline fraction at 40 B per cell, and the error against
+    the FP64 run of the same steps."""
     import numpy as np
     from mumatscale_amd.lib import Subblock
     n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
-    res = {}
-    with Subblock(cfg.params(check_negative_ce=False, precision=abi.PREC_FP32DEV), cfg.nucleation()) as sb:
-        apply_shape(sb, shape)
-        sb.step(warmup)
-        sb.set_kernel_timing(True)
-        t0 = time.perf_counter()
-        sb.step(steps)
-        wall = time.perf_counter() - t0
-        k_ms, _ = sb.kernel_time_ms()
-    res = {"value": round(n * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
-           "k_solute_ms": round(k_ms, 4),
-           "roofline_frac": round(n * 40.0 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "bytes_per_cell": 40.0}
-    half = directional_config(*cfg.sbdim)
-    half.dt = cfg.dt * 0.5
     fields = {}
+    res = {}
     for prec in (abi.PREC_FP32DEV, abi.PREC_FP64):
-        with Subblock(half.params(check_negative_ce=False, precision=prec), half.nucleation()) as sb:
+        with Subblock(cfg.params(check_negative_ce=True, precision=prec), cfg.nucleation()) as sb:
             apply_shape(sb, shape)
+            sb.step(warmup)
+            sb.set_kernel_timing(True)
+            t0 = time.perf_counter()
             rep = sb.step(steps, want_sums=True)
+            wall = time.perf_counter() - t0
+            k_ms, _ = sb.kernel_time_ms()
             fields[prec] = (sb.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1], rep)
+            if prec == abi.PREC_FP32DEV:
+                res = {"value": round(n * steps / wall / 1e6, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+                       "k_solute_ms": round(k_ms, 4),
+                       "roofline_frac": round(n * B_ALG_FP32 / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "bytes_per_cell": B_ALG_FP32, "steps": steps, "warmup": warmup}
     ce32, r32 = fields[abi.PREC_FP32DEV]
     ce64, r64 = fields[abi.PREC_FP64]
+    total0 = n * cfg.c_init
     res["error_vs_fp64"] = {
-        "dt": half.dt, "steps": steps,
+        "dt": cfg.dt, "after_steps": warmup + steps,
         "solute_l2_rel": float(np.linalg.norm(ce32 - ce64) / np.linalg.norm(ce64)),
         "solid_fraction_abs": abs(r32.solid_sum - r64.solid_sum) / n,
         "grains_fp32": r32.num_grains - 1, "grains_fp64": r64.num_grains - 1,
-        "solute_drift_stored": (r32.solute_sum - n * cfg.c_init) / (n * cfg.c_init),
-        "solute_drift_with_ledger": (r32.solute_sum + r32.solute_ledger - n * cfg.c_init) / (n * cfg.c_init)}
+        "solute_drift_stored": (r32.solute_sum - total0) / total0,
+        "solute_drift_with_ledger": (r32.solute_sum + r32.solute_ledger - total0) / total0}
     return res
+
+
+def run_literal_dt(dims, steps, warmup, shape):
+    """The headline workload at the explicit limit dt = 0.9 dx^2 / (6 D_l)
+    itself: the step at which ce first turns negative (where the reference
+    CPU build stops, growth.c:163-165), and the throughput of the same steps
+    run on without that check (the reference GPU build's behaviour)."""
+    from mumatscale_amd.lib import Subblock
+    cfg = directional_config(*dims)
+    n = dims[0] * dims[1] * dims[2]
+    with Subblock(cfg.params(check_negative_ce=True), cfg.nucleation()) as sb:
+        apply_shape(sb, shape)
+        rep = sb.step(warmup + steps, raise_on_error=False)
+        failed = rep.failed_step if rep.status != abi.OK else None
+    with Subblock(cfg.params(check_negative_ce=False), cfg.nucleation()) as sb:
+        apply_shape(sb, shape)
+        sb.step(warmup)
+        t0 = time.perf_counter()
+        sb.step(steps)
+        wall = time.perf_counter() - t0
+    return {"dt": cfg.dt, "negative_ce_at_step": failed,
+            "value_without_check": round(n * steps / wall / 1e6, 1), "steps": steps, "warmup": warmup}
 
 
 def run_coupled(dims, steps, warmup, shape, nsub=(2, 2, 2)):
@@ -206,9 +258,9 @@ def run_adjoint(n=128, nseeds=16, steps=4000):
     the initial ce field along a recorded trajectory of the seeded-grain
     workload (n^3, `nseeds` seeds from seed 4, half the explicit dt), against
     FP64 central finite differences of the frozen-branch replay at
-    h = 1e-6, 1e-5 and 1e-4 relative (the smallest h shows the rounding
-    noise floor of the quotient for weakly coupled parameters).  The target is a free run with D_l, D_s, k scaled by
-    1.1, 0.8, 1.05."""
+    h = 1e-6, 1e-5 and 1e-4 relative, each numerator formed without
+    cancellation (adjoint.l2_loss_difference).  The target is a free run
+    with D_l, D_s, k scaled by 1.1, 0.8, 1.05."""
     import numpy as np
     import torch
     from mumatscale_amd import adjoint as ADJ
@@ -240,9 +292,12 @@ def run_adjoint(n=128, nseeds=16, steps=4000):
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
 
-    def loss(p, c0=None):
+    def ce_T(p, c0=None):
         (ce, _, _), _ = ADJ.replay(traj, *p, ce0=c0)
-        return ADJ.l2_loss(traj, ce, target).item()
+        return ce
+
+    def dloss(pa, pb, ca=None, cb=None):
+        return ADJ.l2_loss_difference(traj, ce_T(pa, ca), ce_T(pb, cb), target)
 
     names = ("d_liq", "d_sol", "part_coef")
     grad, fd, rel = {}, {}, {}
@@ -254,7 +309,7 @@ def run_adjoint(n=128, nseeds=16, steps=4000):
             a, b = list(base), list(base)
             a[i] = v * (1 + h)
             b[i] = v * (1 - h)
-            f = (loss(a) - loss(b)) / (a[i] - b[i])
+            f = dloss(a, b) / (a[i] - b[i])
             fd[names[i]]["%g" % h], rel[names[i]]["%g" % h] = f, abs(g - f) / abs(f)
     rng = np.random.default_rng(11)
     v = torch.zeros_like(traj.ce0)
@@ -262,14 +317,16 @@ def run_adjoint(n=128, nseeds=16, steps=4000):
     grad["ce0_random_direction"] = g = (ce0.grad * v).sum().item()
     fd["ce0_random_direction"], rel["ce0_random_direction"] = {}, {}
     for h in hs:
-        f = (loss(base, traj.ce0 + h * v) - loss(base, traj.ce0 - h * v)) / (2 * h)
+        f = dloss(base, base, traj.ce0 + h * v, traj.ce0 - h * v) / (2 * h)
         fd["ce0_random_direction"]["%g" % h] = f
         rel["ce0_random_direction"]["%g" % h] = abs(g - f) / abs(f)
     K = ADJ.checkpoint_interval(steps)
     P = traj.padded_cells
     return {"grid": "%d^3" % n, "seeds": nseeds, "grains": grains, "steps": steps, "dt": cfg.dt,
             "loss": L.item(), "checkpoint_interval": K,
-            "device_bytes": {"branch_record": steps * P, "checkpoints": ((steps + K - 1) // K) * 3 * 8 * P,
+            "fd_numerator": "sum of (a - b)(a + b - 2 t) over the interior, math.fsum",
+            "device_bytes": {"branch_record": steps * ADJ.record_bytes(traj.params),
+                             "checkpoints": ((steps + K - 1) // K) * 3 * 8 * P,
                              "segment_states": K * 3 * 8 * P},
             "record_s": round(t_rec, 3), "forward_s": round(t_fwd, 3), "backward_s": round(t_bwd, 3),
             "grad": grad, "central_fd": fd, "rel_err": rel,
@@ -338,16 +395,34 @@ def _split(dims, ranks):
     return nsub if left == 1 else None
 
 
+def _run_in_group(cmd, timeout, **kw):
+    """subprocess.run in a process group of its own.  The whole group
+    (mpiexec and the proxies it starts) is killed once the run returns or
+    times out, so no helper outlives it."""
+    import signal
+    import subprocess
+    p = subprocess.Popen(cmd, start_new_session=True, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                         text=True, **kw)
+    try:
+        out, err = p.communicate(timeout=timeout)
+    finally:
+        try:
+            os.killpg(p.pid, signal.SIGKILL)
+        except ProcessLookupError:
+            pass
+        if p.returncode is None:
+            p.communicate()
+    return subprocess.CompletedProcess(cmd, p.returncode, out, err)
+
+
 def reference_baseline(cfg, ranks, lo=2, hi=14, repeats=2):
     """The reference's own CPU build (oracle/_ref, built from its sources) on
     `ranks` MPI ranks of the host, one OpenMP thread each, the domain split
     into as many subblocks.
     Per-step time = (elapsed(hi steps) - elapsed(lo steps)) / (hi - lo), which
     cancels the set-up and the end-of-run grain statistics; each elapsed time
-    is the least of `repeats` runs.  Both step counts stay below the step
-    (about 18 on 8 ranks) at which this workload's Courant check stops the CPU
-    build.  None when the build or
-    mpiexec is missing or a run fails."""
+    is the least of `repeats` runs.  The runs use cfg's dt.  None when the
+    build or mpiexec is missing or a run fails."""
     import re
     import shutil
     import subprocess
@@ -362,15 +437,15 @@ def reference_baseline(cfg, ranks, lo=2, hi=14, repeats=2):
     with tempfile.TemporaryDirectory() as tmp:
         for steps in (lo, hi) * repeats:
             c = directional_config(*cfg.gdim)
-            c.finish_time = steps * c.dt
+            c.dt = cfg.dt
+            c.finish_time = (steps - 0.5) * c.dt      # done after exactly `steps` steps
             c.screenpfreq = 100000
             c.basefilename = "refbase"
             path = os.path.join(tmp, "run%d.in" % steps)
             write_ctrl(c, path, nsub=nsub)
             try:
-                r = subprocess.run([mpiexec, "-hosts", "127.0.0.1", "-n", str(ranks), ref, "-c", path], cwd=tmp,
-                                   env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True, text=True,
-                                   timeout=600)
+                r = _run_in_group([mpiexec, "-hosts", "127.0.0.1", "-n", str(ranks), ref, "-c", path], 600,
+                                  cwd=tmp, env=dict(os.environ, OMP_NUM_THREADS="1"))
             except (OSError, subprocess.SubprocessError) as e:
                 log("reference baseline: %s" % e)
                 return None
@@ -456,12 +531,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=str, default="256x256x512")
+    ap.add_argument("--precision", choices=("fp64", "fp32dev"), default="fp64",
+                    help="storage of the timed headline run (the line's metric is quoted on fp64)")
     ap.add_argument("--zm", type=int, default=128,
                     help="z-marching solute kernel, planes per workgroup (0: cell-parallel)")
     ap.add_argument("--zm-rows", type=int, default=16, help="rows per z-marching workgroup (4, 8, 16)")
     ap.add_argument("--xcd", type=int, default=1, help="1: contiguous tile runs per XCD")
+    ap.add_argument("--lds", type=int, default=0,
+                    help="1: z-marching kernel with its planes staged in LDS by direct loads (k_solute_zl)")
+    ap.add_argument("--capture-chunk", type=int, default=0,
+                    help=">0: capture deals its list slots in chunks of this many, pinned to XCDs")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the FP32-deviation and batched-ensemble measurements")
+                    help="skip the FP32-deviation, literal-dt, coupled, ensemble and adjoint measurements")
     ap.add_argument("--extra-steps", type=int, default=1000)
     ap.add_argument("--no-am", action="store_true",
                     help="skip the AM single track (FP32 against FP64, 1024x512x256)")
@@ -470,26 +551,34 @@ def main():
                     help="print the capture kernel's read floor at steps 250-1000 and exit")
     ap.add_argument("--coupled-only", action="store_true",
                     help="run only the 2x2x2 coupled-ranks leg (and its one-block comparison), print it and exit")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the batched-ensemble leg (--steps steps), print it and exit")
     args = ap.parse_args()
-    if args.capture_footprint:
-        capture_footprint()
-        return
-    if args.coupled_only:
-        dims = tuple(int(v) for v in args.size.split("x"))
-        shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd}
-        print(json.dumps(run_coupled(dims, args.extra_steps, args.warmup, shape)), flush=True)
-        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
-    nx, ny, nz = (int(v) for v in args.size.split("x"))
+    if args.capture_footprint:
+        capture_footprint()
+        return
+    dims = tuple(int(v) for v in args.size.split("x"))
+    shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd, "lds": args.lds, "chunk": args.capture_chunk}
+    if args.coupled_only:
+        print(json.dumps(run_coupled(dims, args.extra_steps, args.warmup, shape)), flush=True)
+        return
+    if args.ensemble_only:
+        print(json.dumps(run_ensemble(args.steps, 20)), flush=True)
+        return
+    nx, ny, nz = dims
     cfg = directional_config(nx, ny, nz)
+    cfg.dt *= 0.5                     # the stable dt (half the explicit limit)
+    fp32 = args.precision == "fp32dev"
+    prec = abi.PREC_FP32DEV if fp32 else abi.PREC_FP64
+    b_alg = B_ALG_FP32 if fp32 else B_ALG_FP64
 
     from mumatscale_amd.lib import Subblock
     t0 = time.time()
-    gpu = Subblock(cfg.params(check_negative_ce=False), cfg.nucleation(), device=0)
-    shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd}
+    gpu = Subblock(cfg.params(check_negative_ce=True, precision=prec), cfg.nucleation(), device=0)
     apply_shape(gpu, shape)
     log("gpu set-up %.1f s" % (time.time() - t0))
     n = nx * ny * nz
@@ -497,7 +586,7 @@ def main():
         gpu.step(args.warmup)
     gpu.set_kernel_timing(True)
     t0 = time.perf_counter()
-    rep = gpu.step(args.steps)        # synchronises on return
+    rep = gpu.step(args.steps)        # synchronises on return; raises on a negative ce
     t1 = time.perf_counter()
     gpu.set_kernel_timing(False)
     k_ms, k_n = gpu.kernel_time_ms()
@@ -505,19 +594,29 @@ def main():
     value = n * args.steps / wall / 1e6
     log("gpu: %d steps in %.3f s, grains %d, gindex %d, k_solute avg %.4f ms over %d launches"
         % (args.steps, wall, rep.num_grains, rep.gindex, k_ms, k_n))
-    achieved = n * B_ALG_FP64 / (k_ms * 1e-3) / 1e9 if k_ms > 0 else None
+    gpu.close()
+    probe = bandwidth_probe(n)
+    log("bandwidth probe:", probe)
+    achieved = n * b_alg / (k_ms * 1e-3) / 1e9 if k_ms > 0 else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": measured_traffic(args.size, shape),
-            "kernel": ("k_solute_zm<%d>" % args.zm_rows) if args.zm > 0 else "k_solute<true,true>",
-            "bytes_per_cell": B_ALG_FP64}
-    gpu.close()
-    extra = {}
+            "traffic": measured_traffic(args.size, shape, "fp32" if fp32 else "fp64"),
+            "kernel": ("k_solute_%s<%d>" % ("zl" if args.lds else "zm", args.zm_rows)) if args.zm > 0
+            else "k_solute<true,true>",
+            "bytes_per_cell": b_alg}
+    if probe and achieved:
+        mix = probe["solute_mix_fp32dev" if fp32 else "solute_mix_fp64"]["GB/s"]
+        roof["probe_same_mix_gbs"] = mix
+        roof["frac_of_probe"] = round(achieved / mix, 4)
+    extra = {"bandwidth_probe": probe}
     if not args.no_extra:
-        extra["fp32dev"] = run_fp32_variant(cfg, args.extra_steps, args.warmup, shape)
-        log("fp32dev:", extra["fp32dev"])
-        extra["coupled_ranks"] = run_coupled((nx, ny, nz), args.extra_steps, args.warmup, shape)
+        if not fp32:
+            extra["fp32dev"] = run_fp32_variant(cfg, args.extra_steps, args.warmup, shape)
+            log("fp32dev:", extra["fp32dev"])
+        extra["literal_dt"] = run_literal_dt(dims, args.extra_steps, args.warmup, shape)
+        log("literal dt:", extra["literal_dt"])
+        extra["coupled_ranks"] = run_coupled(dims, args.extra_steps, args.warmup, shape)
         log("coupled ranks:", extra["coupled_ranks"])
         extra["batched_ensemble"] = run_ensemble(3000, 200)
         log("ensemble:", extra["batched_ensemble"])
@@ -550,9 +649,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "directional solidification %dx%dx%d, plain FP64" % (nx, ny, nz),
+        "config": {"workload": "directional solidification %dx%dx%d, %s storage" % (nx, ny, nz,
+                                                                                  "FP32-deviation" if fp32
+                                                                                  else "plain FP64"),
                    "cells": n, "dt": cfg.dt, "zm_planes": args.zm, "zm_rows": args.zm_rows,
-                   "xcd_remap": args.xcd, "parallelism": "single GPU"},
+                   "xcd_remap": args.xcd, "zm_lds": args.lds, "capture_chunk": args.capture_chunk,
+                   "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

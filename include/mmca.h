@@ -177,6 +177,26 @@ const char *mmca_status_string(int status);
 int  mmca_device_count(int *count);
 
 /*
+ * y[i] = exp(x[i]) on `device` as the Gaussian-beam temperature evaluates it
+ * (temperature.c:108-112 calls the C library's exp 400 times per cell): a
+ * restatement of glibc's double exp, bit-identical with the host library.
+ * Host arrays; no counterpart in the reference (a check of that exp).
+ */
+int  mmca_eval_exp(int device, const double *x, double *y, int64_t n);
+
+/*
+ * Achievable HBM bandwidth for the solute kernel's field mixes: a streaming
+ * kernel with no arithmetic over `cells` cells of freshly allocated buffers,
+ * timed with HIP events over `reps` launches after one warm-up launch.
+ * mix 0: copy of one f64 field (16 B/cell); 1: the FP64 fields (read ce, cl,
+ * fs f64 and gr i32, write ce, cl, fs: 52 B/cell); 2: the FP32-deviation
+ * fields (read ce, cl f32, fs f64, gr i32, write ce, cl, fs: 36 B/cell).
+ * Returns the average ms per launch and the bytes per cell.  No counterpart
+ * in the reference (a measurement for the roofline report).
+ */
+int  mmca_bandwidth_probe(int device, int mix, int64_t cells, int reps, double *ms, double *bytes_per_cell);
+
+/*
  * allocateFields + SetupInitialConditions (setup.c:113-236, 42-96) and
  * tempPrepareSB (temperature.c:156-215): ce = cl = Cinit, thresholds = INF,
  * other fields zero, mold on padded faces.
@@ -266,6 +286,9 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
 #define MMCA_TUNE_ZM_ROWS    1   /* rows per z-marching workgroup: 4, 8 or 16 */
 #define MMCA_TUNE_XCD_REMAP  2   /* 1: workgroups of one XCD take contiguous tiles */
 #define MMCA_TUNE_GRAPH      3   /* batches: 1 = replay steps from a captured HIP graph */
+#define MMCA_TUNE_ZM_LDS     4   /* 1: z-marching kernel with its planes staged in LDS by direct loads */
+#define MMCA_TUNE_CAPTURE_CHUNK 5 /* >0: capture deals list slots in chunks of this many (rounded up to
+                                      256), chunk q on XCD q % 8; 0: 256-slot groups over the whole grid */
 int  mmca_set_tuning(mmca_ctx *ctx, int key, int value);
 int  mmca_get_tuning(mmca_ctx *ctx, int key, int *value);
 
@@ -352,12 +375,22 @@ int  mmca_batch_autotune(const mmca_params *params, const mmca_nucleation_cfg *n
 #define MMCA_BRANCH_REMELT  4   /* fs + dfs < 0: fs = 0, gr = 0, cl = ce           (:266-272) */
 
 /*
+ * Branch record of one step, 3 bits per cell: for each padded row (j, k),
+ * j = 0..ny+1, k = 0..nz+1, and each 64-cell segment s = 0..ceil(nx/64)-1 of
+ * its interior x, three uint64 words at word index
+ * ((k * (ny+2) + j) * ceil(nx/64) + s) * 3 + p; bit (x-1) % 64 of word p is
+ * bit p of the MMCA_BRANCH_* code of interior cell x.  Words of halo rows are
+ * not written.  mmca_record_bytes gives the bytes of one step.
+ */
+int  mmca_record_bytes(const mmca_params *params, int64_t *bytes_per_step);
+
+/*
  * Record the branch of every cell for the next n_steps steps of mmca_step or
- * mmca_doiteration into `codes` (device memory of n_steps * padded cells
- * bytes, step s of the recording at codes + s * padded cells, padded layout;
- * halo bytes are not written).  Recording uses the cell-parallel solute
- * kernel and FP64 storage (MMCA_ERR_ARG otherwise); mmca_step refuses to run
- * past the recorded range.  codes == NULL or n_steps == 0 stops recording.
+ * mmca_doiteration into `codes` (device memory of n_steps * mmca_record_bytes
+ * bytes, step s of the recording at codes + s * mmca_record_bytes).
+ * Recording uses the cell-parallel solute kernel and FP64 storage
+ * (MMCA_ERR_ARG otherwise); mmca_step refuses to run past the recorded
+ * range.  codes == NULL or n_steps == 0 stops recording.
  */
 int  mmca_set_record(mmca_ctx *ctx, uint8_t *codes, int64_t n_steps);
 
@@ -367,8 +400,9 @@ int  mmca_set_record(mmca_ctx *ctx, uint8_t *codes, int64_t n_steps);
  * (its temperature time is timestep * dt), `codes` its branch record.  The
  * constants come from `params` (d_liq, d_sol and part_coef may differ from
  * the recorded run).  Only interior cells of the outputs are written, and
- * the outputs may not be the input buffers.  `stream` is a hipStream_t
- * (NULL: the null stream).
+ * the outputs may not be the input buffers.  `codes` is the step's branch
+ * record (mmca_record_bytes bytes).  `stream` is a hipStream_t (NULL: the
+ * null stream).
  */
 int  mmca_replay_step(const mmca_params *params, uint64_t timestep, const uint8_t *codes,
                       const double *ce, const double *cl, const double *fs,

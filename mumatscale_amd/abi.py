@@ -51,6 +51,8 @@ TUNE_ZM_PLANES = 0
 TUNE_ZM_ROWS = 1
 TUNE_XCD_REMAP = 2
 TUNE_GRAPH = 3
+TUNE_ZM_LDS = 4
+TUNE_CAPTURE_CHUNK = 5
 
 # partitioning branch codes recorded for the adjoint (fs_change_diffuse, growth.c:221-273)
 BRANCH_LIQUID = 0

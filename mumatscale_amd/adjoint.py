@@ -42,6 +42,31 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def record_bytes(params):
+    """Bytes of one step's branch record (3 bits per cell, mmca_record_bytes)."""
+    n = C.c_int64(0)
+    _check(lib().mmca_record_bytes(C.byref(params), C.byref(n)), "mmca_record_bytes")
+    return n.value
+
+
+def unpack_codes(params, packed):
+    """Branch codes per padded cell, uint8 (nsteps, nz+2, ny+2, nx+2), from a
+    packed record (nsteps, record_bytes) on any device: interior cells get
+    their MMCA_BRANCH_* code, halo cells 0."""
+    nx, ny, nz = params.nx, params.ny, params.nz
+    w64 = (nx + 63) // 64
+    T = packed.shape[0]
+    words = packed.contiguous().view(torch.int64).view(T, nz + 2, ny + 2, w64, 3)
+    bit = torch.arange(64, device=packed.device, dtype=torch.int64)
+    out = torch.zeros((T, nz + 2, ny + 2, w64 * 64), dtype=torch.uint8, device=packed.device)
+    for p in range(3):
+        b = (words[..., p].unsqueeze(-1) >> bit) & 1
+        out |= (b.reshape(T, nz + 2, ny + 2, w64 * 64) << p).to(torch.uint8)
+    full = torch.zeros((T, nz + 2, ny + 2, nx + 2), dtype=torch.uint8, device=packed.device)
+    full[:, 1:-1, 1:-1, 1:-1] = out[:, 1:-1, 1:-1, :nx]
+    return full
+
+
 class Trajectory:
     """Branch record of `nsteps` steps after timestep `t0`, with the state
     (ce, cl, fs) at t0 and the parameters of the recorded run."""
@@ -49,7 +74,7 @@ class Trajectory:
     def __init__(self, params, t0, codes, ce0, cl0, fs0):
         self.params = params
         self.t0 = t0
-        self.codes = codes            # uint8 (nsteps, P), cuda
+        self.codes = codes            # uint8 (nsteps, record_bytes), cuda: 3 bits per cell
         self.ce0, self.cl0, self.fs0 = ce0, cl0, fs0
         self.shape = (params.nz + 2, params.ny + 2, params.nx + 2)
 
@@ -59,7 +84,11 @@ class Trajectory:
 
     @property
     def padded_cells(self):
-        return self.codes.shape[1]
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def unpacked_codes(self):
+        """The record as one uint8 code per padded cell (nsteps, P)."""
+        return unpack_codes(self.params, self.codes).view(self.nsteps, -1)
 
     def interior(self, field):
         """Interior view (nz, ny, nx) of a flattened padded field."""
@@ -68,12 +97,11 @@ class Trajectory:
 
 def record(sb, nsteps):
     """Advance `sb` (a FP64 Subblock) by nsteps while recording the branches."""
-    P = sb.padded_cells
     dev = torch.device("cuda", torch.cuda.current_device())
     ce0 = torch.from_numpy(sb.download(abi.FIELD_CE).reshape(-1)).to(dev)
     cl0 = torch.from_numpy(sb.download(abi.FIELD_CL).reshape(-1)).to(dev)
     fs0 = torch.from_numpy(sb.download(abi.FIELD_FS).reshape(-1)).to(dev)
-    codes = torch.zeros((nsteps, P), dtype=torch.uint8, device=dev)
+    codes = torch.zeros((nsteps, record_bytes(sb.params)), dtype=torch.uint8, device=dev)
     t0 = sb.timestep
     torch.cuda.synchronize()          # the library writes on its own stream
     sb.set_record(codes.data_ptr(), nsteps)
@@ -186,3 +214,16 @@ def l2_loss(traj, ce_T, target):
     """sum over the interior of (ce_T - target)^2 (flattened padded fields)."""
     d = traj.interior(ce_T) - traj.interior(target)
     return (d * d).sum()
+
+
+def l2_loss_difference(traj, ce_a, ce_b, target):
+    """l2_loss(ce_a) - l2_loss(ce_b), formed without cancellation: the sum
+    over the interior of (a - b)(a + b - 2 t), each term from the difference
+    of two nearby values, summed exactly (math.fsum).  For a central finite
+    difference of a weakly coupled parameter the two losses agree to many
+    digits, and their difference as two rounded sums would be mostly noise."""
+    a = traj.interior(ce_a).double()
+    b = traj.interior(ce_b).double()
+    t = traj.interior(target).double()
+    terms = ((a - b) * (a + b - 2.0 * t)).reshape(-1).cpu().numpy()
+    return math.fsum(terms.tolist())

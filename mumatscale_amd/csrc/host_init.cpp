@@ -141,6 +141,8 @@ int build_consts(const mmca_params &p, Consts &c)
     c.zm_planes = 64;                   // z-marching solute kernel by default
     c.zm_by = 16;
     c.xcd_remap = 1;
+    c.zm_lds = 0;
+    c.capture_chunk = 0;
     if (p.precision != MMCA_PREC_FP64 && p.precision != MMCA_PREC_FP32DEV)
         return MMCA_ERR_ARG;
     c.f32 = (p.precision == MMCA_PREC_FP32DEV) ? 1 : 0;

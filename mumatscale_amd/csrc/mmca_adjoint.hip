@@ -108,7 +108,7 @@ k_replay(Consts c, const unsigned char *code, double sim_time, const double *ce,
     }
     const double ce_old = ce[q.idx], conc = cl[q.idx], lspfs = fs[q.idx];
     const double ce_new = diffuse_cell(c, ce_old, conc, lspfs, cln, fsn, mo);
-    const int br = code[q.idx];
+    const int br = rec_code(c, code, q.x, q.j, q.k);
     double clv, fsv;
     if (br == MMCA_BRANCH_SOLID)
     {
@@ -162,7 +162,7 @@ k_adj_part(Consts c, Deriv dv, const unsigned char *code, double sim_time, const
         const double ce_old = ce[q.idx], conc = cl[q.idx], lspfs = fs[q.idx];
         const double ce_new = diffuse_cell(c, ce_old, conc, lspfs, cln, fsn, mo);
         const double ga = g_ce[q.idx], gb = g_cl[q.idx], ge = g_fs[q.idx];
-        const int br = code[q.idx];
+        const int br = rec_code(c, code, q.x, q.j, q.k);
         double lamv, gce_d = 0.0, gcl_d = 0.0, gfs_d = 0.0;
         if (br == MMCA_BRANCH_SOLID)
         {
@@ -298,6 +298,27 @@ k_adj_sum(const double *pa, const double *pb, long long nblk, double *g_param)
     }
 }
 
+// The launches of one adjoint step; lam holds P doubles, pa and pb 2 * nblk each.
+hipError_t
+adjoint_launch(const Consts &c, const Deriv &dv, uint64_t timestep, const uint8_t *codes, const double *ce,
+               const double *cl, const double *fs, const double *g_ce, const double *g_cl, const double *g_fs,
+               double *g_ce_in, double *g_cl_in, double *g_fs_in, double *g_param, double *lam, double *pa,
+               double *pb, hipStream_t s)
+{
+    const long long nblk = adj_blocks(c);
+    const double sim_time = timestep * c.dt;
+    const dim3 grid((unsigned) nblk), blk(ADJ_BX, ADJ_BY);
+    if (c.am)
+        hipLaunchKernelGGL(k_adj_part<true>, grid, blk, 0, s, c, dv, codes, sim_time, ce, cl, fs, g_ce, g_cl, g_fs,
+                           lam, g_ce_in, g_cl_in, g_fs_in, pa);
+    else
+        hipLaunchKernelGGL(k_adj_part<false>, grid, blk, 0, s, c, dv, codes, sim_time, ce, cl, fs, g_ce, g_cl,
+                           g_fs, lam, g_ce_in, g_cl_in, g_fs_in, pa);
+    hipLaunchKernelGGL(k_adj_diff, grid, blk, 0, s, c, dv, cl, fs, lam, g_ce_in, g_cl_in, g_fs_in, pb);
+    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(SUM_BS), 0, s, pa, pb, nblk, g_param);
+    return hipGetLastError();
+}
+
 int adj_consts(const mmca_params *p, Consts &c, Deriv &dv)
 {
     if (!p)
@@ -342,6 +363,20 @@ int mmca_replay_step(const mmca_params *params, uint64_t timestep, const uint8_t
     return hipGetLastError() == hipSuccess ? MMCA_OK : MMCA_ERR_HIP;
 }
 
+int mmca_record_bytes(const mmca_params *params, int64_t *bytes_per_step)
+{
+    using namespace mmca;
+    Consts c;
+    Deriv dv;
+    int st = adj_consts(params, c, dv);
+    if (st != MMCA_OK)
+        return st;
+    if (!bytes_per_step)
+        return MMCA_ERR_ARG;
+    *bytes_per_step = (int64_t) sizeof(unsigned long long) * rec_words(c);
+    return MMCA_OK;
+}
+
 int mmca_adjoint_workspace_bytes(const mmca_params *params, int64_t *bytes)
 {
     using namespace mmca;
@@ -370,22 +405,14 @@ int mmca_adjoint_step(const mmca_params *params, uint64_t timestep, const uint8_
     if (!codes || !ce || !cl || !fs || !g_ce || !g_cl || !g_fs || !g_ce_in || !g_cl_in || !g_fs_in ||
         !g_param || !workspace)
         return MMCA_ERR_ARG;
-    hipStream_t s = (hipStream_t) stream;
     const long long nblk = adj_blocks(c);
     double *lam = (double *) workspace;
     double *pa = lam + c.P;
     double *pb = pa + 2 * nblk;
-    const double sim_time = timestep * c.dt;
-    const dim3 grid((unsigned) nblk), blk(ADJ_BX, ADJ_BY);
-    if (c.am)
-        hipLaunchKernelGGL(k_adj_part<true>, grid, blk, 0, s, c, dv, codes, sim_time, ce, cl, fs, g_ce, g_cl, g_fs,
-                           lam, g_ce_in, g_cl_in, g_fs_in, pa);
-    else
-        hipLaunchKernelGGL(k_adj_part<false>, grid, blk, 0, s, c, dv, codes, sim_time, ce, cl, fs, g_ce, g_cl,
-                           g_fs, lam, g_ce_in, g_cl_in, g_fs_in, pa);
-    hipLaunchKernelGGL(k_adj_diff, grid, blk, 0, s, c, dv, cl, fs, lam, g_ce_in, g_cl_in, g_fs_in, pb);
-    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(SUM_BS), 0, s, pa, pb, nblk, g_param);
-    return hipGetLastError() == hipSuccess ? MMCA_OK : MMCA_ERR_HIP;
+    return adjoint_launch(c, dv, timestep, codes, ce, cl, fs, g_ce, g_cl, g_fs, g_ce_in, g_cl_in, g_fs_in, g_param,
+                          lam, pa, pb, (hipStream_t) stream) == hipSuccess
+               ? MMCA_OK
+               : MMCA_ERR_HIP;
 }
 
 }  // extern "C"

@@ -18,7 +18,7 @@
 
 namespace mmca {
 hipError_t launch_nucleate(const View &v, int mode, int cur, hipStream_t s);
-hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, int fsc, int rec, int cur, int nxt,
+hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int lds, int diff, int fsc, int rec, int cur, int nxt,
                          hipStream_t s);
 hipError_t launch_mask_from_gr(const DevBufs &b, const Consts &c, hipStream_t s);
 hipError_t launch_list(const View &v, int cb, int capture, int decide_realloc, hipStream_t s);
@@ -34,6 +34,9 @@ hipError_t launch_reduce(const DevBufs &b, const Consts &c, int cb, double *out2
 hipError_t launch_fill_temp(const DevBufs &b, const Consts &c, double *out, hipStream_t s);
 hipError_t launch_grow_octahedra(const DevBufs &b, const Consts &c, int cb, hipStream_t s);
 hipError_t launch_fill_f64(double *p, long long n, double v, hipStream_t s);
+hipError_t launch_eval_exp(const double *x, double *y, long long n, hipStream_t s);
+hipError_t launch_probe(int mix, long long n, double *a, double *b, double *f, float *ea, float *eb, int *g,
+                        int grid, hipStream_t s);
 long long list_words(const Consts &c);
 int list_blocks(const Consts &c);
 int reduce_blocks(const Consts &c);
@@ -308,7 +311,7 @@ int enqueue_iteration(mmca_ctx *ctx, bool advance)
         HIPCHK(hipEventRecord(ctx->ev[ev0], ctx->stream));
     }
     const int rec = ctx->b.rec != nullptr;
-    HIPCHK(launch_solute(ctx->view, ctx->c.am, ctx->c.f32, rec ? 0 : zm_rows(ctx->c), 1, 1, rec, ctx->cur, nxt,
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, ctx->c.f32, rec ? 0 : zm_rows(ctx->c), ctx->c.zm_lds, 1, 1, rec, ctx->cur, nxt,
                          ctx->stream));
     if (ctx->timing) {
         HIPCHK(hipEventRecord(ctx->ev[ev0 + 1], ctx->stream));
@@ -366,6 +369,108 @@ int mmca_device_count(int *count)
     if (count)
         *count = n;
     return MMCA_OK;
+}
+
+int mmca_eval_exp(int device, const double *x, double *y, int64_t n)
+{
+    if (n < 0 || (n > 0 && (!x || !y)))
+        return MMCA_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return MMCA_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev)
+        return MMCA_ERR_ARG;
+    if (n == 0)
+        return MMCA_OK;
+    if (hipSetDevice(device) != hipSuccess)
+        return MMCA_ERR_HIP;
+    const int64_t chunk = std::min<int64_t>(n, (int64_t) 1 << 24);
+    double *dx = nullptr, *dy = nullptr;
+    int st = MMCA_OK;
+    if (hipMalloc(&dx, (size_t) chunk * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dy, (size_t) chunk * sizeof(double)) != hipSuccess)
+        st = MMCA_ERR_HIP;
+    for (int64_t o = 0; st == MMCA_OK && o < n; o += chunk)
+    {
+        const int64_t m = std::min(chunk, n - o);
+        if (hipMemcpy(dx, x + o, (size_t) m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            launch_eval_exp(dx, dy, (long long) m, nullptr) != hipSuccess ||
+            hipMemcpy(y + o, dy, (size_t) m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            st = MMCA_ERR_HIP;
+    }
+    if (dx)
+        (void) hipFree(dx);
+    if (dy)
+        (void) hipFree(dy);
+    return st;
+}
+
+int mmca_bandwidth_probe(int device, int mix, int64_t cells, int reps, double *ms, double *bytes_per_cell)
+{
+    static const double bpc[3] = { 16.0, 52.0, 36.0 };
+    if (mix < 0 || mix > 2 || cells <= 0 || reps <= 0 || !ms)
+        return MMCA_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return MMCA_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev || hipSetDevice(device) != hipSuccess)
+        return MMCA_ERR_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return MMCA_ERR_HIP;
+    const size_t n = (size_t) cells;
+    double *a = nullptr, *b = nullptr, *f = nullptr;
+    float *ea = nullptr, *eb = nullptr;
+    int *g = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = MMCA_OK;
+    bool ok = hipStreamCreate(&s) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess;
+    if (ok && mix != 2)
+        ok = hipMalloc(&a, n * sizeof(double)) == hipSuccess && hipMalloc(&b, n * sizeof(double)) == hipSuccess;
+    if (ok && mix != 0)
+        ok = hipMalloc(&f, n * sizeof(double)) == hipSuccess && hipMalloc(&g, n * sizeof(int)) == hipSuccess;
+    if (ok && mix == 2)
+        ok = hipMalloc(&ea, n * sizeof(float)) == hipSuccess && hipMalloc(&eb, n * sizeof(float)) == hipSuccess;
+    if (ok)
+        ok = (!a || hipMemsetAsync(a, 0, n * sizeof(double), s) == hipSuccess) &&
+             (!b || hipMemsetAsync(b, 0, n * sizeof(double), s) == hipSuccess) &&
+             (!f || hipMemsetAsync(f, 0, n * sizeof(double), s) == hipSuccess) &&
+             (!g || hipMemsetAsync(g, 0, n * sizeof(int), s) == hipSuccess) &&
+             (!ea || hipMemsetAsync(ea, 0, n * sizeof(float), s) == hipSuccess) &&
+             (!eb || hipMemsetAsync(eb, 0, n * sizeof(float), s) == hipSuccess);
+    // enough workgroups to fill every CU several times over
+    const int grid = (int) std::min<long long>((long long) prop.multiProcessorCount * 32,
+                                               ((long long) n + 255) / 256);
+    if (ok)
+        ok = launch_probe(mix, (long long) n, a, b, f, ea, eb, g, grid, s) == hipSuccess;   // warm-up
+    if (ok)
+        ok = hipEventRecord(e0, s) == hipSuccess;
+    for (int r = 0; ok && r < reps; ++r)
+        ok = launch_probe(mix, (long long) n, a, b, f, ea, eb, g, grid, s) == hipSuccess;
+    float t = 0.0f;
+    if (ok)
+        ok = hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+    if (!ok)
+        st = MMCA_ERR_HIP;
+    else
+    {
+        *ms = (double) t / reps;
+        if (bytes_per_cell)
+            *bytes_per_cell = bpc[mix];
+    }
+    for (void *p : { (void *) a, (void *) b, (void *) f, (void *) ea, (void *) eb, (void *) g })
+        if (p)
+            (void) hipFree(p);
+    if (e0)
+        (void) hipEventDestroy(e0);
+    if (e1)
+        (void) hipEventDestroy(e1);
+    if (s)
+        (void) hipStreamDestroy(s);
+    return st;
 }
 
 int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
@@ -709,7 +814,7 @@ int mmca_sb_diffuse_alloy_decentered(mmca_ctx *ctx)
     HIPCHK(hipMemcpyAsync(ctx->b.oce, ctx->b.ce, (size_t) ctx->c.P * sizeof(double),
                           hipMemcpyDeviceToDevice, ctx->stream));
     TRY(ensure_view(ctx));
-    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 1, 0, 0, ctx->cur, ctx->cur, ctx->stream));
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 0, 1, 0, 0, ctx->cur, ctx->cur, ctx->stream));
     ctx->have_oce = true;
     return check_status(ctx);
 }
@@ -724,7 +829,7 @@ int mmca_fs_change_diffuse(mmca_ctx *ctx)
         return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(ctx->device));
     TRY(ensure_view(ctx));
-    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 0, 1, 0, ctx->cur, ctx->cur, ctx->stream));
+    HIPCHK(launch_solute(ctx->view, ctx->c.am, 0, 0, 0, 0, 1, 0, ctx->cur, ctx->cur, ctx->stream));
     return check_status(ctx);
 }
 
@@ -1366,6 +1471,16 @@ int mmca_set_tuning(mmca_ctx *ctx, int key, int value)
         ctx->c.xcd_remap = value ? 1 : 0;
         ctx->view_dirty = true;
         return MMCA_OK;
+    case MMCA_TUNE_ZM_LDS:
+        ctx->c.zm_lds = value ? 1 : 0;
+        ctx->view_dirty = true;
+        return MMCA_OK;
+    case MMCA_TUNE_CAPTURE_CHUNK:
+        if (value < 0 || value > (1 << 24))
+            return MMCA_ERR_ARG;
+        ctx->c.capture_chunk = (value + 255) / 256 * 256;
+        ctx->view_dirty = true;
+        return MMCA_OK;
     default:
         return MMCA_ERR_ARG;
     }
@@ -1384,6 +1499,12 @@ int mmca_get_tuning(mmca_ctx *ctx, int key, int *value)
         return MMCA_OK;
     case MMCA_TUNE_XCD_REMAP:
         *value = ctx->c.xcd_remap;
+        return MMCA_OK;
+    case MMCA_TUNE_ZM_LDS:
+        *value = ctx->c.zm_lds;
+        return MMCA_OK;
+    case MMCA_TUNE_CAPTURE_CHUNK:
+        *value = ctx->c.capture_chunk;
         return MMCA_OK;
     default:
         return MMCA_ERR_ARG;
@@ -1424,7 +1545,7 @@ struct mmca_batch {
     View view;
     bool view_dirty = true;
     int cur = 0;
-    int zm_planes = 64, zm_by = 16, xcd_remap = 1;
+    int zm_planes = 64, zm_by = 16, xcd_remap = 1, zm_lds = 0, capture_chunk = 0;
     bool timing = false;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
@@ -1447,6 +1568,8 @@ int batch_view(mmca_batch *bt)
         c->c.zm_planes = bt->zm_planes;
         c->c.zm_by = bt->zm_by;
         c->c.xcd_remap = bt->xcd_remap;
+        c->c.zm_lds = bt->zm_lds;
+        c->c.capture_chunk = bt->capture_chunk;
         c->view_dirty = true;
     }
     TRY(build_view(bt->dom.data(), (int) bt->dom.size(), bt->xcd_remap, bt->zm_planes > 0 ? bt->zm_by : 0,
@@ -1488,7 +1611,7 @@ int batch_enqueue(mmca_batch *bt, int cur)
         }
         HIPCHK(hipEventRecord(bt->ev[e0], bt->stream));
     }
-    HIPCHK(launch_solute(bt->view, c0.am, c0.f32, bt->zm_planes > 0 ? bt->zm_by : 0, 1, 1, 0, cur, nxt,
+    HIPCHK(launch_solute(bt->view, c0.am, c0.f32, bt->zm_planes > 0 ? bt->zm_by : 0, bt->zm_lds, 1, 1, 0, cur, nxt,
                          bt->stream));
     if (bt->timing) {
         HIPCHK(hipEventRecord(bt->ev[e0 + 1], bt->stream));
@@ -1743,6 +1866,14 @@ int mmca_batch_set_tuning(mmca_batch *bt, int key, int value)
         break;
     case MMCA_TUNE_XCD_REMAP:
         bt->xcd_remap = value ? 1 : 0;
+        break;
+    case MMCA_TUNE_ZM_LDS:
+        bt->zm_lds = value ? 1 : 0;
+        break;
+    case MMCA_TUNE_CAPTURE_CHUNK:
+        if (value < 0 || value > (1 << 24))
+            return MMCA_ERR_ARG;
+        bt->capture_chunk = (value + 255) / 256 * 256;
         break;
     case MMCA_TUNE_GRAPH:
         bt->use_graph = value != 0;

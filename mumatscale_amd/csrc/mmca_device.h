@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mmca_exp.h"
 #include "mmca_internal.h"
 
 namespace mmca {
@@ -97,8 +98,10 @@ temp_model(const Consts &c, double sim_time, int x, int y, int z)
         double lt = (0.5 + i) * ldt;
         double Tlocal = sqrt(lalpha * lt) * (ls * ls + 4. * lalpha * lt);
         Tinteg += ldt *
-            exp(-((lx + lv * lt) * (lx + lv * lt) + ly * ly) / (ls * ls + 4. * lalpha * lt) -
-                (lz * lz / (4. * lalpha * lt))) / Tlocal;
+            exp_glibc(-((lx + lv * lt) * (lx + lv * lt) + ly * ly) / (ls * ls + 4. * lalpha * lt) -
+                          (lz * lz / (4. * lalpha * lt)),
+                      exp_tab) /
+            Tlocal;
     }
     double Tfront = c.lab * lpow * lalpha / (pi15 * c.lthcon);
     return c.liniT + Tfront * Tinteg;
@@ -109,10 +112,12 @@ temp_model(const Consts &c, double sim_time, int x, int y, int z)
 // order as temp_model<true>, so the same values):
 //   vlt[i] = lv * lt,  A[i] = ls*ls + 4.*lalpha*lt,
 //   Tl[i] = sqrt(lalpha*lt) * (ls*ls + 4.*lalpha*lt),
-//   Z[i] = lz*lz / (4.*lalpha*lt) for one plane (lz depends on z only).
+//   Z[i] = lz*lz / (4.*lalpha*lt) for one plane (lz depends on z only),
+// and a copy of the exp table, so that its lookups are LDS reads.
 constexpr int BEAM_NT = 400;
 struct BeamTab {
     double vlt[BEAM_NT], A[BEAM_NT], Tl[BEAM_NT], Z[BEAM_NT];
+    uint64_t exp_tab[EXP_TAB_WORDS];
 };
 
 __device__ __forceinline__ double beam_lalpha(const Consts &c) { return c.lthcon / (c.rho * c.lcp); }
@@ -139,11 +144,13 @@ beam_tab_terms(const Consts &c, BeamTab &t, int tid, int nthreads)
         t.A[i] = ls * ls + 4. * lalpha * lt;
         t.Tl[i] = sqrt(lalpha * lt) * (ls * ls + 4. * lalpha * lt);
     }
+    for (int i = tid; i < EXP_TAB_WORDS; i += nthreads)
+        t.exp_tab[i] = exp_tab[i];
 }
 
-// fills Z for padded plane z
+// fills Z (BEAM_NT values) for padded plane z
 __device__ __forceinline__ void
-beam_tab_plane(const Consts &c, BeamTab &t, int z, int tid, int nthreads)
+beam_tab_plane(const Consts &c, double *Z, int z, int tid, int nthreads)
 {
     const double lalpha = beam_lalpha(c);
     const double ldt = beam_ldt(c);
@@ -152,13 +159,19 @@ beam_tab_plane(const Consts &c, BeamTab &t, int z, int tid, int nthreads)
     for (int i = tid; i < BEAM_NT; i += nthreads)
     {
         const double lt = (0.5 + i) * ldt;
-        t.Z[i] = lz * lz / (4. * lalpha * lt);
+        Z[i] = lz * lz / (4. * lalpha * lt);
     }
 }
 
-// temp_model<true> on the tables (Z of plane z must be filled)
+__device__ __forceinline__ void
+beam_tab_plane(const Consts &c, BeamTab &t, int z, int tid, int nthreads)
+{
+    beam_tab_plane(c, t.Z, z, tid, nthreads);
+}
+
+// temp_model<true> on the tables; Z holds the plane factors of plane z
 __device__ __forceinline__ double
-beam_temp_tab(const Consts &c, const BeamTab &t, double sim_time, int x, int y, int z)
+beam_temp_tab(const Consts &c, const BeamTab &t, const double *Z, double sim_time, int x, int y, int z)
 {
     double rx = c.origin[0] + ((c.off[0] + x) + 0.5) * c.cell_size;
     double ry = c.origin[1] + ((c.off[1] + y) + 0.5) * c.cell_size;
@@ -193,9 +206,33 @@ beam_temp_tab(const Consts &c, const BeamTab &t, double sim_time, int x, int y, 
     for (int i = 0; i < BEAM_NT; i++)
     {
         const double u = lx + t.vlt[i];
-        Tinteg += ldt * exp(-(u * u + ly * ly) / t.A[i] - t.Z[i]) / t.Tl[i];
+        Tinteg += ldt * exp_glibc(-(u * u + ly * ly) / t.A[i] - Z[i], t.exp_tab) / t.Tl[i];
     }
     return c.liniT + Tfront * Tinteg;
+}
+
+// 64-bit words of one step's branch record
+__host__ __device__ inline long long
+rec_words(const Consts &c)
+{
+    return 3ll * c.w64 * (c.ny + 2) * (c.nz + 2);
+}
+
+// Branch record of one step: per padded row (j, k) and 64-cell segment of x,
+// three 64-bit planes; bit (x - 1) % 64 of plane p is bit p of the
+// MMCA_BRANCH_* code of cell x (3 bits per interior cell).
+__device__ __forceinline__ long long
+rec_word(const Consts &c, int seg, int j, int k)
+{
+    return (((long long) k * (c.ny + 2) + j) * c.w64 + seg) * 3;
+}
+
+__device__ __forceinline__ int
+rec_code(const Consts &c, const unsigned char *rec, int x, int j, int k)
+{
+    const unsigned long long *w = reinterpret_cast<const unsigned long long *>(rec) + rec_word(c, (x - 1) >> 6, j, k);
+    const int bit = (x - 1) & 63;
+    return (int) ((w[0] >> bit) & 1ull) | (int) (((w[1] >> bit) & 1ull) << 1) | (int) (((w[2] >> bit) & 1ull) << 2);
 }
 
 __device__ inline double

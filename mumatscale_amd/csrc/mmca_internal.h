@@ -26,6 +26,8 @@ struct Consts {
     int zm_planes;          // > 0: z-marching solute kernel with this many planes per workgroup
     int zm_by;              // rows per z-marching workgroup (4, 8, 16)
     int xcd_remap;          // 1: contiguous tile runs per XCD
+    int zm_lds;             // 1: z-marching kernel with its planes staged in LDS (k_solute_zl)
+    int capture_chunk;      // > 0: capture deals list slots in chunks of this many, pinned to XCDs
     int f32;                // 1: ce, cl stored as FP32 deviations from cinit
     // diffusion (growth.c:38-40)
     double rs, rl;
@@ -104,7 +106,8 @@ struct DevBufs {
     int nblk_red;
     Counters *cnt;
     // partitioning-branch record (caller's device memory; null when off):
-    // the step with timestep t goes to rec + (t - rec_t0 - 1) * P
+    // the step with timestep t goes to rec + (t - rec_t0 - 1) * rec_words(c) * 8
+    // bytes (3 bits per cell, layout of rec_code)
     unsigned char *rec;
     unsigned long long rec_t0;
     long long rec_steps;

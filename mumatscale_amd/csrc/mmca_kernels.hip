@@ -17,7 +17,8 @@
 //
 // Arithmetic follows the reference's operation order and is compiled with
 // -ffp-contract=off: FP64 results are bit-identical with the reference CPU
-// build (except the Gaussian-beam temperature, which calls the device exp).
+// build.  The Gaussian-beam temperature's exp is a restatement of the host C
+// library's (mmca_exp.h).
 
 #include <hip/hip_runtime.h>
 
@@ -642,6 +643,7 @@ k_solute(View V, int cur, int nxt)
     for (int k = k0; k <= k1; ++k)
     {
         int g_new = 0;
+        int br_rec = 0;
         if (active)
         {
             const long long idx = k * c.sxy + (long long) j * c.sx + x;
@@ -677,13 +679,7 @@ k_solute(View V, int cur, int nxt)
                 int g, branch;
                 partition_cell(c, [&]() { return temp_model<AM>(c, sim_time, x, j, k); }, ce_new, ce_prev,
                                conc, lspfs, g_old, &cl_out, &fs_out, &g, &branch);
-                if (REC)
-                {
-                    // branch record of this step (bounds: set by the host, checked here too)
-                    const long long rs = (long long) (G(b.cnt)->timestep - b.rec_t0) - 1;
-                    if (rs >= 0 && rs < b.rec_steps)
-                        G(b.rec)[rs * c.P + idx] = (unsigned char) branch;
-                }
+                br_rec = branch;
                 st_cl<F32>(b, c, nxt, idx, cl_out);
                 st_stream(fs_buf(b, nxt), idx, fs_out);
                 if (g != g_old)
@@ -696,6 +692,23 @@ k_solute(View V, int cur, int nxt)
             unsigned long long bits = __ballot(active && g_new > 0);
             if (threadIdx.x == 0 && rowok)
                 G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
+        }
+        if (REC)
+        {
+            // branch record of this step, three bit planes per 64-cell segment
+            // (bounds: set by the host, checked here too)
+            const unsigned long long p0 = __ballot(active && (br_rec & 1));
+            const unsigned long long p1 = __ballot(active && (br_rec & 2));
+            const unsigned long long p2 = __ballot(active && (br_rec & 4));
+            const long long rs = (long long) (G(b.cnt)->timestep - b.rec_t0) - 1;
+            if (threadIdx.x == 0 && rowok && rs >= 0 && rs < b.rec_steps)
+            {
+                MMCA_GLOBAL unsigned long long *w = (MMCA_GLOBAL unsigned long long *) b.rec +
+                                                    rs * rec_words(c) + rec_word(c, bx, j, k);
+                w[0] = p0;
+                w[1] = p1;
+                w[2] = p2;
+            }
         }
     }
     if (F32)
@@ -910,7 +923,7 @@ k_solute_zm(View V, int cur, int nxt)
                 c,
                 [&]() {
                     if constexpr (AM)
-                        return beam_temp_tab(c, *beam, sim_time, x, j, k);
+                        return beam_temp_tab(c, *beam, beam->Z, sim_time, x, j, k);
                     else
                         return temp_model<false>(c, sim_time, x, j, k);
                 },
@@ -938,6 +951,257 @@ k_solute_zm(View V, int cur, int nxt)
         plane(k, P1, P0);
     // status and step of a negative ce (any plane of the tile: the step is
     // the same for all)
+    if (c.check_neg && neg)
+        flag_negative_ce(cnt);
+    if (F32)
+    {
+        const double tot = block_sum<64 * BY>(lost, s_red);
+        if (lane == 0 && ty == 0 && tot != 0.0)
+            ledger_add(b, tile, tot);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z-marching fused kernel with its planes staged in LDS by direct global->LDS
+// loads (gfx950 global_load_lds_dword): k_solute_zl.
+//
+// A workgroup of 64 x BY threads owns a 64 x BY column tile and walks its
+// planes upward, as k_solute_zm does, with the same arithmetic per cell.  The
+// planes come through a ring of ZL_RING slots in LDS instead of registers:
+// each slot holds one plane's cl and fs over the tile with a one-cell halo in
+// x and y ((BY + 2) rows x 66), and its ce and gr over the tile.  The loads
+// write LDS directly, so a plane in flight costs no VGPRs, and two planes are
+// in flight while one is computed.  Every wave issues an equal share of a
+// slot's loads (64 lanes x 4 bytes each, per-lane offsets computed once: they
+// are the same for every plane), then waits for its own share of the plane it
+// needs next; the one barrier per plane makes every wave's share visible and
+// frees the slot of the plane before the last.
+//
+// The loads are issued from inline asm, which the compiler's wait-count pass
+// does not see: it would otherwise make every LDS read wait for all loads in
+// flight.  Their completion is waited for explicitly with vmcnt(N), where N
+// counts only loads of this ring issued after the awaited ones; any other
+// memory operation issued in between makes that wait stricter, never looser.
+// Every load address is clamped into the padded box of its field.
+constexpr int ZL_RING = 4;
+
+template <int BY, bool F32>
+struct ZlSlot {
+    static constexpr int E = F32 ? 4 : 8;          // bytes per stored ce, cl value
+    static constexpr int HR = BY + 2, HW = 66;     // halo'd rows and row width
+    static constexpr int pad64(int n) { return (n + 63) / 64 * 64; }
+    static constexpr int CL_DW = pad64(HR * HW * E / 4);
+    static constexpr int FS_DW = pad64(HR * HW * 2);
+    static constexpr int CE_DW = pad64(BY * 64 * E / 4);
+    static constexpr int GR_DW = pad64(BY * 64);
+    static constexpr int CL0 = 0, FS0 = CL_DW, CE0 = FS0 + FS_DW, GR0 = CE0 + CE_DW;
+    static constexpr int DW = GR0 + GR_DW;         // dwords per slot
+    static constexpr int NI = DW / 64;             // load instructions per slot
+    static constexpr int NPW = (NI + BY - 1) / BY; // per wave
+};
+
+// One dword per lane from global (sbase + voff) into LDS (lds + lane * 4).
+__device__ __forceinline__ void
+zl_load(const void *sbase, unsigned voff, unsigned lds)
+{
+    unsigned save;
+    asm volatile("s_mov_b32 %0, m0\n\t"
+                 "s_mov_b32 m0, %3\n\t"
+                 "global_load_lds_dword %1, %2\n\t"
+                 "s_mov_b32 m0, %0"
+                 : "=&s"(save)
+                 : "v"(voff), "s"(sbase), "s"(lds)
+                 : "memory");
+}
+
+template <int N>
+__device__ __forceinline__ void
+zl_wait()
+{
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+template <int BY, bool AM, bool F32>
+__global__ void __launch_bounds__(64 * BY)
+k_solute_zl(View V, int cur, int nxt)
+{
+    using L = ZlSlot<BY, F32>;
+    using T = typename Store<F32>::T;
+    static_assert(L::NPW * 3 + 16 < 64, "loads in flight must stay within the vmcnt range");
+    __shared__ __attribute__((aligned(16))) unsigned s_ring[ZL_RING * L::DW];
+    __shared__ double s_red[BY];
+    __shared__ int s_stop;
+    __shared__ double s_beam[AM ? sizeof(BeamTab) / sizeof(double) : 1];
+    __shared__ double s_z[AM ? 2 : 1][AM ? BEAM_NT : 1];      // plane factors of the beam, double-buffered
+    BeamTab *beam = reinterpret_cast<BeamTab *>(s_beam);
+    long long gtile = blockIdx.x;
+    if (V.xcd_remap)
+    {
+        const long long per = (V.tot_zm + 7) / 8;
+        gtile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    }
+    if (gtile >= V.tot_zm)
+        return;                                   // uniform over the workgroup
+    const int dm = find_dom(V.off_zm, V.ndom, gtile);
+    const long long tile = gtile - V.off_zm[dm];
+    const DevBufs b = V.dom[dm].b;
+    const Consts c = V.dom[dm].c;
+    const int kz = c.zm_planes;
+    Counters *cnt = b.cnt;
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        s_stop = (G(b.cnt)->status != MMCA_OK);
+    __syncthreads();
+    if (s_stop)
+        return;
+    const int lane = threadIdx.x;
+    const int ty = threadIdx.y;
+    const int wv = __builtin_amdgcn_readfirstlane(ty);
+    const int ntx = (c.nx + 63) / 64, nty = (c.ny + BY - 1) / BY;
+    const int bx = (int) (tile % ntx);
+    const int by = (int) ((tile / ntx) % nty);
+    const int bz = (int) (tile / ((long long) ntx * nty));
+    const int x0 = bx * 64 + 1, j0 = by * BY + 1;
+    const int x = x0 + lane;
+    const int j = j0 + ty;
+    const int k0 = bz * kz + 1;
+    const int k1 = min(c.nz, k0 + kz - 1);
+    const int n = k1 - k0 + 1;
+    const bool rowok = (j <= c.ny);
+    const bool active = rowok && (x <= c.nx);
+
+    // this wave's share of a slot: instruction ins[m] of the slot, from field
+    // region reg[m] at per-lane byte offset off[m] within a plane
+    const char *fbase[4] = { (const char *) (F32 ? (const void *) cl32_buf(b, cur) : (const void *) cl_buf(b, cur)),
+                             (const char *) fs_buf(b, cur),
+                             (const char *) (F32 ? (const void *) b.ce32 : (const void *) b.ce),
+                             (const char *) b.gr };
+    const long long fplane[4] = { c.sxy * L::E, c.sxy * 8, c.sxy * L::E, c.sxy * 4 };
+    const char *fb[L::NPW];                                // field base of each load
+    long long fstride[L::NPW];                             // and its bytes per plane
+    unsigned ldso[L::NPW];                                 // LDS byte offset within a slot
+    unsigned off[L::NPW];
+#pragma unroll
+    for (int m = 0; m < L::NPW; ++m)
+    {
+        const int i = min(wv + m * BY, L::NI - 1);        // waves past the end repeat the last load
+        ldso[m] = (unsigned) i * 256u;
+        const int d = i * 64 + lane;
+        int r, dw, esz, rows_halo;
+        if (d < L::FS0)
+            r = 0, dw = d - L::CL0, esz = L::E, rows_halo = 1;
+        else if (d < L::CE0)
+            r = 1, dw = d - L::FS0, esz = 8, rows_halo = 1;
+        else if (d < L::GR0)
+            r = 2, dw = d - L::CE0, esz = L::E, rows_halo = 0;
+        else
+            r = 3, dw = d - L::GR0, esz = 4, rows_halo = 0;
+        const int ru = __builtin_amdgcn_readfirstlane(r);   // uniform: a load lies in one region
+        fb[m] = (ru == 0) ? fbase[0] : (ru == 1) ? fbase[1] : (ru == 2) ? fbase[2] : fbase[3];
+        fstride[m] = (ru == 0) ? fplane[0] : (ru == 1) ? fplane[1] : (ru == 2) ? fplane[2] : fplane[3];
+        const int per = esz / 4;                          // dwords per value
+        const int width = rows_halo ? L::HW : 64;
+        const int nval = rows_halo ? L::HR * L::HW : BY * 64;
+        const int e = min(dw / per, nval - 1);            // padding dwords repeat the last value
+        const int half = dw - (dw / per) * per;
+        const int row = e / width, col = e - row * width;
+        const int gj = min(j0 - rows_halo + row, c.ny + 1);
+        const int gx = min(x0 - rows_halo + col, c.nx + 1);
+        off[m] = (unsigned) ((gj * c.sx + gx) * esz + half * 4);
+    }
+    const unsigned ring_lds = (unsigned) (unsigned long) (__attribute__((address_space(3))) unsigned *) s_ring;
+    // loads of plane sequence number s (plane k0 - 1 + s, clamped to k1 + 1) into slot s % ZL_RING
+    auto issue = [&](int s) {
+        const long long p = min(k0 - 1 + s, k1 + 1);
+        const unsigned slot = ring_lds + (unsigned) ((s % ZL_RING) * L::DW) * 4u;
+#pragma unroll
+        for (int m = 0; m < L::NPW; ++m)
+            zl_load(fb[m] + p * fstride[m], off[m], slot + ldso[m]);
+    };
+    auto cl_at = [&](int q, int row, int col) -> double {
+        const unsigned *sl = s_ring + q * L::DW + L::CL0;
+        return from_raw<F32>(c, reinterpret_cast<const T *>(sl)[row * L::HW + col]);
+    };
+    auto fs_at = [&](int q, int row, int col) -> double {
+        const unsigned *sl = s_ring + q * L::DW + L::FS0;
+        return reinterpret_cast<const double *>(sl)[row * L::HW + col];
+    };
+
+    const MMCA_GLOBAL double *fsc = fs_buf(b, cur);
+    (void) fsc;
+    MMCA_GLOBAL double *fsn_out = fs_buf(b, nxt);
+    const double sim_time = G(b.cnt)->timestep * c.dt;
+    const int mo_x1 = (x == c.nx) ? c.pad_hi[0] : 0, mo_x0 = (x == 1) ? c.pad_lo[0] : 0;
+    const int mo_y1 = (j == c.ny) ? c.pad_hi[1] : 0, mo_y0 = (j == 1) ? c.pad_lo[1] : 0;
+
+    issue(0);
+    issue(1);
+    issue(2);
+    issue(3);
+    if constexpr (AM)
+    {
+        beam_tab_terms(c, *beam, ty * 64 + lane, 64 * BY);
+        beam_tab_plane(c, s_z[1], k0, ty * 64 + lane, 64 * BY);
+    }
+    zl_wait<2 * L::NPW>();                       // planes k0 - 1 and k0 have landed
+    __syncthreads();
+    double clm = cl_at(0, ty + 1, lane + 1), fsm = fs_at(0, ty + 1, lane + 1);
+    double clv = cl_at(1, ty + 1, lane + 1), fsv = fs_at(1, ty + 1, lane + 1);
+    double lost = 0.0;
+    bool neg = false;
+    for (int s = 1; s <= n; ++s)
+    {
+        const int k = k0 - 1 + s;
+        const int q = s % ZL_RING, q1 = (s + 1) % ZL_RING;
+        zl_wait<L::NPW>();                       // this wave's share of plane k + 1 has landed
+        __syncthreads();                         // every wave's share; slot of plane k - 2 is free
+        issue(s + 3);
+        if constexpr (AM)
+            if (s < n)
+                beam_tab_plane(c, s_z[(s + 1) & 1], k + 1, ty * 64 + lane, 64 * BY);
+        const double clp = cl_at(q1, ty + 1, lane + 1), fsp = fs_at(q1, ty + 1, lane + 1);
+        int g_new = 0;
+        if (active)
+        {
+            const long long idx = k * c.sxy + (long long) j * c.sx + x;
+            const double clxm = cl_at(q, ty + 1, lane), fsxm = fs_at(q, ty + 1, lane);
+            const double clxp = cl_at(q, ty + 1, lane + 2), fsxp = fs_at(q, ty + 1, lane + 2);
+            const double clym = cl_at(q, ty, lane + 1), fsym = fs_at(q, ty, lane + 1);
+            const double clyp = cl_at(q, ty + 2, lane + 1), fsyp = fs_at(q, ty + 2, lane + 1);
+            const unsigned *sl = s_ring + q * L::DW;
+            const double ce_old = from_raw<F32>(c, reinterpret_cast<const T *>(sl + L::CE0)[ty * 64 + lane]);
+            const int g_old = reinterpret_cast<const int *>(sl + L::GR0)[ty * 64 + lane];
+            const double cln[6] = { clxp, clxm, clyp, clym, clp, clm };
+            const double fsn[6] = { fsxp, fsxm, fsyp, fsym, fsp, fsm };
+            const int mo[6] = { mo_x1, mo_x0, mo_y1, mo_y0, (k == c.nz) ? c.pad_hi[2] : 0,
+                                (k == 1) ? c.pad_lo[2] : 0 };
+            const double ce_new = diffuse_cell(c, ce_old, clv, fsv, cln, fsn, mo);
+            neg |= (ce_new < 0);
+            lost += st_ce<F32>(b, c, idx, ce_new);
+            double cl_out, fs_out;
+            int branch;
+            partition_cell(
+                c,
+                [&]() {
+                    if constexpr (AM)
+                        return beam_temp_tab(c, *beam, s_z[s & 1], sim_time, x, j, k);
+                    else
+                        return temp_model<false>(c, sim_time, x, j, k);
+                },
+                ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out, &g_new, &branch);
+            st_cl<F32>(b, c, nxt, idx, cl_out);
+            st_stream(fsn_out, idx, fs_out);
+            if (g_new != g_old)
+                G(b.gr)[idx] = g_new;
+        }
+        unsigned long long bits = __ballot(active && g_new > 0);
+        if (rowok && lane == 0)
+            G(b.mask)[((long long) k * (c.ny + 2) + j) * c.w64 + bx] = bits;
+        clm = clv;
+        fsm = fsv;
+        clv = clp;
+        fsv = fsp;
+    }
+    zl_wait<0>();                                // no load may land in LDS after the workgroup ends
     if (c.check_neg && neg)
         flag_negative_ce(cnt);
     if (F32)
@@ -1230,8 +1494,7 @@ k_capture(View V, int cb)
     const long long maxg = c.max_total_grains;
     MMCA_GLOBAL double *fsb = fs_buf(b, cb);
     MMCA_GLOBAL double *clb = cl_buf(b, cb);
-    for (long long t = blk * LIST_BS + threadIdx.x; t < nmem; t += nthr)
-    {
+    auto member = [&](long long t) {
         const int slot = (int) t + 1;
         const unsigned idx = (unsigned) G(b.diff_id)[slot];
         double tx, ty, tz;
@@ -1345,7 +1608,35 @@ k_capture(View V, int cb)
         G(b.dc)[3 * (long long) idx] = tx;
         G(b.dc)[3 * (long long) idx + 1] = ty;
         G(b.dc)[3 * (long long) idx + 2] = tz;
+    };
+    const long long nwg = V.off_list[dm + 1] - V.off_list[dm];
+    const int chunk = c.capture_chunk;
+    if (chunk > 0 && nwg >= 8)
+    {
+        // Contiguous chunks of `chunk` slots, chunk q on the XCD q % 8: the
+        // members of one chunk (about chunk / (nx * ny) planes of the
+        // interface) share most of their solid neighbours, whose lines then
+        // stay in that XCD's L2.  The hardware places workgroup w on XCD
+        // w % 8; the workgroups of this domain on one XCD stride over the
+        // 256-slot groups of that XCD's chunks.
+        const int S = chunk / LIST_BS;
+        const long long o = V.off_list[dm];
+        const int xcd = (int) (blockIdx.x % 8);
+        const long long first = o + ((xcd - (int) (o % 8)) + 8) % 8;
+        const long long rank = (blockIdx.x - first) / 8;
+        const long long nxcd = (o + nwg - 1 - first) / 8 + 1;
+        const long long nchunk = (nmem + chunk - 1) / chunk;
+        const long long mine = (nchunk > xcd) ? (nchunk - 1 - xcd) / 8 + 1 : 0;
+        for (long long u = rank; u < mine * S; u += nxcd)
+        {
+            const long long t = (xcd + 8 * (u / S)) * (long long) chunk + (u % S) * LIST_BS + threadIdx.x;
+            if (t < nmem)
+                member(t);
+        }
     }
+    else
+        for (long long t = blk * LIST_BS + threadIdx.x; t < nmem; t += nthr)
+            member(t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1534,7 +1825,7 @@ hipError_t launch_halo_exchange(const View &v, int phase, int cb, long long max_
 // zm_by: rows of the z-marching kernel (all domains of a view share it); 0
 // selects the cell-parallel kernel.
 template <bool AM, bool F32>
-static void launch_solute_t(const View &v, int zm_by, int diff, int fsc, int rec, int cur, int nxt,
+static void launch_solute_t(const View &v, int zm_by, int lds, int diff, int fsc, int rec, int cur, int nxt,
                             hipStream_t s)
 {
     dim3 blk(SOL_BX, SOL_BY);
@@ -1546,7 +1837,17 @@ static void launch_solute_t(const View &v, int zm_by, int diff, int fsc, int rec
                                cur, nxt);
         return;
     }
-    if (diff && fsc && zm_by > 0)
+    if (diff && fsc && zm_by > 0 && lds)
+    {
+        const long long ng = v.xcd_remap ? ((v.tot_zm + 7) / 8) * 8 : v.tot_zm;
+        if (zm_by == 16)
+            hipLaunchKernelGGL((k_solute_zl<16, AM, F32>), dim3((unsigned) ng), dim3(64, 16), 0, s, v, cur, nxt);
+        else if (zm_by == 8)
+            hipLaunchKernelGGL((k_solute_zl<8, AM, F32>), dim3((unsigned) ng), dim3(64, 8), 0, s, v, cur, nxt);
+        else
+            hipLaunchKernelGGL((k_solute_zl<4, AM, F32>), dim3((unsigned) ng), dim3(64, 4), 0, s, v, cur, nxt);
+    }
+    else if (diff && fsc && zm_by > 0)
     {
         const long long ng = v.xcd_remap ? ((v.tot_zm + 7) / 8) * 8 : v.tot_zm;
         if (zm_by == 16)
@@ -1568,22 +1869,22 @@ static void launch_solute_t(const View &v, int zm_by, int diff, int fsc, int rec
                            nxt);
 }
 
-hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int diff, int fsc, int rec, int cur, int nxt,
-                         hipStream_t s)
+hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int lds, int diff, int fsc, int rec, int cur,
+                         int nxt, hipStream_t s)
 {
     if (am)
     {
         if (f32)
-            launch_solute_t<true, true>(v, zm_by, diff, fsc, rec, cur, nxt, s);
+            launch_solute_t<true, true>(v, zm_by, lds, diff, fsc, rec, cur, nxt, s);
         else
-            launch_solute_t<true, false>(v, zm_by, diff, fsc, rec, cur, nxt, s);
+            launch_solute_t<true, false>(v, zm_by, lds, diff, fsc, rec, cur, nxt, s);
     }
     else
     {
         if (f32)
-            launch_solute_t<false, true>(v, zm_by, diff, fsc, rec, cur, nxt, s);
+            launch_solute_t<false, true>(v, zm_by, lds, diff, fsc, rec, cur, nxt, s);
         else
-            launch_solute_t<false, false>(v, zm_by, diff, fsc, rec, cur, nxt, s);
+            launch_solute_t<false, false>(v, zm_by, lds, diff, fsc, rec, cur, nxt, s);
     }
     return hipGetLastError();
 }
@@ -1674,6 +1975,100 @@ hipError_t launch_fill_f64(double *p, long long n, double v, hipStream_t s)
     if (n <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_fill_f64, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// Streaming probes of the bandwidth a kernel with the solute kernel's field
+// mix can reach: no arithmetic beyond a dependency that keeps every load
+// live.  Each thread takes PROBE_UNROLL cells a grid stride apart, issues all
+// of their loads, then all of their stores.
+//   mix 0: copy          read a f64, write b f64                         16 B/cell
+//   mix 1: FP64 fields   read ce, cl, fs f64 and gr i32; write ce, cl, fs 52 B/cell
+//   mix 2: FP32 fields   read ce, cl f32, fs f64, gr i32; write ce, cl, fs 36 B/cell
+constexpr int PROBE_UNROLL = 4;
+
+template <int MIX>
+__global__ void __launch_bounds__(256)
+k_probe(long long n, double *a, double *b, double *f, float *ea, float *eb, int *g)
+{
+    const long long stride = (long long) gridDim.x * blockDim.x;
+    long long i0 = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i0 < n; i0 += stride * PROBE_UNROLL)
+    {
+        double v0[PROBE_UNROLL], v1[PROBE_UNROLL], v2[PROBE_UNROLL];
+        int gv[PROBE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < PROBE_UNROLL; ++u)
+        {
+            const long long i = min(i0 + u * stride, n - 1);
+            if constexpr (MIX == 0)
+                v0[u] = G(a)[i];
+            else if constexpr (MIX == 1)
+            {
+                v0[u] = G(a)[i];
+                v1[u] = G(b)[i];
+                v2[u] = G(f)[i];
+                gv[u] = G(g)[i];
+            }
+            else
+            {
+                v0[u] = G(ea)[i];
+                v1[u] = G(eb)[i];
+                v2[u] = G(f)[i];
+                gv[u] = G(g)[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PROBE_UNROLL; ++u)
+        {
+            const long long i = i0 + u * stride;
+            if (i >= n)
+                break;
+            if constexpr (MIX == 0)
+                G(b)[i] = v0[u];
+            else if constexpr (MIX == 1)
+            {
+                G(a)[i] = v0[u] + gv[u];
+                G(b)[i] = v1[u];
+                G(f)[i] = v2[u];
+            }
+            else
+            {
+                G(ea)[i] = (float) v0[u] + gv[u];
+                G(eb)[i] = (float) v1[u];
+                G(f)[i] = v2[u];
+            }
+        }
+    }
+}
+
+hipError_t launch_probe(int mix, long long n, double *a, double *b, double *f, float *ea, float *eb, int *g,
+                        int grid, hipStream_t s)
+{
+    if (mix == 0)
+        hipLaunchKernelGGL(k_probe<0>, dim3(grid), dim3(256), 0, s, n, a, b, f, ea, eb, g);
+    else if (mix == 1)
+        hipLaunchKernelGGL(k_probe<1>, dim3(grid), dim3(256), 0, s, n, a, b, f, ea, eb, g);
+    else
+        hipLaunchKernelGGL(k_probe<2>, dim3(grid), dim3(256), 0, s, n, a, b, f, ea, eb, g);
+    return hipGetLastError();
+}
+
+// y[i] = exp(x[i]) through the beam quadrature's exp (mmca_exp.h), with the
+// table read from constant memory as temp_model<true> reads it
+__global__ void
+k_eval_exp(const double *x, double *y, long long n)
+{
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        y[i] = exp_glibc(x[i], exp_tab);
+}
+
+hipError_t launch_eval_exp(const double *x, double *y, long long n, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_eval_exp, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, x, y, n);
     return hipGetLastError();
 }
 

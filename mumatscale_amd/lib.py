@@ -42,6 +42,9 @@ def lib():
     L.mmca_status_string.restype = C.c_char_p
     L.mmca_status_string.argtypes = [C.c_int]
     L.mmca_device_count.argtypes = [i32p]
+    L.mmca_eval_exp.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64]
+    L.mmca_bandwidth_probe.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
     L.mmca_create.argtypes = [C.POINTER(abi.Params), C.c_int, C.POINTER(vp)]
     L.mmca_destroy.argtypes = [vp]
     L.mmca_init_nucleation.argtypes = [vp, C.POINTER(abi.NucleationCfg)]
@@ -92,6 +95,7 @@ def lib():
     dp = C.c_void_p
     L.mmca_replay_step.argtypes = [C.POINTER(abi.Params), C.c_uint64, vp, dp, dp, dp, dp, dp, dp, vp]
     L.mmca_adjoint_workspace_bytes.argtypes = [C.POINTER(abi.Params), C.POINTER(C.c_int64)]
+    L.mmca_record_bytes.argtypes = [C.POINTER(abi.Params), C.POINTER(C.c_int64)]
     L.mmca_adjoint_step.argtypes = [C.POINTER(abi.Params), C.c_uint64, vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
                                     vp, vp]
     if L.mmca_abi_version() != abi.ABI_VERSION:
@@ -132,6 +136,17 @@ def device_count():
     n = C.c_int32(0)
     lib().mmca_device_count(C.byref(n))
     return n.value
+
+
+def eval_exp(x, device=0):
+    """exp of every element of x on the GPU, as the Gaussian-beam temperature
+    evaluates it (bit-identical with the host C library's exp)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    f64p = C.POINTER(C.c_double)
+    _check(lib().mmca_eval_exp(device, x.ctypes.data_as(f64p), y.ctypes.data_as(f64p), x.size),
+           "mmca_eval_exp")
+    return y
 
 
 _FIELD_DTYPES = {

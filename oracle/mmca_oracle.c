@@ -1230,6 +1230,16 @@ int orc_dc_tmp_size(const orc_state *s) { return s->dc_tmp_size; }
 int orc_taint_nucleations(const orc_state *s) { return s->taint_nucleations; }
 uint8_t *orc_dc_taint(orc_state *s) { return s->dc_taint; }
 double *orc_dc_tmp(orc_state *s) { return s->dc_tmp; }
+/* y[i] = exp(x[i]) through the host C library: the checker of the device's
+ * restatement of it (temperature.c:108-112 calls this exp) */
+void
+orc_exp_array(const double *x, double *y, int64_t n)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+        y[i] = exp(x[i]);
+}
+
 size_t orc_sizeof_grain(void) { return sizeof(mmca_grain); }
 size_t orc_sizeof_params(void) { return sizeof(mmca_params); }
 size_t orc_sizeof_nuc_cfg(void) { return sizeof(mmca_nucleation_cfg); }

@@ -23,6 +23,15 @@ def build():
                           cwd=os.path.dirname(HERE))
 
 
+def host_exp(x):
+    """exp of every element through the host C library (glibc)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    f64p = C.POINTER(C.c_double)
+    lib().orc_exp_array(x.ctypes.data_as(f64p), y.ctypes.data_as(f64p), x.size)
+    return y
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -45,6 +54,8 @@ def lib():
                      "orc_grow_octahedra", "orc_capture"):
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = None
+        L.orc_exp_array.restype = None
+        L.orc_exp_array.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64]
         L.orc_field.restype = vp
         L.orc_field.argtypes = [vp, C.c_int]
         L.orc_diff_id.restype = C.POINTER(C.c_int)

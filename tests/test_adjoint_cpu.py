@@ -117,3 +117,28 @@ def test_torch_gradient_matches_central_differences():
           loss(cfg.d_liq, cfg.d_sol, cfg.part_coef, init[0] - eps * v)).item() / (2 * eps)
     g = (ce0.grad * v).sum().item()
     assert abs(g - fd) <= 1e-6 * abs(fd), (g, fd)
+
+
+def test_unpack_codes_follows_the_record_layout():
+    """mumatscale_amd.adjoint.unpack_codes reads the 3-bit record layout that
+    mmca.h states (three uint64 bit planes per 64-cell row segment), packed
+    here independently from per-cell codes."""
+    from mumatscale_amd import adjoint as ADJ
+    from mumatscale_amd.ctrl import read_ctrl
+    from golden_cases import ctrl_path
+    for nx, ny, nz in ((70, 5, 3), (64, 2, 2), (17, 3, 4)):
+        p = read_ctrl(ctrl_path("c4_remelt_renucleate")).params()
+        p.nx, p.ny, p.nz = nx, ny, nz
+        rng = np.random.default_rng(nx)
+        T = 3
+        codes = np.zeros((T, nz + 2, ny + 2, nx + 2), np.uint8)
+        codes[:, 1:-1, 1:-1, 1:-1] = rng.integers(0, 5, size=(T, nz, ny, nx))
+        w64 = (nx + 63) // 64
+        words = np.zeros((T, nz + 2, ny + 2, w64, 3), np.uint64)
+        for t, k, j, x in zip(*np.nonzero(codes)):
+            seg, bit = (x - 1) // 64, (x - 1) % 64
+            for pl in range(3):
+                if (int(codes[t, k, j, x]) >> pl) & 1:
+                    words[t, k, j, seg, pl] |= np.uint64(1) << np.uint64(bit)
+        packed = torch.from_numpy(words.view(np.uint8).reshape(T, -1).copy())
+        assert np.array_equal(ADJ.unpack_codes(p, packed).numpy(), codes)

@@ -54,7 +54,7 @@ def test_record_and_replay_match_oracle(name, make, pre, steps):
     assert st == abi.OK
     final = [o.download(f) for f in FIELDS]
     sb, traj = gpu_record(cfg, pre, steps)
-    codes = traj.codes.cpu().numpy().reshape(rec.shape)
+    codes = traj.unpacked_codes().cpu().numpy().reshape(rec.shape)
     assert np.array_equal(codes, rec)
     for f, ref in zip(FIELDS, final):
         assert np.array_equal(sb.download(f), ref)
@@ -67,7 +67,7 @@ def test_record_and_replay_match_oracle(name, make, pre, steps):
 
 def test_record_refuses_fp32_and_overrun():
     cfg = seeded_grains_config(n=16, nseeds=1, seed=4)
-    codes = torch.zeros((4, (16 + 2) ** 3), dtype=torch.uint8, device="cuda")
+    codes = torch.zeros((4, ADJ.record_bytes(cfg.params(check_negative_ce=False))), dtype=torch.uint8, device="cuda")
     with Subblock(cfg.params(check_negative_ce=False, precision=abi.PREC_FP32DEV), cfg.nucleation()) as sb:
         with pytest.raises(abi.MmcaError):
             sb.set_record(codes.data_ptr(), 4)
@@ -98,7 +98,7 @@ def test_adjoint_matches_torch_autograd():
     shp = traj.shape
     y = [torch.tensor(v, dtype=torch.float64, requires_grad=True) for v in (cfg.d_liq, cfg.d_sol, cfg.part_coef)]
     ce0c = traj.ce0.cpu().view(shp).clone().requires_grad_(True)
-    out = AT.run(cfg.params(), traj.codes.cpu().view((traj.nsteps,) + shp), traj.t0, ce0c,
+    out = AT.run(cfg.params(), traj.unpacked_codes().cpu().view((traj.nsteps,) + shp), traj.t0, ce0c,
                  traj.cl0.cpu().view(shp), traj.fs0.cpu().view(shp), *y)
     assert torch.equal(out[0].detach().reshape(-1), ceT.detach().cpu())
     d = (out[0] - tgt_cpu.view(shp))[1:-1, 1:-1, 1:-1]
@@ -131,21 +131,20 @@ def test_adjoint_matches_central_differences():
     L = ADJ.l2_loss(traj, ADJ.solute_map(traj, *x, ce0), target)
     L.backward()
 
-    def loss(p, c0=None):
+    def ce_T(p, c0=None):
         (ce, _, _), _ = ADJ.replay(traj, *p, ce0=c0)
-        return ADJ.l2_loss(traj, ce, target).item()
+        return ce
 
-    # h = 1e-6 relative, except for D_s: the loss is ~4e4 times less
-    # sensitive to D_s than to D_l (solid diffusion), so at h = 1e-6 the two
-    # perturbed losses differ by ~5e-8 of a loss of ~1e2 and the quotient
-    # carries ~1e-6 of rounding noise from the replays (measured 1.3e-6);
-    # at h = 1e-4 that noise is 1e-8 and the O(h^2) truncation is below it.
+    # h = 1e-6 relative for every parameter.  The loss is ~4e4 times less
+    # sensitive to D_s than to D_l (solid diffusion): the two perturbed losses
+    # agree to ~5e-8 of their value, so the numerator is formed without
+    # cancellation, as the exact sum of (a - b)(a + b - 2 t) over the cells.
     for i, v in enumerate(base):
-        h = 1e-4 if i == 1 else 1e-6
+        h = 1e-6
         a, b = list(base), list(base)
         a[i] = v * (1 + h)
         b[i] = v * (1 - h)
-        fd = (loss(a) - loss(b)) / (a[i] - b[i])
+        fd = ADJ.l2_loss_difference(traj, ce_T(a), ce_T(b), target) / (a[i] - b[i])
         g = x[i].grad.item()
         print("param", i, "h", h, "adjoint", g, "fd", fd, "rel", abs(g - fd) / abs(fd), flush=True)
         assert abs(g - fd) <= 1e-6 * abs(fd), (i, g, fd)
@@ -153,7 +152,7 @@ def test_adjoint_matches_central_differences():
     v = torch.zeros_like(traj.ce0)
     traj.interior(v)[...] = torch.from_numpy(rng.normal(size=traj.interior(v).shape)).cuda()
     eps = 1e-6
-    fd = (loss(base, traj.ce0 + eps * v) - loss(base, traj.ce0 - eps * v)) / (2 * eps)
+    fd = ADJ.l2_loss_difference(traj, ce_T(base, traj.ce0 + eps * v), ce_T(base, traj.ce0 - eps * v), target) / (2 * eps)
     g = (ce0.grad * v).sum().item()
     print("ce0 direction adjoint", g, "fd", fd, flush=True)
     assert abs(g - fd) <= 1e-6 * abs(fd), (g, fd)

@@ -20,13 +20,14 @@ def bits(a):
 
 
 @pytest.mark.parametrize("graph", [0, 1])
-@pytest.mark.parametrize("zm", [0, 16])
-def test_batch_matches_oracle(graph, zm):
+@pytest.mark.parametrize("zm,lds", [(0, 0), (16, 0), (16, 1)])
+def test_batch_matches_oracle(graph, zm, lds):
     cfgs = [load_case(c)[0] for c in CASES]
     orcs = [OracleSubblock(c.params(), c.nucleation()) for c in cfgs]
     with Batch([c.params() for c in cfgs], [c.nucleation() for c in cfgs]) as bt:
         bt.set_tuning(abi.TUNE_GRAPH, graph)
         bt.set_tuning(abi.TUNE_ZM_PLANES, zm)
+        bt.set_tuning(abi.TUNE_ZM_LDS, lds)
         done = 0
         for t in (1, 2, 7, 40, 121, 400, 1000):
             reps = bt.step(t - done)

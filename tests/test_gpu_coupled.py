@@ -5,9 +5,8 @@ fs, gr and dc, its interface list and the grain table must hash the same as
 the reference's after every checked step: halos cross the subblock faces
 as ExchangeFacesForVar exchanges them (calculate.c:227-330), each rank draws
 sites and orientations from its own stream, and new grains get ids in rank
-order (grain.c:841-916).  The Gaussian-beam case is bit-exact until the beam
-turns on (the device exp differs from glibc's in the last bit); after that
-its grain counts and list lengths are compared.
+order (grain.c:841-916).  The Gaussian-beam case is bit-exact before and
+after the beam turns on (the device's exp gives glibc's bits).
 
 dc is left out for a rank from the first step at which its list length
 equals its dc_tmp capacity: there the reference reads the slot one past its
@@ -57,8 +56,6 @@ def test_coupled_batch_matches_reference_ranks(case):
     cfg, rows, expected = load_case_mr(case)
     n = expected["ranks"]
     last = expected["last_step"]
-    am = cfg.thermal_ana != 0
-    beam_on = int(1e-4 / cfg.dt) if am else None
     params = [cfg.params(check_negative_ce=True, whole_domain=False, sb_coords=rank_coords(cfg, r))
               for r in range(n)]
     nucs = [cfg.nucleation(rank=r) for r in range(n)]
@@ -77,8 +74,6 @@ def test_coupled_batch_matches_reference_ranks(case):
             for r, dom in enumerate(bt.domains):
                 ref = rows[r][t]
                 assert (ngr, len(dom.interface_list())) == (ref[0], ref[1]), (case, t, r)
-                if am and t > beam_on:
-                    continue
                 got = rank_state(dom, ngr, grains)
                 bad = [name for name, a, b in zip(IHASH_NAMES, got, ref[2:]) if a != b]
                 if taint[r] is not None and t >= taint[r] and bad == ["dc"]:

@@ -49,12 +49,14 @@ def test_fp32dev_fields_do_not_depend_on_tile_shape():
     cfg, _, _, _ = load_case("c1_directional")
     p32 = cfg.params(precision=abi.PREC_FP32DEV)
     out = []
-    for planes, rows, xcd in ((16, 16, 1), (7, 4, 0), (32, 8, 1), (0, 4, 0)):
-        print("fp32 solute kernel shape", planes, rows, xcd, flush=True)
+    for planes, rows, xcd, lds in ((16, 16, 1, 0), (7, 4, 0, 0), (32, 8, 1, 0), (0, 4, 0, 0), (16, 16, 1, 1),
+                                   (7, 4, 0, 1), (32, 8, 1, 1)):
+        print("fp32 solute kernel shape", planes, rows, xcd, lds, flush=True)
         with Subblock(p32, cfg.nucleation()) as sb:
             sb.set_tuning(abi.TUNE_ZM_PLANES, planes)
             sb.set_tuning(abi.TUNE_ZM_ROWS, rows)
             sb.set_tuning(abi.TUNE_XCD_REMAP, xcd)
+            sb.set_tuning(abi.TUNE_ZM_LDS, lds)
             rep = sb.step(300, want_sums=True)
             out.append(([sb.download(f) for f in (abi.FIELD_CE, abi.FIELD_CL, abi.FIELD_FS, abi.FIELD_GR)], rep))
     f0, r0 = out[0]

@@ -68,26 +68,31 @@ def checkpoints(last):
     return pts
 
 
-BIT_EXACT_CASES = [c for c in CASES if c != "c3_am_beam"]
+# every golden case, the Gaussian-beam one included (its exp is glibc's, mmca_exp.h)
+BIT_EXACT_CASES = list(CASES)
 
 
 # solute kernel variants (results must not depend on them): planes per
-# z-marching workgroup (0 = cell-parallel kernel), rows per workgroup, XCD remap
-VARIANTS = [(0, 4, 0), (8, 8, 1), (5, 16, 0), (16, 4, 1)]
+# z-marching workgroup (0 = cell-parallel kernel), rows per workgroup, XCD
+# remap, planes staged in LDS by direct loads (k_solute_zl)
+VARIANTS = [(0, 4, 0, 0), (8, 8, 1, 0), (5, 16, 0, 0), (16, 4, 1, 0), (8, 8, 1, 1), (5, 16, 0, 1), (16, 4, 1, 1),
+            (1, 16, 1, 1)]
 
 
-@pytest.mark.parametrize("variant", VARIANTS, ids=["cell", "zm8x8", "zm5x16", "zm16x4"])
+@pytest.mark.parametrize("variant", VARIANTS, ids=["cell", "zm8x8", "zm5x16", "zm16x4", "zl8x8", "zl5x16", "zl16x4",
+                                                   "zl1x16"])
 @pytest.mark.parametrize("case", BIT_EXACT_CASES)
 def test_gpu_matches_oracle(case, variant):
     cfg, hashes, expected, _ = load_case(case)
     last = expected["last_step"]
-    zm, rows, xcd = variant
+    zm, rows, xcd, lds = variant
     print("case", case, "steps", last, "variant", variant, flush=True)
     orc = OracleSubblock(cfg.params(check_negative_ce=True), cfg.nucleation())
     with Subblock(cfg.params(check_negative_ce=True), cfg.nucleation()) as gpu:
         gpu.set_tuning(abi.TUNE_ZM_PLANES, zm)
         gpu.set_tuning(abi.TUNE_ZM_ROWS, rows)
         gpu.set_tuning(abi.TUNE_XCD_REMAP, xcd)
+        gpu.set_tuning(abi.TUNE_ZM_LDS, lds)
         assert_bits_equal(gpu.download(abi.FIELD_THRESHOLD), orc.field(abi.FIELD_THRESHOLD), "thresholds")
         done = 0
         for t in checkpoints(last):
@@ -153,30 +158,7 @@ def test_gpu_temperature_field_matches_oracle():
                 orc.temp_update()
                 a = gpu.download(abi.FIELD_TEMP)
                 b = orc.field(abi.FIELD_TEMP)
-                if case == "c3_am_beam":
-                    # device exp/sqrt against glibc's: last-bit differences only
-                    assert np.allclose(a, b, rtol=1e-13, atol=0), (case, t, np.max(np.abs(a - b)))
-                else:
-                    assert np.array_equal(bits(a), bits(b)), (case, t)
-
-
-def test_gpu_am_beam_exact_while_beam_off_then_close():
-    """Gaussian-beam case: bit-exact while the beam is off (t <= 1e-4 s,
-    temperature.c:102-103 makes T = T_sub exactly), then the device exp makes
-    last-bit differences; the run is compared on grain count and solid
-    fraction at the end (tolerance 1 %)."""
-    cfg, _, expected, _ = load_case("c3_am_beam")
-    orc = OracleSubblock(cfg.params(), cfg.nucleation())
-    with Subblock(cfg.params(), cfg.nucleation()) as gpu:
-        orc.step(99)
-        gpu.step(99)
-        assert_same_state(gpu, orc, "am step 99")
-        last = expected["last_step"]
-        orc.step(last - 99)
-        rep = gpu.step(last - 99, want_sums=True)
-        n = cfg.params().nx * cfg.params().ny * cfg.params().nz
-        assert abs(rep.num_grains - orc.num_grains) <= 0.01 * orc.num_grains
-        assert abs(rep.solid_sum / n - orc.solid_sum() / n) <= 0.01
+                assert np.array_equal(bits(a), bits(b)), (case, t)
 
 
 def test_gpu_solid_volume_and_sums():
@@ -190,24 +172,33 @@ def test_gpu_solid_volume_and_sums():
         assert gpu.solid_volume() == pytest.approx(orc.solid_sum() * cfg.cell_size ** 3, rel=1e-13)
 
 
-def test_gpu_full_size_directional_block():
-    """The 256x256x512 directional workload at full size: the first steps
-    bit for bit against the oracle, then total solute over 300 steps (zero
-    flux on every face: sum of ce is conserved up to rounding)."""
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("lds,chunk", [(1, 16384)], ids=["zl_capture16k"])
+def test_gpu_full_size_directional_block(lds, chunk):
+    """The 256x256x512 directional workload at full size and the stable dt
+    (half the explicit limit; at the literal limit the reference stops with a
+    negative ce within ~25 steps): bit for bit against the oracle at every
+    checkpoint up to step 1000 (ce, cl, fs, gr, dc, the interface list, the
+    grain table), and total solute conserved to 1e-10 relative (zero flux on
+    every face).  At this size the list grid has enough workgroups for the
+    XCD-pinned capture schedule, which the small golden blocks do not reach."""
     import bench
     cfg = bench.directional_config()
+    cfg.dt *= 0.5
     print("full size: oracle set-up", flush=True)
-    orc = OracleSubblock(cfg.params(check_negative_ce=False), cfg.nucleation())
-    with Subblock(cfg.params(check_negative_ce=False), cfg.nucleation()) as gpu:
+    orc = OracleSubblock(cfg.params(check_negative_ce=True), cfg.nucleation())
+    with Subblock(cfg.params(check_negative_ce=True), cfg.nucleation()) as gpu:
+        gpu.set_tuning(abi.TUNE_ZM_PLANES, 128)
+        gpu.set_tuning(abi.TUNE_ZM_LDS, lds)
+        gpu.set_tuning(abi.TUNE_CAPTURE_CHUNK, chunk)
         s0 = orc.solute_sum()
-        for t in (1, 2, 3):
-            orc.step(1)
-            gpu.step(1)
-            print("full size: step", t, flush=True)
-            for f in (abi.FIELD_CE, abi.FIELD_CL, abi.FIELD_FS, abi.FIELD_GR):
-                assert_bits_equal(gpu.download(f), orc.field(f), "full size step %d field %d" % (t, f))
-            assert gpu.num_grains == orc.num_grains
-            assert np.array_equal(gpu.interface_list(), orc.diff_id())
+        done = 0
+        for t in (1, 2, 3, 50, 200, 400, 700, 1000):
+            assert orc.step(t - done) == abi.OK
+            rep = gpu.step(t - done, want_sums=True)
+            done = t
+            print("full size: step", t, "grains", orc.num_grains, "list", orc.gindex, flush=True)
+            assert_same_state(gpu, orc, "full size step %d" % t, check_d=(t in (200, 1000)))
+            assert abs(rep.solute_sum - s0) <= 1e-10 * abs(s0), (t, rep.solute_sum, s0)
+        assert orc.num_grains > 100
         orc.close()
-        rep = gpu.step(297, want_sums=True)
-        assert abs(rep.solute_sum - s0) <= 1e-10 * abs(s0), (rep.solute_sum, s0)

@@ -1,0 +1,159 @@
+"""The additive-manufacturing track with FP32-deviation storage and the
+batched ensemble, on the GPU.
+
+AM track (Gaussian-beam temperature, temperature.c:62-123, over a Voronoi
+substrate): the FP64 run is bit-identical with the oracle; the FP32-deviation
+run (ce, cl stored as FP32 deviations from c0, every other value and all
+arithmetic FP64) closes its solute balance with the ledger to 1e-12, has the
+FP64 run's grain count, and stays within stated bounds of its solid fraction
+and solute field.  Both at a block the oracle finishes in seconds (300 steps,
+beam melting and regrowing part of the substrate) and at the full
+1024 x 512 x 256 track for 40 steps (the beam turns on at step 16).
+
+Ensemble (96 domains of sides in {32..128}, 3 K mean bulk undercooling,
+uniform cooling rates in [1, 1000] K/s): the FP64 batch under HIP-graph
+replay with the autotuned solute-kernel shape equals every domain's own
+oracle run bit for bit; the FP32-deviation batch has every domain's grain
+count and closes every domain's solute balance.
+"""
+import numpy as np
+import pytest
+
+from mumatscale_amd import abi
+from mumatscale_amd.lib import Batch, Subblock, autotune_batch
+from mumatscale_amd.workloads import am_track_config, apply_substrate, ensemble_configs, voronoi_substrate
+from oracle.oracle import OracleSubblock
+
+pytestmark = pytest.mark.gpu
+
+AUTOTUNE_SHAPES = {(0, 4), (8, 4), (16, 4), (8, 8), (16, 8), (32, 8), (64, 8), (16, 16), (32, 16), (64, 16)}
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.int64) if a.dtype == np.float64 else a
+
+
+def _fp32_errors(c64, r64, c32, r32, n, c_init):
+    return {"l2": float(np.linalg.norm(c32 - c64) / np.linalg.norm(c64)),
+            "fs": abs(r32.solid_sum - r64.solid_sum) / n,
+            "ledger": abs(r32.solute_sum + r32.solute_ledger - n * c_init) / (n * c_init),
+            "grains": (r32.num_grains, r64.num_grains)}
+
+
+def small_track():
+    """A 96 x 48 x 32 block (dx 0.5 um) under a 30 W beam moving at 0.1 m/s
+    along x: it melts a pool in the 60-grain substrate that moves through
+    the block and regrows behind it within 300 steps."""
+    cfg = am_track_config(nx=96, ny=48, nz=32)
+    cfg.laser["power"] = 30.0
+    cfg.laser["speed"] = 0.1
+    return cfg
+
+
+def test_am_track_fp64_bit_exact_and_fp32_bounds():
+    cfg = small_track()
+    gr, recs = voronoi_substrate(cfg, 60, 2)
+    n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
+    p64 = cfg.params(check_negative_ce=False)
+    p32 = cfg.params(check_negative_ce=False, precision=abi.PREC_FP32DEV)
+    orc = OracleSubblock(p64, cfg.nucleation())
+    orc.set_grains(recs)
+    orc.upload(abi.FIELD_GR, gr)
+    fs = np.zeros(gr.shape)
+    fs[1:-1, 1:-1, 1:-1] = 1.0
+    orc.upload(abi.FIELD_FS, fs)
+    melted = 0
+    with Subblock(p64, cfg.nucleation()) as a, Subblock(p32, cfg.nucleation()) as b:
+        apply_substrate(a, gr, recs)
+        apply_substrate(b, gr, recs)
+        done = 0
+        for t in (10, 17, 30, 60, 100, 150, 200, 300):
+            assert orc.step(t - done) == abi.OK
+            ra = a.step(t - done, want_sums=True)
+            rb = b.step(t - done, want_sums=True)
+            done = t
+            g = a.download(abi.FIELD_GR)
+            melted = max(melted, int(np.sum(g[1:-1, 1:-1, 1:-1] <= 0)))
+            print("am track step", t, "liquid cells", int(np.sum(g[1:-1, 1:-1, 1:-1] <= 0)),
+                  "grains", ra.num_grains, flush=True)
+            for f in (abi.FIELD_CE, abi.FIELD_CL, abi.FIELD_FS, abi.FIELD_GR):
+                assert np.array_equal(bits(a.download(f)), bits(orc.field(f))), (t, f)
+            assert ra.num_grains == orc.num_grains
+            assert np.array_equal(a.interface_list(), orc.diff_id()), t
+            c64 = a.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1]
+            c32 = b.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1]
+            e = _fp32_errors(c64, ra, c32, rb, n, cfg.c_init)
+            print("  fp32 vs fp64:", e, flush=True)
+            assert e["ledger"] <= 1e-12
+            assert e["grains"][0] == e["grains"][1]
+            assert e["fs"] <= 1e-6
+            assert e["l2"] <= 1e-6
+    assert melted > n // 20                      # the beam melted part of the substrate
+
+
+@pytest.mark.timeout(600)
+def test_am_track_full_size_fp32_against_fp64():
+    cfg = am_track_config()
+    gr, recs = voronoi_substrate(cfg, 2000, 2)
+    n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
+    steps = 40
+    out = {}
+    for prec in (abi.PREC_FP64, abi.PREC_FP32DEV):
+        print("full track: precision", prec, flush=True)
+        with Subblock(cfg.params(check_negative_ce=False, precision=prec), cfg.nucleation()) as sb:
+            apply_substrate(sb, gr, recs)
+            rep = sb.step(steps, want_sums=True)
+            out[prec] = (sb.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1], rep,
+                         int(np.sum(sb.download(abi.FIELD_GR)[1:-1, 1:-1, 1:-1] <= 0)))
+    (c64, r64, liq64), (c32, r32, liq32) = out[abi.PREC_FP64], out[abi.PREC_FP32DEV]
+    e = _fp32_errors(c64, r64, c32, r32, n, cfg.c_init)
+    print("full track: liquid cells", liq64, liq32, "errors", e, flush=True)
+    assert liq64 > 0 and liq32 == liq64
+    assert e["ledger"] <= 1e-12
+    assert e["grains"][0] == e["grains"][1]
+    assert e["fs"] <= 1e-6
+    assert e["l2"] <= 1e-6
+
+
+@pytest.mark.timeout(900)
+def test_ensemble_batch_graph_autotuned_against_oracle():
+    steps = 200
+    cfgs = ensemble_configs()
+    for c in cfgs:
+        c.dt *= 0.5
+    p64 = [c.params(check_negative_ce=True) for c in cfgs]
+    p32 = [c.params(check_negative_ce=True, precision=abi.PREC_FP32DEV) for c in cfgs]
+    nuc = [c.nucleation() for c in cfgs]
+    planes, rows, ms = autotune_batch(p64, nuc, warmup=5, trial_steps=10)
+    print("autotuned shape", planes, rows, ms, flush=True)
+    assert (planes, rows) in AUTOTUNE_SHAPES and ms > 0
+    reps = {}
+    with Batch(p64, nuc) as bt:
+        bt.set_tuning(abi.TUNE_ZM_PLANES, planes)
+        bt.set_tuning(abi.TUNE_ZM_ROWS, rows)
+        bt.set_tuning(abi.TUNE_GRAPH, 1)
+        reps[64] = bt.step(steps, want_sums=True)
+        for i, (c, d) in enumerate(zip(cfgs, bt.domains)):
+            orc = OracleSubblock(p64[i], nuc[i])
+            assert orc.step(steps) == abi.OK
+            for f in (abi.FIELD_CE, abi.FIELD_CL, abi.FIELD_FS, abi.FIELD_GR):
+                assert np.array_equal(bits(d.download(f)), bits(orc.field(f))), (i, f)
+            assert reps[64][i].num_grains == orc.num_grains, i
+            assert np.array_equal(d.interface_list(), orc.diff_id()), i
+            orc.close()
+            if i % 16 == 15:
+                print("ensemble: %d domains equal the oracle" % (i + 1), flush=True)
+    with Batch(p32, nuc) as bt:
+        bt.set_tuning(abi.TUNE_ZM_PLANES, planes)
+        bt.set_tuning(abi.TUNE_ZM_ROWS, rows)
+        bt.set_tuning(abi.TUNE_GRAPH, 1)
+        reps[32] = bt.step(steps, want_sums=True)
+    grown = 0
+    for c, a, b in zip(cfgs, reps[64], reps[32]):
+        n = c.sbdim[0] * c.sbdim[1] * c.sbdim[2]
+        assert b.num_grains == a.num_grains
+        assert abs(b.solute_sum + b.solute_ledger - n * c.c_init) <= 1e-12 * n * c.c_init
+        assert abs(b.solid_sum - a.solid_sum) / n <= 1e-6
+        grown += a.num_grains > 1
+    assert grown > 48

@@ -163,7 +163,7 @@ typedef struct mmca_report {
 #define MMCA_FIELD_FS         2  /* double */
 #define MMCA_FIELD_GR         3  /* int32  */
 #define MMCA_FIELD_DC         4  /* double[3] per cell, packed x,y,z (decentered_t) */
-#define MMCA_FIELD_D          5  /* double, download only: half-diagonal of solid cells */
+#define MMCA_FIELD_D          5  /* double: half-diagonal of solid cells (upload: the array MMCA_TUNE_TRACK_D keeps) */
 #define MMCA_FIELD_TEMP       6  /* double, download only: temperature at current timestep */
 #define MMCA_FIELD_THRESHOLD  7  /* float, download only: INFINITY where no site */
 #define MMCA_FIELD_MOLD       8  /* int8, download only */
@@ -232,6 +232,31 @@ int  mmca_download_field(mmca_ctx *ctx, int field, void *host);
 int  mmca_get_grains(mmca_ctx *ctx, mmca_grain *out, int32_t max, int32_t *num_grains);
 int  mmca_get_interface_list(mmca_ctx *ctx, int32_t *out, int32_t max, int32_t *gindex);
 /*
+ * Slots 0..n-1 of the interface-list array as they are (n <= cells + 2):
+ * slots 1..gindex hold the list, slots past it what earlier, longer lists
+ * left there, and slot 0 is never written -- as in the reference's diff_id
+ * array (growth.c:461-538), which its checkpoint stores whole
+ * (checkpoint.c:1169).
+ */
+int  mmca_get_list_slots(mmca_ctx *ctx, int32_t *out, int64_t n);
+/* Restores slots gindex+1 .. n-1 (the stale part) from in[gindex+1 .. n-1];
+ * slots 0..gindex are left as they are (n <= cells + 2). */
+int  mmca_set_list_slots(mmca_ctx *ctx, const int32_t *in, int64_t n);
+/*
+ * The subblock's FixedNuc and NucVolume points (nucleation_t, globals.h:147-160:
+ * position [m], global cell, growth direction, threshold) in the order
+ * init_nucleation_sites gives them to the subblock (grain.c:603-713), as its
+ * checkpoint stores them (nuc_pt, checkpoint.c:1188-1204).  Copies at most
+ * max points to out and sets *n to their number.
+ */
+typedef struct mmca_nuc_point {
+    double x, y, z;
+    uint64_t cx, cy, cz;
+    double gx, gy, gz;
+    double threshold;
+} mmca_nuc_point;
+int  mmca_get_nucleation_points(mmca_ctx *ctx, mmca_nuc_point *out, int32_t max, int32_t *n);
+/*
  * Replace the grain table by records 0..num_grains-1 and set num_grains, as
  * a restart restores grain_cache and bp->num_grains (addGrainCache,
  * checkpoint.c:676-745).  Record 0 stands for the liquid; rotation
@@ -252,6 +277,15 @@ int  mmca_set_grains(mmca_ctx *ctx, const mmca_grain *grains, int32_t num_grains
  */
 int  mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size);
 int  mmca_load_state(mmca_ctx *ctx, const void *buf, int64_t size);
+/*
+ * As mmca_save_state, with flags.  MMCA_STATE_NO_FIELDS leaves ce, cl, fs,
+ * gr and dc out of the image of an FP64 block (for a caller that stores them
+ * itself, as the checkpoint files in the reference's layout do): after
+ * mmca_load_state of such an image, mmca_upload_field must restore them.
+ * FP32-deviation blocks keep their fields in the image.
+ */
+#define MMCA_STATE_NO_FIELDS 1
+int  mmca_save_state_ex(mmca_ctx *ctx, int flags, void *buf, int64_t capacity, int64_t *size);
 int  mmca_get_timestep(mmca_ctx *ctx, uint64_t *timestep);
 /*
  * cache_io_data (loop.c:74-113) and the host copies the output writers read
@@ -289,6 +323,9 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
 #define MMCA_TUNE_ZM_LDS     4   /* 1: z-marching kernel with its planes staged in LDS by direct loads */
 #define MMCA_TUNE_CAPTURE_CHUNK 5 /* >0: capture deals list slots in chunks of this many (rounded up to
                                       256), chunk q on XCD q % 8; 0: 256-slot groups over the whole grid */
+#define MMCA_TUNE_TRACK_D    6   /* 1: keep d as the reference's array holds it (growth.c:392-403 writes
+                                    solid cells only, so a cell that melts keeps its last value);
+                                    MMCA_FIELD_D then gives that array */
 int  mmca_set_tuning(mmca_ctx *ctx, int key, int value);
 int  mmca_get_tuning(mmca_ctx *ctx, int key, int *value);
 
@@ -342,10 +379,14 @@ int  mmca_batch_couple(mmca_batch *batch, const int32_t nsub[3]);
  * image holds every rank's fields, list, dc_tmp slots and stream, and the
  * shared grain table once.  Load into a batch created with the same
  * parameters and nucleation set-up and, for a coupled image, coupled with
- * the same nsub; the run then continues bit for bit.
+ * the same nsub; the run then continues bit for bit.  A coupled batch that
+ * stopped on an error (every rank stops at the step of the first one, as the
+ * reference's error() ends every rank) is refused with MMCA_ERR_STATE.
  */
 int  mmca_batch_save_state(mmca_batch *batch, void *buf, int64_t capacity, int64_t *size);
 int  mmca_batch_load_state(mmca_batch *batch, const void *buf, int64_t size);
+/* mmca_save_state_ex for a batch (every domain with the same flags). */
+int  mmca_batch_save_state_ex(mmca_batch *batch, int flags, void *buf, int64_t capacity, int64_t *size);
 int  mmca_batch_set_tuning(mmca_batch *batch, int key, int value);
 int  mmca_batch_set_kernel_timing(mmca_batch *batch, int enable);
 int  mmca_batch_kernel_time_ms(mmca_batch *batch, double *avg_ms, int64_t *launches);

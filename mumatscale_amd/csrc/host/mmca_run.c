@@ -60,22 +60,46 @@ typedef struct io_lib {
     mmca_vtk *(*vopen)(const mmca_run_cfg *, const int *, const int *);
     int (*vwrite)(mmca_vtk *, mmca_ctx *const *, const mmca_params *, uint64_t);
     void (*vclose)(mmca_vtk *);
+    int (*cwrite)(const mmca_run_cfg *, const int *, mmca_batch *, mmca_ctx *const *, const mmca_params *,
+                  double *const *, uint64_t, const uint64_t *, int64_t);
+    int (*cread)(const mmca_run_cfg *, const int *, mmca_batch *, mmca_ctx *const *, const mmca_params *,
+                 uint64_t *, int64_t, int64_t *);
     mmca_xdmf *x;                  /* DataFormat XDMF */
     mmca_vtk *v;                   /* DataFormat VTK */
 } io_lib;
+
+/* libmmca_io.so beside this program */
+static void *
+io_open(io_lib *w, char *path, size_t n)
+{
+    ssize_t k = readlink("/proc/self/exe", path, n - 32);
+    if (k <= 0)
+        return NULL;
+    path[k] = 0;
+    char *slash = strrchr(path, '/');
+    strcpy(slash ? slash + 1 : path, "libmmca_io.so");
+    if (!w->so)
+        w->so = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    return w->so;
+}
+
+/* the checkpoint writer of libmmca_io.so (mmca_ckpt.h); -1 without it */
+static int
+ckpt_load(io_lib *w)
+{
+    char path[PATH_MAX];
+    if (!io_open(w, path, sizeof(path)))
+        return -1;
+    *(void **) &w->cwrite = dlsym(w->so, "mmca_ckpt_write");
+    *(void **) &w->cread = dlsym(w->so, "mmca_ckpt_read");
+    return (w->cwrite && w->cread) ? 0 : -1;
+}
 
 static int
 io_load(io_lib *w, int vtk)
 {
     char path[PATH_MAX];
-    ssize_t n = readlink("/proc/self/exe", path, sizeof(path) - 32);
-    if (n <= 0)
-        return -1;
-    path[n] = 0;
-    char *slash = strrchr(path, '/');
-    strcpy(slash ? slash + 1 : path, "libmmca_io.so");
-    w->so = dlopen(path, RTLD_NOW | RTLD_LOCAL);
-    if (!w->so)
+    if (!io_open(w, path, sizeof(path)))
     {
         fprintf(stderr, "DataFormat %s needs %s: %s\n", vtk ? "VTK" : "XDMF", path, dlerror());
         return -1;
@@ -220,6 +244,47 @@ load_checkpoint(mmca_batch *bt, mmca_ctx *ctx, io_lib *out)
     free(buf);
     free(ren);
     return 0;
+}
+
+/*
+ * writeCheckpoint (loop.c:129-145) in the reference's file layout
+ * (writeTaskCheckpoint, checkpoint.c:955-1238): one HDF5 file per rank,
+ * renamed to cp_main.h5 / cp_task_<rank>.h5, with the restart image that
+ * lacks the fields and the XDMF renumbering in rank 0's file (mmca_ckpt.h).
+ */
+static int
+ckpt_write(const io_lib *out, const mmca_run_cfg *cfg, const int nsub[3], mmca_batch *bt, mmca_ctx *const *doms,
+           const mmca_params *ps, double *const *oce, uint64_t t)
+{
+    const int64_t nren = out->x ? out->get_renumber(out->x, NULL, 0) : 0;
+    uint64_t *ren = malloc(sizeof(uint64_t) * (size_t) (nren > 0 ? nren : 1));
+    if (!ren)
+        return -1;
+    if (out->x)
+        out->get_renumber(out->x, ren, nren);
+    const int rc = out->cwrite(cfg, nsub, bt, doms, ps, oce, t, ren, nren);
+    free(ren);
+    if (rc)
+    {
+        fprintf(stderr, "Unable to write checkpoint at timestep %lu\n", (unsigned long) t);
+        return -1;
+    }
+    printf("Checkpoint at timestep %lu \n", (unsigned long) t);
+    return 0;
+}
+
+static int
+ckpt_read(io_lib *out, const mmca_run_cfg *cfg, const int nsub[3], mmca_batch *bt, mmca_ctx *const *doms,
+          const mmca_params *ps)
+{
+    const int64_t max = cfg->max_total_grains + 2;
+    uint64_t *ren = malloc(sizeof(uint64_t) * (size_t) max);
+    int64_t nren = 0;
+    int rc = !ren || out->cread(cfg, nsub, bt, doms, ps, ren, max, &nren);
+    if (rc == 0 && out->x && out->set_renumber(out->x, ren, nren))
+        rc = 1;
+    free(ren);
+    return rc ? -1 : 0;
 }
 
 static void
@@ -453,13 +518,48 @@ main(int argc, char **argv)
     }
     /* checkpointfreq: the reference reads DataWriteStart into it (read_ctrl.c:594-599) */
     const long long checkpoint_freq = cp_freq >= 0 ? cp_freq : cfg.data_write_start;
+    /* checkpoints in the reference's layout when libmmca_io.so is there, else
+     * the restart image cp_main.mmca; the layout keeps the reference's d array
+     * and its oce (ce before the last step, fetched one step before each
+     * checkpoint) */
+    const int one_sub[3] = { 1, 1, 1 };
+    const int *lay = nr > 1 ? cfg.nsub : one_sub;
+    const int h5cp = (checkpoint_freq > 0 || restart) && ckpt_load(&out) == 0;
+    double **oce = NULL;
+    if (h5cp)
+    {
+        if (check(bt ? mmca_batch_set_tuning(bt, MMCA_TUNE_TRACK_D, 1) : mmca_set_tuning(doms[0], MMCA_TUNE_TRACK_D, 1),
+                  "track d"))
+            return 1;
+        oce = calloc((size_t) nr, sizeof(double *));
+        for (int r = 0; oce && r < nr; r++)
+            if (!(oce[r] = malloc(sizeof(double) * (size_t) (ps[r].nx + 2) * (ps[r].ny + 2) * (ps[r].nz + 2))))
+                oce = NULL;
+        if (!oce)
+        {
+            fprintf(stderr, "out of host memory\n");
+            return 1;
+        }
+    }
+    else
+        out.cwrite = NULL;
     uint64_t t = 0;
     if (restart)
     {
-        if (load_checkpoint(bt, doms[0], &out) || check(mmca_get_timestep(doms[0], &t), "mmca_get_timestep"))
+        FILE *h5 = h5cp ? fopen("cp_main.h5", "rb") : NULL;
+        if (h5)
+            fclose(h5);
+        if ((h5 ? ckpt_read(&out, &cfg, lay, bt, doms, ps) : load_checkpoint(bt, doms[0], &out)) ||
+            check(mmca_get_timestep(doms[0], &t), "mmca_get_timestep"))
             return 1;
         printf("Restarting from timestep %lu\n", (unsigned long) t);
     }
+    /* oce of the next checkpoint: ce one step before it */
+    const int keep_oce = h5cp && checkpoint_freq > 0;
+    if (keep_oce && (t + 1) % checkpoint_freq == 0)
+        for (int r = 0; r < nr; r++)
+            if (check(mmca_download_field(doms[r], MMCA_FIELD_CE, oce[r]), "download ce"))
+                return 1;
     printf("Initial Subblocks assigned.  Starting calculations.\n");
     const double total_volume = (p.gdim[0] * p.gdim[1] * p.gdim[2]) * pow(cfg.cell_size, 3);
     const double cell_vol = pow(cfg.cell_size, 3.0);
@@ -486,6 +586,8 @@ main(int argc, char **argv)
         }
         if (checkpoint_freq > 0 && (t / checkpoint_freq + 1) * checkpoint_freq < next)
             next = (t / checkpoint_freq + 1) * checkpoint_freq;
+        if (keep_oce && next % checkpoint_freq == 0 && next - 1 > t)
+            next = next - 1;                  /* stop before the checkpoint for its oce */
         if (fin && (fin > t ? fin : t + 1) < next)
             next = fin > t ? fin : t + 1;
         if (max_steps >= 0 && (uint64_t) max_steps < next)
@@ -523,11 +625,18 @@ main(int argc, char **argv)
             status = MMCA_ERR_ARG;
             break;
         }
-        if (checkpoint_freq > 0 && t % checkpoint_freq == 0 && save_checkpoint(bt, doms[0], &out, t))
+        if (checkpoint_freq > 0 && t % checkpoint_freq == 0 &&
+            (h5cp ? ckpt_write(&out, &cfg, lay, bt, doms, ps, oce, t) : save_checkpoint(bt, doms[0], &out, t)))
         {
             status = MMCA_ERR_ARG;
             break;
         }
+        if (keep_oce && (t + 1) % checkpoint_freq == 0)
+            for (int r = 0; r < nr && status == MMCA_OK; r++)
+                if (check(mmca_download_field(doms[r], MMCA_FIELD_CE, oce[r]), "download ce"))
+                    status = MMCA_ERR_ARG;
+        if (status != MMCA_OK)
+            break;
         int done = 0;
         if (cfg.finish_time && t * cfg.dt >= cfg.finish_time)
         {

@@ -940,6 +940,11 @@ int mmca_upload_field(mmca_ctx *ctx, int field, const void *host)
     case MMCA_FIELD_DC:
         HIPCHK(hipMemcpy(b.dc, host, 3 * P * sizeof(double), hipMemcpyHostToDevice));
         break;
+    case MMCA_FIELD_D:
+        // the reference's d array (kept with MMCA_TUNE_TRACK_D; a restart's input)
+        TRY(ensure_d(ctx));
+        HIPCHK(hipMemcpy(b.d, host, P * sizeof(double), hipMemcpyHostToDevice));
+        break;
     case MMCA_FIELD_OCE:
         if (ctx->c.f32)
             return MMCA_ERR_STATE;
@@ -1029,6 +1034,48 @@ int mmca_download_field(mmca_ctx *ctx, int field, void *host)
     default:
         return MMCA_ERR_ARG;
     }
+    return MMCA_OK;
+}
+
+int mmca_get_nucleation_points(mmca_ctx *ctx, mmca_nuc_point *out, int32_t max, int32_t *n)
+{
+    static_assert(sizeof(mmca_nuc_point) == sizeof(NucPt), "nucleation_t layout");
+    if (!ctx || max < 0 || (max > 0 && !out))
+        return MMCA_ERR_ARG;
+    if (!ctx->nucleation_ready)
+        return MMCA_ERR_STATE;
+    const int32_t np = (int32_t) ctx->nuc_pt.size();
+    for (int32_t i = 0; i < np && i < max; i++) {
+        const NucPt &q = ctx->nuc_pt[(size_t) i];
+        out[i] = mmca_nuc_point{ q.x, q.y, q.z, (uint64_t) q.cx, (uint64_t) q.cy, (uint64_t) q.cz,
+                                 q.gx, q.gy, q.gz, q.threshold };
+    }
+    if (n)
+        *n = np;
+    return MMCA_OK;
+}
+
+int mmca_get_list_slots(mmca_ctx *ctx, int32_t *out, int64_t n)
+{
+    if (!ctx || !out || n < 0 || n > (int64_t) ctx->c.nx * ctx->c.ny * ctx->c.nz + 2)
+        return MMCA_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (n > 0)
+        HIPCHK(hipMemcpy(out, ctx->b.diff_id, (size_t) n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return MMCA_OK;
+}
+
+int mmca_set_list_slots(mmca_ctx *ctx, const int32_t *in, int64_t n)
+{
+    if (!ctx || !in || n < 0 || n > (int64_t) ctx->c.nx * ctx->c.ny * ctx->c.nz + 2)
+        return MMCA_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    TRY(sync_counters(ctx));
+    const int64_t first = (int64_t) ctx->hc.gindex + 1;
+    if (n > first)
+        HIPCHK(hipMemcpy(ctx->b.diff_id + first, in + first, (size_t) (n - first) * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
     return MMCA_OK;
 }
 
@@ -1137,7 +1184,7 @@ struct StateHeader {
     int64_t P;
     int32_t nx, ny, nz, nsites;
     int64_t max_total_grains;
-    int32_t cur, oriented, n_ext, pad_;
+    int32_t cur, oriented, n_ext, flags;    // flags: MMCA_STATE_NO_FIELDS (0 in images without it)
     int64_t nledger, n_angles;
 };
 
@@ -1227,14 +1274,30 @@ int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc, T
         TRY(io.dev(b.ce32, P * sizeof(float)));
         TRY(io.dev(b.cl32[cur], P * sizeof(float)));
         TRY(io.dev(b.ledger, (size_t) (2 * h.nledger) * sizeof(double)));
-    } else {
+    } else if (!(h.flags & MMCA_STATE_NO_FIELDS)) {
         TRY(io.dev(b.ce, P * sizeof(double)));
         TRY(io.dev(b.cl[cur], P * sizeof(double)));
     }
-    TRY(io.dev(b.fs[cur], P * sizeof(double)));
-    TRY(io.dev(b.gr, P * sizeof(int)));
-    TRY(io.dev(b.dc, 3 * P * sizeof(double)));
-    // interface list (slots 0..gindex) and the persistent dc_tmp slots 0..dc_tmp_size
+    if (!(h.flags & MMCA_STATE_NO_FIELDS)) {
+        TRY(io.dev(b.fs[cur], P * sizeof(double)));
+        TRY(io.dev(b.gr, P * sizeof(int)));
+        TRY(io.dev(b.dc, 3 * P * sizeof(double)));
+    }
+    // interface list (slots 0..gindex) and the persistent dc_tmp slots 0..dc_tmp_size;
+    // capture writes the cells the list names, so a loaded list must name interior cells
+    if (io.load) {
+        const int64_t nb = (int64_t) (hc.gindex + 1) * (int64_t) sizeof(int);
+        if (io.cap - io.pos < nb)
+            return MMCA_ERR_ARG;
+        const Consts &c = ctx->c;
+        for (int32_t s = 1; s <= hc.gindex; s++) {
+            int32_t v;
+            memcpy(&v, io.in + io.pos + (int64_t) s * (int64_t) sizeof(int), sizeof(int));
+            const int64_t z = v / c.sxy, rem = v - z * c.sxy, y = rem / c.sx, x = rem - y * c.sx;
+            if (v < 0 || x < 1 || x > c.nx || y < 1 || y > c.ny || z < 1 || z > c.nz)
+                return MMCA_ERR_ARG;
+        }
+    }
     TRY(io.dev(b.diff_id, (size_t) (hc.gindex + 1) * sizeof(int)));
     const size_t nslot = (size_t) std::max(hc.dc_tmp_size, 0) + 1;
     TRY(io.dev(b.dtx, nslot * sizeof(double)));
@@ -1260,7 +1323,7 @@ int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc, T
 }
 
 // Appends the image of one block (header and body) to io.
-int save_block(mmca_ctx *ctx, StateIO &io, TablePart table)
+int save_block(mmca_ctx *ctx, StateIO &io, TablePart table, int flags)
 {
     TRY(sync_counters(ctx));
     StateHeader h;
@@ -1279,6 +1342,8 @@ int save_block(mmca_ctx *ctx, StateIO &io, TablePart table)
     h.n_ext = (int32_t) ctx->ext_grains.size();
     h.nledger = ctx->c.f32 ? ctx->b.nledger : 0;
     h.n_angles = ctx->hc.random_used;
+    // FP32-deviation fields stay in the image: FP64 copies would not give them back exactly
+    h.flags = ctx->c.f32 ? 0 : (flags & MMCA_STATE_NO_FIELDS);
     if ((int64_t) ctx->rand_angles.size() < 3 * h.n_angles)
         return MMCA_ERR_STATE;
     Counters hc = ctx->hc;
@@ -1304,7 +1369,8 @@ int load_block(mmca_ctx *ctx, StateIO &io, TablePart table, StateHeader *out_h, 
         h.max_total_grains != ctx->p.max_total_grains || h.oriented != ctx->oriented ||
         h.precision != (ctx->c.f32 ? MMCA_PREC_FP32DEV : MMCA_PREC_FP64) || h.cur < 0 || h.cur > 1 ||
         (ctx->c.f32 && h.nledger != ctx->b.nledger) || h.n_angles < 0 || h.n_angles > ctx->rand_capacity ||
-        h.n_ext < 0 || (table != TABLE_OWN && h.n_ext != 0))
+        h.n_ext < 0 || (table != TABLE_OWN && h.n_ext != 0) || (h.flags & ~MMCA_STATE_NO_FIELDS) != 0 ||
+        (ctx->c.f32 && h.flags != 0))
         return MMCA_ERR_ARG;
     // the counters come first after the header: check the sizes they imply
     Counters hc;
@@ -1317,7 +1383,14 @@ int load_block(mmca_ctx *ctx, StateIO &io, TablePart table, StateHeader *out_h, 
     io.ctx = ctx;
     io.s = ctx->stream;
     io.pos += sizeof(h);
-    TRY(state_body(ctx, io, h, hc, table));
+    const int st = state_body(ctx, io, h, hc, table);
+    if (st != MMCA_OK) {
+        // device buffers may be partly overwritten: the block refuses to step from here
+        (void) hipStreamSynchronize(ctx->stream);
+        ctx->hc.status = MMCA_ERR_STATE;
+        (void) push_counters(ctx);
+        return st;
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->cur = h.cur;
     ctx->q0 = hc.random_used;
@@ -1340,7 +1413,12 @@ extern "C" {
 
 int mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size)
 {
-    if (!ctx || capacity < 0 || (buf && capacity == 0))
+    return mmca_save_state_ex(ctx, 0, buf, capacity, size);
+}
+
+int mmca_save_state_ex(mmca_ctx *ctx, int flags, void *buf, int64_t capacity, int64_t *size)
+{
+    if (!ctx || capacity < 0 || (buf && capacity == 0) || (flags & ~MMCA_STATE_NO_FIELDS) != 0)
         return MMCA_ERR_ARG;
     if (!ctx->peers.empty())
         return MMCA_ERR_STATE;   // a coupled rank: mmca_batch_save_state covers the whole batch
@@ -1352,7 +1430,7 @@ int mmca_save_state(mmca_ctx *ctx, void *buf, int64_t capacity, int64_t *size)
     io.in = nullptr;
     io.cap = capacity;
     io.load = false;
-    TRY(save_block(ctx, io, TABLE_OWN));
+    TRY(save_block(ctx, io, TABLE_OWN, flags));
     if (size)
         *size = io.pos;
     return MMCA_OK;
@@ -1481,6 +1559,12 @@ int mmca_set_tuning(mmca_ctx *ctx, int key, int value)
         ctx->c.capture_chunk = (value + 255) / 256 * 256;
         ctx->view_dirty = true;
         return MMCA_OK;
+    case MMCA_TUNE_TRACK_D:
+        if (value)
+            TRY(ensure_d(ctx));
+        ctx->c.track_d = value ? 1 : 0;
+        ctx->view_dirty = true;
+        return MMCA_OK;
     default:
         return MMCA_ERR_ARG;
     }
@@ -1505,6 +1589,9 @@ int mmca_get_tuning(mmca_ctx *ctx, int key, int *value)
         return MMCA_OK;
     case MMCA_TUNE_CAPTURE_CHUNK:
         *value = ctx->c.capture_chunk;
+        return MMCA_OK;
+    case MMCA_TUNE_TRACK_D:
+        *value = ctx->c.track_d;
         return MMCA_OK;
     default:
         return MMCA_ERR_ARG;
@@ -1545,7 +1632,7 @@ struct mmca_batch {
     View view;
     bool view_dirty = true;
     int cur = 0;
-    int zm_planes = 64, zm_by = 16, xcd_remap = 1, zm_lds = 0, capture_chunk = 0;
+    int zm_planes = 64, zm_by = 16, xcd_remap = 1, zm_lds = 0, capture_chunk = 0, track_d = 0;
     bool timing = false;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
@@ -1570,6 +1657,7 @@ int batch_view(mmca_batch *bt)
         c->c.xcd_remap = bt->xcd_remap;
         c->c.zm_lds = bt->zm_lds;
         c->c.capture_chunk = bt->capture_chunk;
+        c->c.track_d = bt->track_d;
         c->view_dirty = true;
     }
     TRY(build_view(bt->dom.data(), (int) bt->dom.size(), bt->xcd_remap, bt->zm_planes > 0 ? bt->zm_by : 0,
@@ -1737,13 +1825,24 @@ int mmca_batch_couple(mmca_batch *bt, const int32_t nsub[3])
 
 int mmca_batch_save_state(mmca_batch *bt, void *buf, int64_t capacity, int64_t *size)
 {
-    if (!bt || capacity < 0 || (buf && capacity == 0))
+    return mmca_batch_save_state_ex(bt, 0, buf, capacity, size);
+}
+
+int mmca_batch_save_state_ex(mmca_batch *bt, int flags, void *buf, int64_t capacity, int64_t *size)
+{
+    if (!bt || capacity < 0 || (buf && capacity == 0) || (flags & ~MMCA_STATE_NO_FIELDS) != 0)
         return MMCA_ERR_ARG;
     for (mmca_ctx *c : bt->dom)
         if (!c->nucleation_ready)
             return MMCA_ERR_STATE;
     HIPCHK(hipSetDevice(bt->device));
-    HIPCHK(hipStreamSynchronize(bt->stream));
+    TRY(batch_sync(bt));
+    // the reference's error() ends every rank: a coupled batch that stopped
+    // on an error has no state to continue from
+    if (bt->coupled)
+        for (mmca_ctx *c : bt->dom)
+            if (c->hc.status != MMCA_OK)
+                return MMCA_ERR_STATE;
     BatchStateHeader h;
     memset(&h, 0, sizeof(h));
     memcpy(h.magic, BATCH_MAGIC, 8);
@@ -1761,7 +1860,7 @@ int mmca_batch_save_state(mmca_batch *bt, void *buf, int64_t capacity, int64_t *
     TRY(io.host(&h, sizeof(h)));
     for (size_t d = 0; d < bt->dom.size(); d++) {
         const TablePart t = !bt->coupled ? TABLE_OWN : d == 0 ? TABLE_COUPLED_OWNER : TABLE_SHARED;
-        TRY(save_block(bt->dom[d], io, t));
+        TRY(save_block(bt->dom[d], io, t, flags));
     }
     if (size)
         *size = io.pos;
@@ -1874,6 +1973,11 @@ int mmca_batch_set_tuning(mmca_batch *bt, int key, int value)
         if (value < 0 || value > (1 << 24))
             return MMCA_ERR_ARG;
         bt->capture_chunk = (value + 255) / 256 * 256;
+        break;
+    case MMCA_TUNE_TRACK_D:
+        for (mmca_ctx *d : bt->dom)
+            TRY(mmca_set_tuning(d, MMCA_TUNE_TRACK_D, value));
+        bt->track_d = value ? 1 : 0;
         break;
     case MMCA_TUNE_GRAPH:
         bt->use_graph = value != 0;

@@ -211,6 +211,35 @@ beam_temp_tab(const Consts &c, const BeamTab &t, const double *Z, double sim_tim
     return c.liniT + Tfront * Tinteg;
 }
 
+// Octahedron half-diagonal of a solid cell (grow_octahedra, growth.c:392-403)
+// from its centre offset, fs and grain g.
+__device__ __forceinline__ double
+octa_d(const Consts &c, const DevBufs &b, long long i, int g, double fs)
+{
+    const MMCA_GLOBAL double *R = G(b.g_rot) + 9 * (long long) g;
+    double rx0 = 0.0 - G(b.dc)[3 * i];
+    double ry0 = 0.0 - G(b.dc)[3 * i + 1];
+    double rz0 = 0.0 - G(b.dc)[3 * i + 2];
+    double gx0 = R[0] * rx0 + R[3] * ry0 + R[6] * rz0;
+    double gy0 = R[1] * rx0 + R[4] * ry0 + R[7] * rz0;
+    double gz0 = R[2] * rx0 + R[5] * ry0 + R[8] * rz0;
+    double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
+    ds1 = ref_min(ds1, 1.5);
+    return ds1 + fs * c.rev_fsgrow;
+}
+
+// With track_d: a cell that stops being solid keeps, in the reference's d
+// array, the half-diagonal of its last step as a solid cell (grow_octahedra
+// writes solid cells only).  That value follows from the state the
+// partitioning step starts from: the cell's dc and fs are the ones of its
+// last growth step (capture writes only liquid cells).
+__device__ __forceinline__ void
+keep_stale_d(const Consts &c, const DevBufs &b, long long i, int g_old, int g_new, double fs_old)
+{
+    if (c.track_d && g_old > 0 && g_new <= 0 && g_old < c.max_total_grains)
+        G(b.d)[i] = octa_d(c, b, i, g_old, fs_old);
+}
+
 // 64-bit words of one step's branch record
 __host__ __device__ inline long long
 rec_words(const Consts &c)

@@ -28,6 +28,7 @@ struct Consts {
     int xcd_remap;          // 1: contiguous tile runs per XCD
     int zm_lds;             // 1: z-marching kernel with its planes staged in LDS (k_solute_zl)
     int capture_chunk;      // > 0: capture deals list slots in chunks of this many, pinned to XCDs
+    int track_d;            // 1: keep the reference's d array (stale values of cells no longer solid)
     int f32;                // 1: ce, cl stored as FP32 deviations from cinit
     // diffusion (growth.c:38-40)
     double rs, rl;

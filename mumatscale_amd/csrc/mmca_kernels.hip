@@ -454,8 +454,8 @@ k_halo_exchange(View V, int phase, int cb)
     const int ax = f / 2, hi = f % 2;
     const DevBufs b = V.dom[d].b;    // private copy: stores cannot alias it
     const Consts c = V.dom[d].c;
-    if (c.nsub[0] == 0)
-        return;
+    if (c.nsub[0] == 0 || G(b.cnt)->status != MMCA_OK)
+        return;                                   // a stopped batch keeps its halos
     // Everything indexed by the face's axis is picked with selects, not array
     // subscripts: a runtime subscript would put the descriptor in private
     // (scratch) memory for every thread.
@@ -591,11 +591,24 @@ partition_cell(const Consts &c, TempFn temp, double ce_new, double ce_prev, doub
     *g_out = g;
 }
 
+// growth.c:163-165: error() ends every rank.  In a coupled batch the status
+// goes to every domain of the view, so that each rank's later kernels (and
+// the halo exchange) stop at the same point; the ranks share one timestep.
 __device__ __forceinline__ void
-flag_negative_ce(Counters *cnt)
+flag_negative_ce(const View &V, const Consts &c, Counters *cnt)
 {
-    if (atomicCAS(&cnt->status, MMCA_OK, MMCA_ERR_NEGATIVE_CE) == MMCA_OK)
-        cnt->failed_step = cnt->timestep;
+    if (c.nsub[0] == 0)
+    {
+        if (atomicCAS(&cnt->status, MMCA_OK, MMCA_ERR_NEGATIVE_CE) == MMCA_OK)
+            cnt->failed_step = cnt->timestep;
+        return;
+    }
+    for (int d = 0; d < V.ndom; ++d)
+    {
+        Counters *o = V.dom[d].b.cnt;
+        if (atomicCAS(&o->status, MMCA_OK, MMCA_ERR_NEGATIVE_CE) == MMCA_OK)
+            o->failed_step = o->timestep;
+    }
 }
 
 // Cell-parallel kernel.  DIFF && FSC: fused step reading buffers `cur`,
@@ -668,7 +681,7 @@ k_solute(View V, int cur, int nxt)
                 }
                 ce_new = diffuse_cell(c, ce_old, conc, lspfs, cln, fsn, mo);
                 if (c.check_neg && ce_new < 0)
-                    flag_negative_ce(cnt);
+                    flag_negative_ce(V, c, cnt);
                 lost += st_ce<F32>(b, c, idx, ce_new);
             }
             if (FSC)
@@ -684,6 +697,7 @@ k_solute(View V, int cur, int nxt)
                 st_stream(fs_buf(b, nxt), idx, fs_out);
                 if (g != g_old)
                     G(b.gr)[idx] = g;
+                keep_stale_d(c, b, idx, g_old, g, lspfs);
                 g_new = g;
             }
         }
@@ -932,6 +946,7 @@ k_solute_zm(View V, int cur, int nxt)
             st_stream(fsn_out, idx, fs_out);
             if (g_new != g_old)
                 G(b.gr)[idx] = g_new;
+            keep_stale_d(c, b, idx, g_old, g_new, fsv);
         }
         unsigned long long bits = __ballot(active && g_new > 0);
         if (rowok && lane == 0)
@@ -952,7 +967,7 @@ k_solute_zm(View V, int cur, int nxt)
     // status and step of a negative ce (any plane of the tile: the step is
     // the same for all)
     if (c.check_neg && neg)
-        flag_negative_ce(cnt);
+        flag_negative_ce(V, c, cnt);
     if (F32)
     {
         const double tot = block_sum<64 * BY>(lost, s_red);
@@ -1192,6 +1207,7 @@ k_solute_zl(View V, int cur, int nxt)
             st_stream(fsn_out, idx, fs_out);
             if (g_new != g_old)
                 G(b.gr)[idx] = g_new;
+            keep_stale_d(c, b, idx, g_old, g_new, fsv);
         }
         unsigned long long bits = __ballot(active && g_new > 0);
         if (rowok && lane == 0)
@@ -1203,7 +1219,7 @@ k_solute_zl(View V, int cur, int nxt)
     }
     zl_wait<0>();                                // no load may land in LDS after the workgroup ends
     if (c.check_neg && neg)
-        flag_negative_ce(cnt);
+        flag_negative_ce(V, c, cnt);
     if (F32)
     {
         const double tot = block_sum<64 * BY>(lost, s_red);
@@ -1777,19 +1793,13 @@ k_grow_octahedra(DevBufs b, Consts c, int cb)
     if (x < 1 || x > c.nx || y < 1 || y > c.ny || z < 1 || z > c.nz)
         return;
     int grid = G(b.gr)[i];
-    if (grid > 0 && grid < c.max_total_grains)
-    {
-        const MMCA_GLOBAL double *R = G(b.g_rot) + 9 * (long long) grid;
-        double rx0 = 0.0 - G(b.dc)[3 * i];
-        double ry0 = 0.0 - G(b.dc)[3 * i + 1];
-        double rz0 = 0.0 - G(b.dc)[3 * i + 2];
-        double gx0 = R[0] * rx0 + R[3] * ry0 + R[6] * rz0;
-        double gy0 = R[1] * rx0 + R[4] * ry0 + R[7] * rz0;
-        double gz0 = R[2] * rx0 + R[5] * ry0 + R[8] * rz0;
-        double ds1 = fabs(gx0) + fabs(gy0) + fabs(gz0);
-        ds1 = ref_min(ds1, 1.5);
-        G(b.d)[i] = ds1 + fs_buf(b, cb)[i] * c.rev_fsgrow;
-    }
+    // with track_d the solid set is the one of the last growth step (the
+    // mask the solute kernel wrote, before capture), as in the reference
+    const bool solid = c.track_d
+        ? ((G(b.mask)[((long long) z * (c.ny + 2) + y) * c.w64 + (x - 1) / 64] >> ((x - 1) % 64)) & 1ull) != 0
+        : grid > 0;
+    if (solid && grid > 0 && grid < c.max_total_grains)
+        G(b.d)[i] = octa_d(c, b, i, grid, fs_buf(b, cb)[i]);
 }
 
 // ---------------------------------------------------------------------------

@@ -66,6 +66,10 @@ def lib():
     L.mmca_io_fields.argtypes = [vp, vp, vp, vp, vp]
     L.mmca_rand.argtypes = [vp, i32p, C.c_int32]
     L.mmca_save_state.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.mmca_save_state_ex.argtypes = [vp, C.c_int, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.mmca_get_list_slots.argtypes = [vp, C.POINTER(C.c_int32), C.c_int64]
+    L.mmca_set_list_slots.argtypes = [vp, C.POINTER(C.c_int32), C.c_int64]
+    L.mmca_get_nucleation_points.argtypes = [vp, vp, C.c_int32, C.POINTER(C.c_int32)]
     L.mmca_load_state.argtypes = [vp, vp, C.c_int64]
     L.mmca_padded_cells.argtypes = [vp, C.POINTER(C.c_int64)]
     L.mmca_set_kernel_timing.argtypes = [vp, C.c_int]
@@ -81,6 +85,7 @@ def lib():
     L.mmca_batch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.mmca_batch_couple.argtypes = [vp, i32p]
     L.mmca_batch_save_state.argtypes = [vp, vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.mmca_batch_save_state_ex.argtypes = [vp, C.c_int, vp, C.c_int64, C.POINTER(C.c_int64)]
     L.mmca_batch_load_state.argtypes = [vp, vp, C.c_int64]
     L.mmca_batch_set_kernel_timing.argtypes = [vp, C.c_int]
     L.mmca_batch_kernel_time_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]

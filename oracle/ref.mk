@@ -7,7 +7,9 @@
 # NUC_SEP, EBUG=0; no GPU offload) at its default Release -O3, with gcc at the
 # default x86-64 -march (no FMA).  MPI (MPICH) and HDF5/zlib are the copies
 # installed in the image under /opt/conda; nothing is stubbed.  Build outputs
-# go to oracle/_ref/ only.
+# go to oracle/_ref/ only.  ref_hook.c wraps doiteration (state dumps) and
+# writeMainCheckpoint (skipped on request: it crashes before any checkpoint
+# file is complete).
 
 REF_SRC ?= /root/reference/src
 CONDA   ?= /opt/conda
@@ -22,7 +24,7 @@ SRCS := packing calculate ca_main checkpoint curvature distribute face_util \
 CC      := gcc
 CFLAGS  := -std=gnu99 -O3 -DNDEBUG -fopenmp -DINDEX_SEP -DNUC_SEP -DEBUG=0 \
            -I$(REF_SRC) -I$(CONDA)/include -w
-LDFLAGS := -L$(CONDA)/lib -Wl,-rpath,$(CONDA)/lib -Wl,--wrap=doiteration
+LDFLAGS := -L$(CONDA)/lib -Wl,-rpath,$(CONDA)/lib -Wl,--wrap=doiteration -Wl,--wrap=writeMainCheckpoint
 LIBS    := -lhdf5 -lmpi -lz -lm -fopenmp
 
 OBJS := $(patsubst %,$(OBJ)/%.o,$(SRCS)) $(OBJ)/ref_hook.o

@@ -33,6 +33,26 @@ extern SB_struct *lsp;
 extern grain_t *grain_cache;
 
 void __real_doiteration(void);
+void __real_writeMainCheckpoint(void);
+
+/*
+ * The reference's main checkpoint writer reads gtasks.task, which nothing
+ * allocates (checkpoint.c:1391), so every run that writes checkpoints dies
+ * there on rank 0 before any rank's task file is complete.  With
+ * MMCA_REF_TASK_CHECKPOINT set this wrapper skips it.  Rank 0 then writes
+ * its task file to the same path (checkpoint.c:134-138 call the two writers
+ * in turn, and get_rank_cp_path gives rank 0 the main file's name), which
+ * replaces the main file in a run where the main writer works: what is left
+ * on disk is what writeTaskCheckpoint (checkpoint.c:955-1238) writes on
+ * every rank, renamed by clean_checkpoint (checkpoint.c:165-177).
+ */
+void
+__wrap_writeMainCheckpoint(void)
+{
+    if (getenv("MMCA_REF_TASK_CHECKPOINT"))
+        return;
+    __real_writeMainCheckpoint();
+}
 
 static uint64_t
 fnv1a(const void *p, size_t n)

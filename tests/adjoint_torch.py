@@ -8,6 +8,7 @@ order +x,-x,+y,-y,+z,-z, zero flux into mold faces) and fs_change_diffuse
 (MMCA_BRANCH_*), in the reference's operation order.  Only the uniform
 cooling temperature model (temperature.c:36-39) is implemented here.
 """
+import numpy as np
 import torch
 
 from mumatscale_amd import abi
@@ -81,4 +82,55 @@ def run(params, codes, t0, ce0, cl0, fs0, d_liq, d_sol, k):
     ce, cl, fs = ce0, cl0, fs0
     for s in range(codes.shape[0]):
         ce, cl, fs = step(params, cst, codes[s], t0 + s + 1, ce, cl, fs)
+    return ce, cl, fs
+
+
+def run_numpy(params, codes, t0, ce0, cl0, fs0, d_liq, d_sol, k, dtype=np.longdouble):
+    """run() in numpy at `dtype` (np.longdouble: the x87 80-bit format, 64-bit
+    significand) on numpy padded fields: the frozen-branch map evaluated with
+    11 more bits than FP64, for finite differences whose rounding noise is
+    far below FP64's.  Parameters may be given at that precision."""
+    one = dtype(1)
+    dx, dt = dtype(params.cell_size), dtype(params.dt)
+    dtx = dt / (dx * dx)
+    d_liq, d_sol, k = dtype(d_liq), dtype(d_sol), dtype(k)
+    rs, rl = d_sol * dtx, d_liq * dtx
+    k_inv, km = one / k, one - k
+    km_inv, m_inv = one / km, one / dtype(params.m_solute[2])
+    D_xt = d_liq / dx / dx * dt
+    ce, cl, fs = (np.array(a, dtype=dtype) for a in (ce0, cl0, fs0))
+    nz, ny, nx = ce.shape[0] - 2, ce.shape[1] - 2, ce.shape[2] - 2
+    w = [m.numpy().astype(dtype) for m in _mold(params, (nz, ny, nx))]
+    offs = ((0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0))
+
+    def at(a, dz, dy, dxx):
+        return a[1 + dz:nz + 1 + dz, 1 + dy:ny + 1 + dy, 1 + dxx:nx + 1 + dxx]
+
+    half = dtype(0.5)
+    for s in range(codes.shape[0]):
+        ce_c, cl_c, fs_c = at(ce, 0, 0, 0), at(cl, 0, 0, 0), at(fs, 0, 0, 0)
+        nbsum = np.zeros_like(ce_c)
+        for o, m in zip(offs, w):
+            fs_av = half * (fs_c + at(fs, *o))
+            r = (rs * fs_av + rl * (one - fs_av)) * m
+            nbsum = nbsum + r * (at(cl, *o) - cl_c)
+        ce_new = ce_c + nbsum
+        T = dtype(params.t_init) - dtype((t0 + s + 1) * params.dt) * dtype(params.cool_rate)
+        c_int = dtype(params.c_init) + m_inv * (T - dtype(params.liquidus_temp))
+        dce = ce_new - ce_c
+        A = one - km * fs_c
+        dcl = (c_int - cl_c) * D_xt * 2 + dce / A
+        dfs = (dcl * A - dce) * km_inv / cl_c
+        b = np.asarray(codes[s])[1:-1, 1:-1, 1:-1]
+        liquid = (b == abi.BRANCH_LIQUID) | (b == abi.BRANCH_REMELT)
+        solid = b == abi.BRANCH_SOLID
+        full = b == abi.BRANCH_FULL
+        cl_new = np.where(liquid, ce_new, np.where(solid, ce_new * k_inv, cl_c + dcl))
+        fs_new = np.where(liquid, dtype(0), np.where(solid, fs_c, np.where(full, one, fs_c + dfs)))
+        ce = ce.copy()
+        cl = cl.copy()
+        fs = fs.copy()
+        ce[1:-1, 1:-1, 1:-1] = ce_new
+        cl[1:-1, 1:-1, 1:-1] = cl_new
+        fs[1:-1, 1:-1, 1:-1] = fs_new
     return ce, cl, fs

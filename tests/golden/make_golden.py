@@ -57,6 +57,7 @@ CASES = os.path.join(HERE, "cases")
 CASES_MR = os.path.join(HERE, "cases_mr")
 CASES_XDMF = os.path.join(HERE, "cases_xdmf")
 CASES_VTK = os.path.join(HERE, "cases_vtk")
+CASES_CKPT = os.path.join(HERE, "cases_ckpt")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 REF_BIN = os.path.join(REPO, "oracle", "_ref", "mumatscale_ref")
 
@@ -176,6 +177,35 @@ def run_case_xdmf(name, out_dir):
         return exp
 
 
+def run_case_ckpt(name, out_dir):
+    """The reference's checkpoint files for a control file of cases_ckpt/.
+    Its main-file writer fails on every run that writes checkpoints (it
+    reads gtasks.task, which nothing allocates, checkpoint.c:1391) before
+    any file is complete; MMCA_REF_TASK_CHECKPOINT makes the link-time hook
+    skip it, so each rank's writeTaskCheckpoint (checkpoint.c:955-1238)
+    writes its file, rank 0's under the main file's name, which a working
+    main writer's file would carry too (rank 0 writes its task file to the
+    same path right after).  The digest covers the files left after the
+    last checkpoint step."""
+    ctrl = os.path.join(CASES_CKPT, name + ".in")
+    n, base = nranks(ctrl), base_name(ctrl)
+    with tempfile.TemporaryDirectory() as tmp:
+        env = dict(os.environ, MMCA_REF_TASK_CHECKPOINT="1", OMP_NUM_THREADS="1")
+        proc = subprocess.run([MPIEXEC, "-hosts", "127.0.0.1", "-n", str(n), REF_BIN, "-c", ctrl], cwd=tmp,
+                              env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=3600)
+        text = proc.stdout.decode(errors="replace")
+        digest = output_digest.checkpoint_digest(tmp)
+        os.makedirs(out_dir, exist_ok=True)
+        with gzip.open(os.path.join(out_dir, "digest.json.gz"), "wt") as fh:
+            json.dump(digest, fh, indent=0, sort_keys=True)
+        steps = [int(l.split()[-1]) for l in text.splitlines() if l.startswith("Checkpoint at timestep")]
+        exp = {"returncode": proc.returncode, "ranks": n, "checkpoint_steps": sorted(set(steps)),
+               "files": sorted(digest)}
+        with open(os.path.join(out_dir, "expected.json"), "w") as fh:
+            json.dump(exp, fh, indent=1)
+        return exp
+
+
 def run_case_vtk(name, out_dir):
     ctrl = os.path.join(CASES_VTK, name + ".in")
     n, base = nranks(ctrl), base_name(ctrl)
@@ -243,6 +273,15 @@ def main():
         if os.path.isdir(out):
             shutil.rmtree(out)
         exp = run_case_vtk(n, out)
+        print(n, exp)
+    names = sorted(f[:-3] for f in os.listdir(CASES_CKPT) if f.endswith(".in"))
+    if len(sys.argv) > 1:
+        names = [n for n in names if n in sys.argv[1:]]
+    for n in names:
+        out = os.path.join(HERE, n)
+        if os.path.isdir(out):
+            shutil.rmtree(out)
+        exp = run_case_ckpt(n, out)
         print(n, exp)
 
 

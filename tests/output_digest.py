@@ -121,3 +121,98 @@ def vtk_digest(root, base):
     prefix = os.path.join(root, base)
     paths = sorted(glob.glob(prefix + "_*.pvtu") + glob.glob(prefix + "_[0-9][0-9][0-9][0-9]/*.vtu"))
     return {os.path.relpath(p, root): hashlib.sha256(open(p, "rb").read()).hexdigest()[:16] for p in paths}
+
+
+def _dataset_paths(header):
+    """Paths of every dataset in an `h5dump -H` listing, compound types included."""
+    out, stack = [], []
+    for line in header.splitlines():
+        s = line.strip()
+        m = re.match(r'(GROUP|DATASET|DATATYPE|DATASPACE|STORAGE_LAYOUT|FILTERS|FILLVALUE|ALLOCATION_TIME)\b'
+                     r'(?: "(.*)")?.*\{\s*$', s)
+        if m:
+            kind, name = m.group(1), m.group(2)
+            if kind == "DATASET":
+                out.append("/" + "/".join(p for p in [n for k, n in stack if k == "GROUP"] + [name] if p))
+            stack.append((kind, (name or "").strip("/")))
+            continue
+        opened, closed = s.count("{"), s.count("}")
+        for _ in range(opened):
+            stack.append(("other", ""))
+        for _ in range(closed):
+            if stack:
+                stack.pop()
+    return out
+
+
+def _drop_group(lines, name):
+    """The listing without the top-level group `name` and its contents."""
+    out, depth = [], 0
+    for line in lines:
+        s = line.strip()
+        if depth == 0 and s == 'GROUP "%s" {' % name:
+            depth = 1
+            continue
+        if depth > 0:
+            depth += s.count("{") - s.count("}")
+            continue
+        out.append(line)
+    return out
+
+
+def h5_digest_each(path, drop=()):
+    """As h5_digest, with every dataset dumped on its own (compound types
+    have no fixed entry in the size table); top-level groups named in
+    `drop` are left out."""
+    header = subprocess.run([H5DUMP, "-p", "-H", path], stdout=subprocess.PIPE, check=True).stdout.decode()
+    lines = header.splitlines()
+    lines[0] = "HDF5 {"
+    for g in drop:
+        lines = _drop_group(lines, g)
+    header = "\n".join(lines)
+    data = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, "data.bin")
+        for ds in _dataset_paths(header):
+            r = subprocess.run([H5DUMP, "-b", "LE", "-d", ds, "-o", raw, path], stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL)
+            if r.returncode == 0:
+                blob = open(raw, "rb").read()
+            else:
+                # h5dump does not write nested compound members in binary:
+                # their values as text, every double to 17 digits
+                txt = subprocess.run([H5DUMP, "-m", "%.17g", "-d", ds, path], stdout=subprocess.PIPE,
+                                     check=True).stdout.decode()
+                blob = txt[txt.index("DATA {"):].encode()
+            data[ds] = hashlib.sha256(blob).hexdigest()[:16]
+    return {"header": "\n".join(lines), "data": data}
+
+
+def checkpoint_digest(root, drop=()):
+    """Digest of the checkpoint files a run left in root: cp_main.h5 and
+    cp_task_<rank>.h5 (writeTaskCheckpoint's layout, checkpoint.c:955-1238),
+    without the top-level groups named in `drop`."""
+    files = sorted(glob.glob(os.path.join(root, "cp_main.h5")) + glob.glob(os.path.join(root, "cp_task_*.h5")))
+    return {os.path.basename(p): h5_digest_each(p, drop) for p in files}
+
+
+def compare_files(got, want, skip=()):
+    """Differences between two {file: digest} maps; datasets whose path ends
+    in one of `skip` are left out."""
+    bad = []
+    for f in sorted(set(got) | set(want)):
+        if f not in got or f not in want:
+            bad.append("%s only in %s" % (f, "output" if f in got else "fixture"))
+            continue
+        g, w = got[f], want[f]
+        if g["header"] != w["header"]:
+            a, b = g["header"].splitlines(), w["header"].splitlines()
+            first = next((i for i, (x, y) in enumerate(zip(a, b)) if x != y), min(len(a), len(b)))
+            bad.append("%s: structure differs from line %d: %r vs %r" % (
+                f, first + 1, a[first] if first < len(a) else None, b[first] if first < len(b) else None))
+        for ds in sorted(set(g["data"]) | set(w["data"])):
+            if ds.endswith(tuple(skip)):
+                continue
+            if g["data"].get(ds) != w["data"].get(ds):
+                bad.append("%s:%s data differs" % (f, ds))
+    return bad

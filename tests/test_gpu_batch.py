@@ -60,3 +60,24 @@ def test_batch_restart_image_continues_bit_for_bit():
                 assert np.array_equal(bits(da.download(f)), bits(db.download(f))), (name, f)
             assert np.array_equal(da.interface_list(), db.interface_list()), name
             assert da.num_grains == db.num_grains, name
+
+
+def test_corrupted_restart_image_is_refused():
+    """A restart image whose interface list names a cell outside the
+    interior is refused (capture would write through it), and a block whose
+    load failed part way refuses to step."""
+    from mumatscale_amd.lib import Subblock
+    cfg = load_case("c1_directional")[0]
+    with Subblock(cfg.params(), cfg.nucleation()) as a, Subblock(cfg.params(), cfg.nucleation()) as b:
+        a.step(120)
+        lst = a.interface_list()
+        assert len(lst) > 8
+        img = bytearray(a.save_state())
+        pat = np.asarray(lst[:8], dtype=np.int32).tobytes()
+        at = bytes(img).find(pat)
+        assert at > 0
+        img[at:at + 4] = np.int32(-5).tobytes()                 # slot 1 -> outside the box
+        with pytest.raises(abi.MmcaError):
+            b.load_state(bytes(img))
+        with pytest.raises(abi.MmcaError):
+            b.step(1)

@@ -116,7 +116,8 @@ def test_xdmf_output_matches_reference(case, tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("case,stop", [("x2_directional_one_block", 200), ("x3_cooling_x2", 120)])
 def test_restarted_run_writes_the_same_files(case, stop, tmp_path):
-    """writeCheckpoint / restart with this build's image (cp_main.mmca): a run
+    """writeCheckpoint / restart through the checkpoint files (the
+    reference's layout, with this build's restart extras in /MMCA): a run
     stopped at `stop` and restarted with --restart writes the reference's
     files for every step, byte for byte (the later ones from the restarted
     run), and ends with its grain summary.  The time series of the restarted run starts afresh, as the

@@ -470,6 +470,23 @@ int  mmca_adjoint_step(const mmca_params *params, uint64_t timestep, const uint8
                        double *g_ce_in, double *g_cl_in, double *g_fs_in, double *g_param,
                        void *workspace, void *stream);
 
+/*
+ * mmca_adjoint_step without a workspace argument and without any allocation:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a fixed 32 MiB device array of the library
+ * (each GPU holds its own copy until the process ends), which covers blocks
+ * of up to about 4.1 M padded cells; larger blocks are refused with
+ * MMCA_ERR_ARG.  Every call on one GPU uses that GPU's array, so calls that
+ * overlap on one GPU (other streams, other threads) race.  Results are those
+ * of mmca_adjoint_step.
+ */
+int  mmca_adjoint_step_noalloc(const mmca_params *params, uint64_t timestep, const uint8_t *codes,
+                               const double *ce, const double *cl, const double *fs,
+                               const double *g_ce, const double *g_cl, const double *g_fs,
+                               double *g_ce_in, double *g_cl_in, double *g_fs_in, double *g_param,
+                               void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,12 +24,48 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <mutex>
+
 #include "host_init.h"
 #include "mmca_device.h"
 #include "mmca_internal.h"
 
+// Workspace of mmca_adjoint_step_noalloc, for callers that cannot allocate:
+// lam over the padded cells, then the per-workgroup partial sums.  Each GPU
+// holds its own copy of this array until the process ends; every call on
+// that GPU uses it, so calls that overlap on one GPU race.
+constexpr long long ADJ_SCRATCH_DOUBLES = 1ll << 22;     // 32 MiB
+static __device__ __align__(256) double scratch[ADJ_SCRATCH_DOUBLES];
+
 namespace mmca {
 namespace {
+
+// Device address of `scratch` on the current device.  The host-side name of
+// the array is not a device address: it is looked up once per device with
+// that device current, and only the looked-up address is passed to kernels.
+constexpr int ADJ_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[ADJ_MAX_DEVICES] = {};
+
+int adj_scratch_ptr(double **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return MMCA_ERR_HIP;
+    if (dev < 0 || dev >= ADJ_MAX_DEVICES)
+        return MMCA_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev])
+    {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return MMCA_ERR_HIP;
+        g_scratch_dev[dev] = (double *) p;
+    }
+    *out = g_scratch_dev[dev];
+    return MMCA_OK;
+}
 
 constexpr int ADJ_BX = 64;
 constexpr int ADJ_BY = 4;
@@ -298,6 +334,22 @@ k_adj_sum(const double *pa, const double *pb, long long nblk, double *g_param)
     }
 }
 
+// lam = 0 on the halo cells of the padded box (the stencil reads them; they
+// are never written by k_adj_part)
+__global__ void __launch_bounds__(256)
+k_zero_halo(Consts c, double *lam)
+{
+    const long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.P)
+        return;
+    const int x = (int) (i % c.sx);
+    const long long r = i / c.sx;
+    const int j = (int) (r % (c.ny + 2));
+    const int k = (int) (r / (c.ny + 2));
+    if (x == 0 || x > c.nx || j == 0 || j > c.ny || k == 0 || k > c.nz)
+        lam[i] = 0.0;
+}
+
 // The launches of one adjoint step; lam holds P doubles, pa and pb 2 * nblk each.
 hipError_t
 adjoint_launch(const Consts &c, const Deriv &dv, uint64_t timestep, const uint8_t *codes, const double *ce,
@@ -411,6 +463,43 @@ int mmca_adjoint_step(const mmca_params *params, uint64_t timestep, const uint8_
     double *pb = pa + 2 * nblk;
     return adjoint_launch(c, dv, timestep, codes, ce, cl, fs, g_ce, g_cl, g_fs, g_ce_in, g_cl_in, g_fs_in, g_param,
                           lam, pa, pb, (hipStream_t) stream) == hipSuccess
+               ? MMCA_OK
+               : MMCA_ERR_HIP;
+}
+
+int mmca_adjoint_step_noalloc(const mmca_params *params, uint64_t timestep, const uint8_t *codes,
+                              const double *ce, const double *cl, const double *fs, const double *g_ce,
+                              const double *g_cl, const double *g_fs, double *g_ce_in, double *g_cl_in,
+                              double *g_fs_in, double *g_param, void *stream)
+{
+    using namespace mmca;
+    Consts c;
+    Deriv dv;
+    int st = adj_consts(params, c, dv);
+    if (st != MMCA_OK)
+        return st;
+    if (!codes || !ce || !cl || !fs || !g_ce || !g_cl || !g_fs || !g_ce_in || !g_cl_in || !g_fs_in || !g_param)
+        return MMCA_ERR_ARG;
+    const long long nblk = adj_blocks(c);
+    if (c.P + 4 * nblk > ADJ_SCRATCH_DOUBLES)
+    {
+        fprintf(stderr, "mmca_adjoint_step_noalloc: %lld padded cells need %lld bytes of workspace, above the "
+                        "%lld of its fixed device array\n",
+                (long long) c.P, (long long) sizeof(double) * (c.P + 4 * nblk),
+                (long long) sizeof(double) * ADJ_SCRATCH_DOUBLES);
+        return MMCA_ERR_ARG;
+    }
+    double *lam = nullptr;
+    st = adj_scratch_ptr(&lam);
+    if (st != MMCA_OK)
+        return st;
+    double *pa = lam + c.P;
+    double *pb = pa + 2 * nblk;
+    hipStream_t s = (hipStream_t) stream;
+    // a previous call (another shape) may have left values where this shape's halo lies
+    hipLaunchKernelGGL(k_zero_halo, dim3((unsigned) ((c.P + 255) / 256)), dim3(256), 0, s, c, lam);
+    return adjoint_launch(c, dv, timestep, codes, ce, cl, fs, g_ce, g_cl, g_fs, g_ce_in, g_cl_in, g_fs_in, g_param,
+                          lam, pa, pb, s) == hipSuccess
                ? MMCA_OK
                : MMCA_ERR_HIP;
 }

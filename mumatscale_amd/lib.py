@@ -103,6 +103,8 @@ def lib():
     L.mmca_record_bytes.argtypes = [C.POINTER(abi.Params), C.POINTER(C.c_int64)]
     L.mmca_adjoint_step.argtypes = [C.POINTER(abi.Params), C.c_uint64, vp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
                                     vp, vp]
+    L.mmca_adjoint_step_noalloc.argtypes = [C.POINTER(abi.Params), C.c_uint64, vp, dp, dp, dp, dp, dp, dp, dp,
+                                            dp, dp, dp, vp]
     if L.mmca_abi_version() != abi.ABI_VERSION:
         raise RuntimeError("libmmca ABI version mismatch")
     _lib = L

@@ -336,10 +336,16 @@ def run_adjoint(n=128, nseeds=16, steps=4000):
 
 def run_am_track(steps=60):
     """The AM single track over a 2000-grain Voronoi substrate (1024 x 512 x
-    256, dx = 0.5 um, Gaussian-beam temperature) with FP32-deviation storage,
-    and its error against the FP64 run of the same steps.  Every solid cell
-    evaluates the 400-term beam quadrature each step (temperature.c:88-118),
-    so this workload is bound by FP64 VALU work, not HBM."""
+    256, dx = 0.5 um, Gaussian-beam temperature), its error against the FP64
+    run of the same steps, in three forms:
+      fp32dev            FP32-deviation storage, the beam temperature from its
+                         separable factors (a 400-deep product over the block,
+                         computed once per step);
+      fp32dev_quadrature FP32-deviation storage, the 400-term quadrature per
+                         cell (temperature.c:88-118);
+      fp64               FP64 storage and the quadrature, bit for bit the
+                         reference's.
+    The quadrature forms are bound by FP64 VALU work, not HBM."""
     import numpy as np
     from mumatscale_amd.lib import Subblock
     from mumatscale_amd.workloads import am_track_config, apply_substrate, voronoi_substrate
@@ -349,8 +355,12 @@ def run_am_track(steps=60):
     log("am track: substrate built in %.1f s" % (time.time() - t0))
     n = cfg.sbdim[0] * cfg.sbdim[1] * cfg.sbdim[2]
     out, fields = {}, {}
-    for prec in (abi.PREC_FP32DEV, abi.PREC_FP64):
+    legs = (("fp32dev", abi.PREC_FP32DEV, 1), ("fp32dev_quadrature", abi.PREC_FP32DEV, 0),
+            ("fp64", abi.PREC_FP64, None))
+    for key, prec, sep in legs:
         with Subblock(cfg.params(check_negative_ce=False, precision=prec), cfg.nucleation()) as sb:
+            if sep is not None:
+                sb.set_tuning(abi.TUNE_BEAM_SEP, sep)
             apply_substrate(sb, gr, recs)
             sb.set_kernel_timing(True)
             t0 = time.perf_counter()
@@ -358,27 +368,33 @@ def run_am_track(steps=60):
             wall = time.perf_counter() - t0
             k_ms, _ = sb.kernel_time_ms()
             ce = sb.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1]
-            fields[prec] = (ce, rep)
-            key = "fp32dev" if prec == abi.PREC_FP32DEV else "fp64"
+            temp = sb.download(abi.FIELD_TEMP)[1:-1, 1:-1, 1:-1] if key != "fp32dev_quadrature" else None
+            fields[key] = (ce, rep, temp)
             out[key] = {"value": round(n * steps / wall / 1e6, 1), "unit": "Mcell-updates/s",
                         "ms_per_step": round(wall / steps * 1e3, 3), "k_solute_ms": round(k_ms, 3),
-                        "beam_terms_per_s": round(n * 400 / (k_ms * 1e-3), 1) if k_ms > 0 else None,
                         "grains": rep.num_grains - 1, "solid_fraction": rep.solid_sum / n}
-    (c32, r32), (c64, r64) = fields[abi.PREC_FP32DEV], fields[abi.PREC_FP64]
-    out["error_vs_fp64"] = {
-        "solute_l2_rel": float(np.linalg.norm(c32 - c64) / np.linalg.norm(c64)),
-        "solid_fraction_abs": abs(r32.solid_sum - r64.solid_sum) / n,
-        "grains_fp32": r32.num_grains - 1, "grains_fp64": r64.num_grains - 1,
-        "solute_drift_with_ledger": (r32.solute_sum + r32.solute_ledger - n * cfg.c_init) / (n * cfg.c_init)}
+            if key != "fp32dev":
+                out[key]["beam_terms_per_s"] = round(n * 400 / (k_ms * 1e-3), 1) if k_ms > 0 else None
+            log("am track %s: %.3f ms/step" % (key, wall / steps * 1e3))
+    c64, r64, t64 = fields["fp64"]
+    for key in ("fp32dev", "fp32dev_quadrature"):
+        c32, r32, t32 = fields[key]
+        err = {"solute_l2_rel": float(np.linalg.norm(c32 - c64) / np.linalg.norm(c64)),
+               "solid_fraction_abs": abs(r32.solid_sum - r64.solid_sum) / n,
+               "grains_fp32": r32.num_grains - 1, "grains_fp64": r64.num_grains - 1,
+               "solute_drift_with_ledger": (r32.solute_sum + r32.solute_ledger - n * cfg.c_init) / (n * cfg.c_init)}
+        if t32 is not None:
+            err["temperature_rel_max"] = float(np.max(np.abs(t32 - t64) / np.maximum(np.abs(t64), 1.0)))
+        out[key]["error_vs_fp64"] = err
     # The compiled quadrature loop issues 51 FP64 VALU instructions per term
     # (two divisions and an exp each); MI355X issues 78.6e12 / 2 FP64 lane
     # operations per second.
     valu_per_term = 51
-    for key in ("fp32dev", "fp64"):
+    for key in ("fp32dev_quadrature", "fp64"):
         t = out[key]["beam_terms_per_s"]
         out[key]["valu_frac"] = round(t * valu_per_term / (78.6e12 / 2), 3) if t else None
     out.update({"cells": n, "steps": steps, "dt": cfg.dt, "substrate_grains": len(recs) - 1,
-                "bound": "FP64 VALU (beam quadrature on every solid cell)", "valu_ops_per_term": valu_per_term})
+                "valu_ops_per_term": valu_per_term})
     return out
 
 
@@ -489,7 +505,7 @@ def cpu_baseline(cfg, target_s):
                       % (steps + 1, dt, nthr)}
 
 
-def capture_footprint(steps=(250, 500, 750, 1000)):
+def capture_footprint(shape, steps=(250, 500, 750, 1000)):
     """The capture kernel's read floor on the directional workload: at each
     of `steps`, the interface-list length and the distinct 128-byte lines of
     gr, dc and fs that its solid face neighbours occupy, plus the list's own
@@ -499,6 +515,7 @@ def capture_footprint(steps=(250, 500, 750, 1000)):
     from mumatscale_amd.lib import Subblock
     cfg = directional_config()
     sb = Subblock(cfg.params(check_negative_ce=False), cfg.nucleation(), device=0)
+    apply_shape(sb, shape)
     done = 0
     for t in steps:
         rep = sb.step(t - done)
@@ -553,21 +570,26 @@ def main():
                     help="run only the 2x2x2 coupled-ranks leg (and its one-block comparison), print it and exit")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched-ensemble leg (--steps steps), print it and exit")
+    ap.add_argument("--am-only", action="store_true",
+                    help="run only the AM single-track leg (--am-steps steps), print it and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the CA step on one GPU; --gpus %d is not supported"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
-    if args.capture_footprint:
-        capture_footprint()
-        return
     dims = tuple(int(v) for v in args.size.split("x"))
     shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd, "lds": args.lds, "chunk": args.capture_chunk}
+    if args.capture_footprint:
+        capture_footprint(shape)
+        return
     if args.coupled_only:
         print(json.dumps(run_coupled(dims, args.extra_steps, args.warmup, shape)), flush=True)
         return
     if args.ensemble_only:
         print(json.dumps(run_ensemble(args.steps, 20)), flush=True)
+        return
+    if args.am_only:
+        print(json.dumps(run_am_track(args.am_steps)), flush=True)
         return
     nx, ny, nz = dims
     cfg = directional_config(nx, ny, nz)

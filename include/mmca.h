@@ -164,7 +164,8 @@ typedef struct mmca_report {
 #define MMCA_FIELD_GR         3  /* int32  */
 #define MMCA_FIELD_DC         4  /* double[3] per cell, packed x,y,z (decentered_t) */
 #define MMCA_FIELD_D          5  /* double: half-diagonal of solid cells (upload: the array MMCA_TUNE_TRACK_D keeps) */
-#define MMCA_FIELD_TEMP       6  /* double, download only: temperature at current timestep */
+#define MMCA_FIELD_TEMP       6  /* double, download only: temperature at current timestep (the
+                                    separable form on the interior with MMCA_TUNE_BEAM_SEP) */
 #define MMCA_FIELD_THRESHOLD  7  /* float, download only: INFINITY where no site */
 #define MMCA_FIELD_MOLD       8  /* int8, download only */
 #define MMCA_FIELD_OCE        9  /* double: ce before the last diffusion */
@@ -326,6 +327,12 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
 #define MMCA_TUNE_TRACK_D    6   /* 1: keep d as the reference's array holds it (growth.c:392-403 writes
                                     solid cells only, so a cell that melts keeps its last value);
                                     MMCA_FIELD_D then gives that array */
+#define MMCA_TUNE_BEAM_SEP   7   /* FP32-deviation Gaussian-beam blocks only (default 1 there): the
+                                    beam temperature (temperature.c:104-112) as the sum over its 400
+                                    nodes of the product of an x, a y and a z factor, computed per
+                                    step for the whole block (within 1e-13 of the quadrature,
+                                    not its bits); 0: the quadrature per cell.  FP64 storage
+                                    always uses the quadrature (value 1 is refused there) */
 int  mmca_set_tuning(mmca_ctx *ctx, int key, int value);
 int  mmca_get_tuning(mmca_ctx *ctx, int key, int *value);
 

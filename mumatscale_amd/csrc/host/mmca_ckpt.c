@@ -89,6 +89,167 @@ neighbours(const int nsub[3], const int pad[3], int r, int32_t nb[6][2])
     nb[F_LEFT][0] = sb_id(nsub, pad, c[0] - 1, c[1], c[2]);
 }
 
+/* ---- halo planes as the reference's face exchanges leave them -------------
+ * The reference's arrays carry in their halo planes what its face exchanges
+ * last put there (ExchangeFacesForVar / FinishExchangeForVar,
+ * calculate.c:55-191): for every face with a neighbour rank other than
+ * itself, in face order, the neighbour's whole face layer, edge and corner
+ * halo cells included (computeHaloInfo / computeFaceInfo, packing.c:217-300),
+ * as it was when all ranks sent.  cl, fs and dc are sent at the start of a
+ * step, gr after partitioning, d after grow_octahedra; kernels write no halo
+ * cell.  This build exchanges face cells only, at its own points of the step,
+ * so the writer rebuilds those planes from the fields of the last steps: a
+ * face cell needs the neighbour's layer one exchange back, an edge cell two
+ * (through the neighbour's halo), a corner three. */
+
+/* halo plane of face f (receiver) and the neighbour's layer it receives */
+static void
+face_planes(int f, const int n[3], int *axis, int *halo, int *layer)
+{
+    switch (f)
+    {
+    case F_TOP: *axis = 2; *halo = n[2] + 1; *layer = 1; break;
+    case F_BOTTOM: *axis = 2; *halo = 0; *layer = n[2]; break;
+    case F_LEFT: *axis = 0; *halo = 0; *layer = n[0]; break;
+    case F_RIGHT: *axis = 0; *halo = n[0] + 1; *layer = 1; break;
+    case F_FRONT: *axis = 1; *halo = 0; *layer = n[1]; break;
+    default: *axis = 1; *halo = n[1] + 1; *layer = 1; break;
+    }
+}
+
+/* One exchange over all ranks: dst[r] receives, in its halo planes, the
+ * face layers of src[neighbour] (all ranks send before any unpacks, so src
+ * and dst are different arrays).  esize bytes per cell. */
+static void
+exchange_halos(const int nsub[3], const int pad[3], const int n[3], size_t esize, unsigned char *const *dst,
+               unsigned char *const *src)
+{
+    const int nr = nsub[0] * nsub[1] * nsub[2];
+    const size_t sx = (size_t) n[0] + 2, sxy = sx * ((size_t) n[1] + 2);
+    const size_t stride[3] = { 1, sx, sxy };
+    for (int r = 0; r < nr; r++)
+    {
+        int32_t nb[6][2];
+        neighbours(nsub, pad, r, nb);
+        for (int f = 0; f < 6; f++)
+        {
+            const int q = nb[f][0];
+            if (q < 0 || q == r)
+                continue;
+            int axis, halo, layer;
+            face_planes(f, n, &axis, &halo, &layer);
+            const int u = axis == 0 ? 1 : 0, v = axis == 2 ? 1 : 2;   /* the plane's two axes */
+            for (int b = 0; b < n[v] + 2; b++)
+                for (int a = 0; a < n[u] + 2; a++)
+                {
+                    const size_t base = (size_t) a * stride[u] + (size_t) b * stride[v];
+                    memcpy(dst[r] + (base + (size_t) halo * stride[axis]) * esize,
+                           src[q] + (base + (size_t) layer * stride[axis]) * esize, esize);
+                }
+        }
+    }
+}
+
+/* interior of a (1 for cells with no coordinate in a halo layer) */
+static int
+is_interior(const int n[3], int x, int y, int z)
+{
+    return x >= 1 && x <= n[0] && y >= 1 && y <= n[1] && z >= 1 && z <= n[2];
+}
+
+/* number of coordinates of (x, y, z) in a halo layer */
+static int
+halo_axes(const int n[3], int x, int y, int z)
+{
+    return (x == 0 || x == n[0] + 1) + (y == 0 || y == n[1] + 1) + (z == 0 || z == n[2] + 1);
+}
+
+/* Rebuilds the halo of one field over all ranks.  snap[k][r] is rank r's
+ * padded array after the k-th step of the history (oldest first, k <
+ * nsnap; NULL where not kept), cur[r] the array after the checkpoint's step,
+ * into which the result goes.  sent_at_start: the field is sent at the start
+ * of a step (cl, fs, dc: the previous step's values); otherwise during the
+ * step, with that step's interior (d).  own_faces: the face cells of this
+ * build's arrays already hold what the reference's exchange put there (gr,
+ * exchanged at the same point of the step), only edges and corners are
+ * rebuilt. */
+static int
+rebuild_halos(const int nsub[3], const int pad[3], const int n[3], size_t esize, unsigned char *const *const *snap,
+              int nsnap, unsigned char *const *cur, int sent_at_start, int own_faces)
+{
+    const int nr = nsub[0] * nsub[1] * nsub[2];
+    const size_t P = ((size_t) n[0] + 2) * ((size_t) n[1] + 2) * ((size_t) n[2] + 2);
+    int k0 = 0;
+    while (k0 < nsnap && !snap[k0])
+        k0++;
+    if (k0 == nsnap)
+        return 0;                                   /* no history: the arrays stay as they are */
+    unsigned char **model = calloc((size_t) nr, sizeof(*model));
+    unsigned char **next = calloc((size_t) nr, sizeof(*next));
+    int rc = (model && next) ? 0 : -1;
+    for (int r = 0; r < nr && rc == 0; r++)
+    {
+        model[r] = malloc(P * esize);
+        next[r] = malloc(P * esize);
+        if (!model[r] || !next[r])
+            rc = -1;
+        else
+            memcpy(model[r], snap[k0][r], P * esize);
+    }
+    /* each later step: the exchange of that step, then that step's interior */
+    for (int k = k0 + 1; k <= nsnap && rc == 0; k++)
+    {
+        unsigned char *const *after = k < nsnap ? snap[k] : cur;
+        if (!after)
+            break;
+        for (int r = 0; r < nr; r++)
+            memcpy(next[r], model[r], P * esize);
+        if (!sent_at_start)
+            /* sent during the step: the sender's interior is already this step's */
+            for (int r = 0; r < nr; r++)
+                for (int z = 1; z <= n[2]; z++)
+                    for (int y = 1; y <= n[1]; y++)
+                    {
+                        const size_t i = (((size_t) z * (n[1] + 2) + y) * (n[0] + 2) + 1) * esize;
+                        memcpy(model[r] + i, after[r] + i, (size_t) n[0] * esize);
+                    }
+        exchange_halos(nsub, pad, n, esize, next, model);
+        for (int r = 0; r < nr; r++)
+            for (int z = 0; z < n[2] + 2; z++)
+                for (int y = 0; y < n[1] + 2; y++)
+                    for (int x = 0; x < n[0] + 2; x++)
+                    {
+                        const int h = halo_axes(n, x, y, z);
+                        if (h == 0 || (own_faces && h == 1))
+                        {
+                            const size_t i = (((size_t) z * (n[1] + 2) + y) * (n[0] + 2) + x) * esize;
+                            memcpy(next[r] + i, after[r] + i, esize);
+                        }
+                    }
+        unsigned char **t = model;
+        model = next;
+        next = t;
+    }
+    if (rc == 0)
+        for (int r = 0; r < nr; r++)
+            for (int z = 0; z < n[2] + 2; z++)
+                for (int y = 0; y < n[1] + 2; y++)
+                    for (int x = 0; x < n[0] + 2; x++)
+                        if (!is_interior(n, x, y, z))
+                        {
+                            const size_t i = (((size_t) z * (n[1] + 2) + y) * (n[0] + 2) + x) * esize;
+                            memcpy(cur[r] + i, model[r] + i, esize);
+                        }
+    for (int r = 0; r < nr && model && next; r++)
+    {
+        free(model[r]);
+        free(next[r]);
+    }
+    free(model);
+    free(next);
+    return rc ? fail("out of host memory") : 0;
+}
+
 /* ---- chunks compressed off the HDF5 call ---------------------------------
  * A dataset here is one chunk (the whole array).  Its stored bytes are what
  * the dataset's filters make of the array: H5Z shuffle (byte b of element i
@@ -507,10 +668,57 @@ write_task_file(const mmca_run_cfg *cfg, const int nsub[3], int r, const mmca_pa
     return 0;
 }
 
+/* The halo planes of cl, fs, dc, gr and d over all ranks from the history
+ * (rebuild_halos); the ranks share one subblock shape. */
+static int
+rank_halos(const int nsub[3], const mmca_params *p, const mmca_ckpt_snap *const *hist, rank_arrays *ra, int nr)
+{
+    const int n[3] = { p->nx, p->ny, p->nz };
+    const int pad[3] = { p->pad_bndy[0], p->pad_bndy[1], p->pad_bndy[2] };
+    /* field, bytes per cell, sent at the start of a step, face cells already right */
+    static const struct { int arr; size_t esize; int at_start, own_faces; } F[5] = {
+        { A_CL, 8, 1, 0 }, { A_FS, 8, 1, 0 }, { A_DC, 24, 1, 0 }, { A_GR, 4, 1, 1 }, { A_D, 8, 0, 0 } };
+    unsigned char **ptr = calloc((size_t) (MMCA_CKPT_HISTORY + 1) * nr, sizeof(*ptr));
+    if (!ptr)
+        return fail("out of host memory");
+    int rc = 0;
+    for (int v = 0; v < 5 && rc == 0; v++)
+    {
+        unsigned char **snap[MMCA_CKPT_HISTORY];
+        unsigned char **cur = ptr + (size_t) MMCA_CKPT_HISTORY * nr;
+        for (int k = 0; k < MMCA_CKPT_HISTORY; k++)
+        {
+            snap[k] = NULL;
+            if (!hist[k])
+                continue;
+            unsigned char **row = ptr + (size_t) k * nr;
+            int whole = 1;
+            for (int r = 0; r < nr; r++)
+            {
+                const mmca_ckpt_snap *h = &hist[k][r];
+                void *a = F[v].arr == A_CL ? (void *) h->cl : F[v].arr == A_FS ? (void *) h->fs
+                        : F[v].arr == A_DC ? (void *) h->dc : F[v].arr == A_GR ? (void *) h->gr : (void *) h->d;
+                row[r] = a;
+                whole = whole && a;
+            }
+            if (whole)
+                snap[k] = row;
+        }
+        for (int r = 0; r < nr; r++)
+            cur[r] = ra[r].a[F[v].arr];
+        rc = rebuild_halos(nsub, pad, n, F[v].esize, (unsigned char *const *const *) snap, MMCA_CKPT_HISTORY,
+                           cur, F[v].at_start, F[v].own_faces);
+    }
+    free(ptr);
+    return rc;
+}
+
 int
 mmca_ckpt_write(const mmca_run_cfg *cfg, const int nsub[3], mmca_batch *bt, mmca_ctx *const *doms,
-                const mmca_params *ps, double *const *oce, uint64_t ts, const uint64_t *renumber, int64_t nren)
+                const mmca_params *ps, const mmca_ckpt_snap *const *hist, uint64_t ts, const uint64_t *renumber,
+                int64_t nren)
 {
+    const mmca_ckpt_snap *last = hist ? hist[MMCA_CKPT_HISTORY - 1] : NULL;
     const int nr = nsub[0] * nsub[1] * nsub[2];
     if (nr < 1 || (!bt && nr != 1))
         return fail("bad rank layout");
@@ -541,7 +749,7 @@ mmca_ckpt_write(const mmca_run_cfg *cfg, const int nsub[3], mmca_batch *bt, mmca
         rc = fail("out of host memory");
     for (int r = 0; r < nr && rc == 0; r++)
     {
-        rc = fetch_rank(doms[r], &ps[r], oce ? oce[r] : NULL, &ra[r]);
+        rc = fetch_rank(doms[r], &ps[r], last ? last[r].ce : NULL, &ra[r]);
         const size_t P = (size_t) (ps[r].nx + 2) * (ps[r].ny + 2) * (ps[r].nz + 2);
         for (int v = 0; v < NARR && rc == 0; v++)
         {
@@ -551,6 +759,8 @@ mmca_ckpt_write(const mmca_run_cfg *cfg, const int nsub[3], mmca_batch *bt, mmca
             z->esize = ARR_ESIZE[v];
         }
     }
+    if (rc == 0 && nr > 1 && hist)
+        rc = rank_halos(nsub, &ps[0], hist, ra, nr);
     zjob *gj = jobs ? &jobs[(size_t) nr * NARR] : NULL;
     if (rc == 0)
     {

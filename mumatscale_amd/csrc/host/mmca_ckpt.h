@@ -25,19 +25,34 @@
 #include "mmca.h"
 #include "mmca_ctrl.h"
 
+/* Steps before a checkpoint whose fields the writer uses: the halo planes
+ * of the reference's arrays hold what its face exchanges of those steps
+ * left there (an edge cell comes through a neighbour's halo, a corner
+ * through two). */
+#define MMCA_CKPT_HISTORY 3
+
+/* One rank's padded fields after a step (as mmca_download_field gives them;
+ * NULL where not kept). */
+typedef struct mmca_ckpt_snap {
+    double *ce, *cl, *fs, *dc, *d;
+    int32_t *gr;
+} mmca_ckpt_snap;
+
 /*
  * Writes the checkpoint of timestep ts.  bt: the coupled batch of the ranks,
  * or NULL for one block (doms[0]).  doms[r], ps[r]: rank r's subblock and
  * parameters, rank r at Cart coordinates x = r % nsub[0],
- * y = r / nsub[0] % nsub[1], z = the rest.  oce[r]: rank r's ce before the
- * last step's diffusion (padded, FP64), or NULL to store ce.  The blocks
- * must keep the reference's d array (MMCA_TUNE_TRACK_D).  renumber[0..nren):
- * the XDMF renumbering to store with it.  Returns 0, or -1 with a message
- * on stderr.
+ * y = r / nsub[0] % nsub[1], z = the rest.  hist[k][r]: rank r's fields after
+ * step ts - MMCA_CKPT_HISTORY + k, or hist[k] NULL where that step was not
+ * kept: ce after step ts - 1 is the file's oce (ce before the last step's
+ * diffusion; ce itself without it), and with several ranks cl, fs, dc, gr
+ * and d of those steps give the halo planes.  The blocks must keep the
+ * reference's d array (MMCA_TUNE_TRACK_D).  renumber[0..nren): the XDMF
+ * renumbering to store with it.  Returns 0, or -1 with a message on stderr.
  */
 int mmca_ckpt_write(const mmca_run_cfg *cfg, const int nsub[3], mmca_batch *bt, mmca_ctx *const *doms,
-                    const mmca_params *ps, double *const *oce, uint64_t ts, const uint64_t *renumber,
-                    int64_t nren);
+                    const mmca_params *ps, const mmca_ckpt_snap *const *hist, uint64_t ts,
+                    const uint64_t *renumber, int64_t nren);
 
 /*
  * Restores the run written by mmca_ckpt_write into bt / doms, created with the

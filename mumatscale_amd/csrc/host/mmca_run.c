@@ -43,6 +43,7 @@
 
 #include "mmca.h"
 #include "mmca_ctrl.h"
+#include "mmca_ckpt.h"
 #include "mmca_vtk.h"
 #include "mmca_xdmf.h"
 
@@ -61,7 +62,7 @@ typedef struct io_lib {
     int (*vwrite)(mmca_vtk *, mmca_ctx *const *, const mmca_params *, uint64_t);
     void (*vclose)(mmca_vtk *);
     int (*cwrite)(const mmca_run_cfg *, const int *, mmca_batch *, mmca_ctx *const *, const mmca_params *,
-                  double *const *, uint64_t, const uint64_t *, int64_t);
+                  const mmca_ckpt_snap *const *, uint64_t, const uint64_t *, int64_t);
     int (*cread)(const mmca_run_cfg *, const int *, mmca_batch *, mmca_ctx *const *, const mmca_params *,
                  uint64_t *, int64_t, int64_t *);
     mmca_xdmf *x;                  /* DataFormat XDMF */
@@ -254,7 +255,7 @@ load_checkpoint(mmca_batch *bt, mmca_ctx *ctx, io_lib *out)
  */
 static int
 ckpt_write(const io_lib *out, const mmca_run_cfg *cfg, const int nsub[3], mmca_batch *bt, mmca_ctx *const *doms,
-           const mmca_params *ps, double *const *oce, uint64_t t)
+           const mmca_params *ps, const mmca_ckpt_snap *const *hist, uint64_t t)
 {
     const int64_t nren = out->x ? out->get_renumber(out->x, NULL, 0) : 0;
     uint64_t *ren = malloc(sizeof(uint64_t) * (size_t) (nren > 0 ? nren : 1));
@@ -262,7 +263,7 @@ ckpt_write(const io_lib *out, const mmca_run_cfg *cfg, const int nsub[3], mmca_b
         return -1;
     if (out->x)
         out->get_renumber(out->x, ren, nren);
-    const int rc = out->cwrite(cfg, nsub, bt, doms, ps, oce, t, ren, nren);
+    const int rc = out->cwrite(cfg, nsub, bt, doms, ps, hist, t, ren, nren);
     free(ren);
     if (rc)
     {
@@ -271,6 +272,84 @@ ckpt_write(const io_lib *out, const mmca_run_cfg *cfg, const int nsub[3], mmca_b
     }
     printf("Checkpoint at timestep %lu \n", (unsigned long) t);
     return 0;
+}
+
+/* Fields kept from the steps before a checkpoint (mmca_ckpt_write): ce one
+ * step before it (its oce) and, with several ranks, cl, fs, dc, gr and d of
+ * the last MMCA_CKPT_HISTORY steps (its halo planes).  Slot k holds step
+ * checkpoint - MMCA_CKPT_HISTORY + k. */
+typedef struct ckpt_hist {
+    int depth;                                  /* steps kept: 1 (one rank) or MMCA_CKPT_HISTORY */
+    mmca_ckpt_snap *slot[MMCA_CKPT_HISTORY];
+    uint64_t step[MMCA_CKPT_HISTORY];
+} ckpt_hist;
+
+static int
+hist_alloc(ckpt_hist *h, const mmca_params *ps, int nr)
+{
+    h->depth = nr > 1 ? MMCA_CKPT_HISTORY : 1;
+    for (int k = 0; k < MMCA_CKPT_HISTORY; k++)
+    {
+        h->step[k] = UINT64_MAX;
+        h->slot[k] = NULL;
+        if (k < MMCA_CKPT_HISTORY - h->depth)
+            continue;
+        if (!(h->slot[k] = calloc((size_t) nr, sizeof(mmca_ckpt_snap))))
+            return -1;
+        for (int r = 0; r < nr; r++)
+        {
+            const size_t P = (size_t) (ps[r].nx + 2) * (ps[r].ny + 2) * (ps[r].nz + 2);
+            mmca_ckpt_snap *c = &h->slot[k][r];
+            if (!(c->ce = malloc(P * sizeof(double))))
+                return -1;
+            if (nr > 1 && (!(c->cl = malloc(P * sizeof(double))) || !(c->fs = malloc(P * sizeof(double))) ||
+                           !(c->dc = malloc(3 * P * sizeof(double))) || !(c->d = malloc(P * sizeof(double))) ||
+                           !(c->gr = malloc(P * sizeof(int32_t)))))
+                return -1;
+        }
+    }
+    return 0;
+}
+
+/* keeps the fields after step t when a checkpoint follows within the depth */
+static int
+hist_take(ckpt_hist *h, mmca_ctx *const *doms, int nr, long long freq, uint64_t t)
+{
+    if (freq <= 0)
+        return 0;
+    const uint64_t cp = (t / (uint64_t) freq + 1) * (uint64_t) freq;
+    if (cp - t > (uint64_t) h->depth)
+        return 0;
+    const int k = MMCA_CKPT_HISTORY - (int) (cp - t);
+    for (int r = 0; r < nr; r++)
+    {
+        mmca_ckpt_snap *c = &h->slot[k][r];
+        if (mmca_download_field(doms[r], MMCA_FIELD_CE, c->ce) != MMCA_OK ||
+            (c->cl && mmca_download_field(doms[r], MMCA_FIELD_CL, c->cl) != MMCA_OK) ||
+            (c->fs && mmca_download_field(doms[r], MMCA_FIELD_FS, c->fs) != MMCA_OK) ||
+            (c->dc && mmca_download_field(doms[r], MMCA_FIELD_DC, c->dc) != MMCA_OK) ||
+            (c->d && mmca_download_field(doms[r], MMCA_FIELD_D, c->d) != MMCA_OK) ||
+            (c->gr && mmca_download_field(doms[r], MMCA_FIELD_GR, c->gr) != MMCA_OK))
+        {
+            fprintf(stderr, "download of the fields before a checkpoint failed\n");
+            return -1;
+        }
+    }
+    h->step[k] = t;
+    return 0;
+}
+
+/* the next step before `next` at which hist_take keeps fields */
+static uint64_t
+hist_stop(const ckpt_hist *h, long long freq, uint64_t t, uint64_t next)
+{
+    if (freq <= 0)
+        return next;
+    const uint64_t cp = (t / (uint64_t) freq + 1) * (uint64_t) freq;
+    for (int j = h->depth; j >= 1; j--)
+        if (cp > (uint64_t) j && cp - (uint64_t) j > t)
+            return cp - (uint64_t) j < next ? cp - (uint64_t) j : next;
+    return next;
 }
 
 static int
@@ -525,17 +604,14 @@ main(int argc, char **argv)
     const int one_sub[3] = { 1, 1, 1 };
     const int *lay = nr > 1 ? cfg.nsub : one_sub;
     const int h5cp = (checkpoint_freq > 0 || restart) && ckpt_load(&out) == 0;
-    double **oce = NULL;
+    ckpt_hist hist;
+    memset(&hist, 0, sizeof(hist));
     if (h5cp)
     {
         if (check(bt ? mmca_batch_set_tuning(bt, MMCA_TUNE_TRACK_D, 1) : mmca_set_tuning(doms[0], MMCA_TUNE_TRACK_D, 1),
                   "track d"))
             return 1;
-        oce = calloc((size_t) nr, sizeof(double *));
-        for (int r = 0; oce && r < nr; r++)
-            if (!(oce[r] = malloc(sizeof(double) * (size_t) (ps[r].nx + 2) * (ps[r].ny + 2) * (ps[r].nz + 2))))
-                oce = NULL;
-        if (!oce)
+        if (checkpoint_freq > 0 && hist_alloc(&hist, ps, nr))
         {
             fprintf(stderr, "out of host memory\n");
             return 1;
@@ -554,12 +630,10 @@ main(int argc, char **argv)
             return 1;
         printf("Restarting from timestep %lu\n", (unsigned long) t);
     }
-    /* oce of the next checkpoint: ce one step before it */
-    const int keep_oce = h5cp && checkpoint_freq > 0;
-    if (keep_oce && (t + 1) % checkpoint_freq == 0)
-        for (int r = 0; r < nr; r++)
-            if (check(mmca_download_field(doms[r], MMCA_FIELD_CE, oce[r]), "download ce"))
-                return 1;
+    /* the fields the next checkpoint needs from the steps before it */
+    const int keep_hist = h5cp && checkpoint_freq > 0;
+    if (keep_hist && hist_take(&hist, doms, nr, checkpoint_freq, t))
+        return 1;
     printf("Initial Subblocks assigned.  Starting calculations.\n");
     const double total_volume = (p.gdim[0] * p.gdim[1] * p.gdim[2]) * pow(cfg.cell_size, 3);
     const double cell_vol = pow(cfg.cell_size, 3.0);
@@ -586,8 +660,8 @@ main(int argc, char **argv)
         }
         if (checkpoint_freq > 0 && (t / checkpoint_freq + 1) * checkpoint_freq < next)
             next = (t / checkpoint_freq + 1) * checkpoint_freq;
-        if (keep_oce && next % checkpoint_freq == 0 && next - 1 > t)
-            next = next - 1;                  /* stop before the checkpoint for its oce */
+        if (keep_hist)
+            next = hist_stop(&hist, checkpoint_freq, t, next);   /* stop before the checkpoint for its history */
         if (fin && (fin > t ? fin : t + 1) < next)
             next = fin > t ? fin : t + 1;
         if (max_steps >= 0 && (uint64_t) max_steps < next)
@@ -625,16 +699,20 @@ main(int argc, char **argv)
             status = MMCA_ERR_ARG;
             break;
         }
-        if (checkpoint_freq > 0 && t % checkpoint_freq == 0 &&
-            (h5cp ? ckpt_write(&out, &cfg, lay, bt, doms, ps, oce, t) : save_checkpoint(bt, doms[0], &out, t)))
+        if (checkpoint_freq > 0 && t % checkpoint_freq == 0)
         {
-            status = MMCA_ERR_ARG;
-            break;
+            const mmca_ckpt_snap *hp[MMCA_CKPT_HISTORY];
+            for (int k = 0; k < MMCA_CKPT_HISTORY; k++)
+                hp[k] = (t + (uint64_t) k >= MMCA_CKPT_HISTORY &&
+                         hist.step[k] == t + (uint64_t) k - MMCA_CKPT_HISTORY) ? hist.slot[k] : NULL;
+            if (h5cp ? ckpt_write(&out, &cfg, lay, bt, doms, ps, hp, t) : save_checkpoint(bt, doms[0], &out, t))
+            {
+                status = MMCA_ERR_ARG;
+                break;
+            }
         }
-        if (keep_oce && (t + 1) % checkpoint_freq == 0)
-            for (int r = 0; r < nr && status == MMCA_OK; r++)
-                if (check(mmca_download_field(doms[r], MMCA_FIELD_CE, oce[r]), "download ce"))
-                    status = MMCA_ERR_ARG;
+        if (keep_hist && hist_take(&hist, doms, nr, checkpoint_freq, t))
+            status = MMCA_ERR_ARG;
         if (status != MMCA_OK)
             break;
         int done = 0;

@@ -180,6 +180,7 @@ int build_consts(const mmca_params &p, Consts &c)
     c.max_total_grains = p.max_total_grains;
     c.check_neg = p.check_negative_ce ? 1 : 0;
     c.am = (p.temp_model == MMCA_TEMP_AM_BEAM) ? 1 : 0;
+    c.beam_sep = (c.am && c.f32) ? 1 : 0;   // FP64 storage keeps the reference's quadrature
     c.am_layers = p.am_layers;
     c.am_tracks = p.am_tracks;
     c.gnsbx = p.gnsbx;

@@ -18,6 +18,7 @@
 
 namespace mmca {
 hipError_t launch_nucleate(const View &v, int mode, int cur, hipStream_t s);
+hipError_t launch_beam_field(const DevBufs &b, const Consts &c, int adv, hipStream_t s);
 hipError_t launch_solute(const View &v, int am, int f32, int zm_by, int lds, int diff, int fsc, int rec, int cur, int nxt,
                          hipStream_t s);
 hipError_t launch_mask_from_gr(const DevBufs &b, const Consts &c, hipStream_t s);
@@ -211,6 +212,44 @@ int ensure_d(mmca_ctx *ctx)
     return MMCA_OK;
 }
 
+// Grain ids of a gr field from outside (an upload, a restart image): every
+// id indexes the grain table of max_total_grains + 1 slots.
+int check_grain_ids(const void *gr, size_t n, long long max_total_grains)
+{
+    for (size_t i = 0; i < n; i++) {
+        int32_t g;
+        memcpy(&g, (const char *) gr + i * sizeof(int32_t), sizeof(g));
+        if ((long long) g > max_total_grains)
+            return MMCA_ERR_ARG;
+    }
+    return MMCA_OK;
+}
+
+// buffers of the separable beam temperature (Consts::beam_sep)
+int ensure_beam(mmca_ctx *ctx)
+{
+    DevBufs &b = ctx->b;
+    const Consts &c = ctx->c;
+    if (!b.temp) {
+        TRY(dev_alloc(&b.temp, (size_t) c.P));
+        TRY(dev_alloc(&b.beam_x, (size_t) BEAM_NT * ((c.nx + 127) / 128 * 128)));   // rows of whole 128-column tiles
+        TRY(dev_alloc(&b.beam_y, (size_t) BEAM_NT * c.ny));
+        TRY(dev_alloc(&b.beam_z, (size_t) BEAM_NT * c.nz));
+        TRY(dev_alloc(&b.beam_w, (size_t) BEAM_NT * c.ny * c.nz));
+        HIPCHK(hipMemsetAsync(b.temp, 0, (size_t) c.P * sizeof(double), ctx->stream));
+        ctx->view_dirty = true;
+    }
+    return MMCA_OK;
+}
+
+// the separable beam field of timestep + adv, before a kernel that reads it
+int refresh_beam(mmca_ctx *ctx, int adv, hipStream_t s)
+{
+    if (ctx->c.beam_sep)
+        HIPCHK(launch_beam_field(ctx->b, ctx->c, adv, s));
+    return MMCA_OK;
+}
+
 int check_status(mmca_ctx *ctx)
 {
     TRY(sync_counters(ctx));
@@ -300,6 +339,7 @@ int enqueue_iteration(mmca_ctx *ctx, bool advance)
 {
     const int nxt = ctx->cur ^ 1;
     TRY(ensure_view(ctx));
+    TRY(refresh_beam(ctx, advance ? 1 : 0, ctx->stream));
     HIPCHK(launch_nucleate(ctx->view, advance ? 7 : 3, ctx->cur, ctx->stream));
     const size_t ev0 = ctx->ev_used;
     if (ctx->timing) {
@@ -311,8 +351,8 @@ int enqueue_iteration(mmca_ctx *ctx, bool advance)
         HIPCHK(hipEventRecord(ctx->ev[ev0], ctx->stream));
     }
     const int rec = ctx->b.rec != nullptr;
-    HIPCHK(launch_solute(ctx->view, ctx->c.am, ctx->c.f32, rec ? 0 : zm_rows(ctx->c), ctx->c.zm_lds, 1, 1, rec, ctx->cur, nxt,
-                         ctx->stream));
+    HIPCHK(launch_solute(ctx->view, ctx->c.am && !ctx->c.beam_sep, ctx->c.f32, rec ? 0 : zm_rows(ctx->c),
+                         ctx->c.zm_lds, 1, 1, rec, ctx->cur, nxt, ctx->stream));
     if (ctx->timing) {
         HIPCHK(hipEventRecord(ctx->ev[ev0 + 1], ctx->stream));
         ctx->ev_used = ev0 + 2;
@@ -538,6 +578,8 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     ctx->rand_capacity = (int) ctx->p.max_total_grains;
     TRY(dev_alloc(&b.rand_rot, (size_t) ctx->rand_capacity * 9));
     TRY(dev_alloc(&b.cnt, 1));
+    if (c.beam_sep)
+        TRY(ensure_beam(ctx));
     b.nsites = 0;
     // SetupInitialConditions (setup.c:58-65): ce = cl = Cinit, all else 0
     if (c.f32) {   // deviations from cinit
@@ -588,7 +630,8 @@ int mmca_destroy(mmca_ctx *ctx)
                      b.diff_id, b.nbr, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
                      b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.g_dom, b.fixed_rot,
                      b.rand_rot,
-                     b.oce, b.d, b.red_part, b.cnt, ctx->d_red, ctx->d_desc, ctx->d_off, ctx->io_buf };
+                     b.oce, b.d, b.temp, b.beam_x, b.beam_y, b.beam_z, b.beam_w, b.red_part, b.cnt, ctx->d_red,
+                     ctx->d_desc, ctx->d_off, ctx->io_buf };
     for (void *p : ptrs)
         if (p)
             (void) hipFree(p);
@@ -668,6 +711,8 @@ int mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report)
     ctx->hc.new_grains = 0;
     HIPCHK(hipMemcpyAsync(&ctx->b.cnt->new_grains, &ctx->hc.new_grains, sizeof(int),
                           hipMemcpyHostToDevice, ctx->stream));
+    const int cur0 = ctx->cur;
+    const uint64_t t0 = ctx->hc.timestep;
     int st = courant_check(ctx);
     if (st != MMCA_OK && nsteps > 0) {
         ctx->hc.status = st;
@@ -705,6 +750,10 @@ int mmca_step(mmca_ctx *ctx, int nsteps, int want_sums, mmca_report *report)
         if (want_sums)
             HIPCHK(launch_reduce(ctx->b, ctx->c, ctx->cur, ctx->d_red, ctx->stream));
         TRY(sync_counters(ctx));
+        // stopped on an error: the buffers of the failed step stay current
+        // (the steps issued after it skipped every kernel)
+        if (ctx->hc.status != MMCA_OK)
+            ctx->cur = cur0 ^ (int) ((ctx->hc.timestep - t0) & 1);
         if (ctx->timing)
             TRY(collect_timing(ctx));
     }
@@ -779,6 +828,7 @@ int mmca_cell_nucleation(mmca_ctx *ctx, int32_t *n_new)
         return MMCA_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     TRY(ensure_view(ctx));
+    TRY(refresh_beam(ctx, 0, ctx->stream));
     HIPCHK(launch_nucleate(ctx->view, 1, ctx->cur, ctx->stream));
     TRY(sync_counters(ctx));
     if (n_new)
@@ -935,6 +985,7 @@ int mmca_upload_field(mmca_ctx *ctx, int field, const void *host)
         HIPCHK(hipMemcpy(b.fs[1], host, P * sizeof(double), hipMemcpyHostToDevice));
         break;
     case MMCA_FIELD_GR:
+        TRY(check_grain_ids(host, P, ctx->p.max_total_grains));
         HIPCHK(hipMemcpy(b.gr, host, P * sizeof(int), hipMemcpyHostToDevice));
         break;
     case MMCA_FIELD_DC:
@@ -1000,6 +1051,7 @@ int mmca_download_field(mmca_ctx *ctx, int field, void *host)
     case MMCA_FIELD_TEMP: {
         double *tmp = nullptr;
         TRY(dev_alloc(&tmp, P));
+        TRY(refresh_beam(ctx, 0, ctx->stream));
         HIPCHK(launch_fill_temp(b, c, tmp, ctx->stream));
         HIPCHK(hipMemcpyAsync(host, tmp, P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1280,6 +1332,12 @@ int state_body(mmca_ctx *ctx, StateIO &io, const StateHeader &h, Counters &hc, T
     }
     if (!(h.flags & MMCA_STATE_NO_FIELDS)) {
         TRY(io.dev(b.fs[cur], P * sizeof(double)));
+        if (io.load) {
+            // the kernels index the grain table with gr
+            if (io.cap - io.pos < (int64_t) (P * sizeof(int)))
+                return MMCA_ERR_ARG;
+            TRY(check_grain_ids(io.in + io.pos, P, ctx->p.max_total_grains));
+        }
         TRY(io.dev(b.gr, P * sizeof(int)));
         TRY(io.dev(b.dc, 3 * P * sizeof(double)));
     }
@@ -1463,6 +1521,8 @@ int mmca_io_fields(mmca_ctx *ctx, float *fs_io, float *ce_io, float *temperature
         TRY(dev_alloc(&ctx->io_buf, 4 * P));
     float *d_fs = ctx->io_buf, *d_ce = d_fs + P, *d_t = d_ce + P;
     int *d_gr = (int *) (d_t + P);
+    if (temperature_io)
+        TRY(refresh_beam(ctx, 0, ctx->stream));
     HIPCHK(launch_io_fields(ctx->b, ctx->c, ctx->cur, fs_io ? d_fs : nullptr, ce_io ? d_ce : nullptr,
                             temperature_io ? d_t : nullptr, gr_io ? d_gr : nullptr, ctx->stream));
     if (fs_io)
@@ -1565,6 +1625,14 @@ int mmca_set_tuning(mmca_ctx *ctx, int key, int value)
         ctx->c.track_d = value ? 1 : 0;
         ctx->view_dirty = true;
         return MMCA_OK;
+    case MMCA_TUNE_BEAM_SEP:
+        if (value && !(ctx->c.am && ctx->c.f32))
+            return MMCA_ERR_ARG;   // FP64 storage keeps the reference's quadrature
+        if (value)
+            TRY(ensure_beam(ctx));
+        ctx->c.beam_sep = value ? 1 : 0;
+        ctx->view_dirty = true;
+        return MMCA_OK;
     default:
         return MMCA_ERR_ARG;
     }
@@ -1592,6 +1660,9 @@ int mmca_get_tuning(mmca_ctx *ctx, int key, int *value)
         return MMCA_OK;
     case MMCA_TUNE_TRACK_D:
         *value = ctx->c.track_d;
+        return MMCA_OK;
+    case MMCA_TUNE_BEAM_SEP:
+        *value = ctx->c.beam_sep;
         return MMCA_OK;
     default:
         return MMCA_ERR_ARG;
@@ -1685,6 +1756,8 @@ int batch_enqueue(mmca_batch *bt, int cur)
 {
     const Consts &c0 = bt->dom[0]->c;
     const int nxt = cur ^ 1;
+    for (mmca_ctx *c : bt->dom)
+        TRY(refresh_beam(c, 1, bt->stream));
     if (bt->coupled) {
         HIPCHK(launch_halo_exchange(bt->view, 0, cur, bt->max_face, bt->stream));
         HIPCHK(launch_nucleate_coupled(bt->view, cur, bt->stream));
@@ -1699,8 +1772,8 @@ int batch_enqueue(mmca_batch *bt, int cur)
         }
         HIPCHK(hipEventRecord(bt->ev[e0], bt->stream));
     }
-    HIPCHK(launch_solute(bt->view, c0.am, c0.f32, bt->zm_planes > 0 ? bt->zm_by : 0, bt->zm_lds, 1, 1, 0, cur, nxt,
-                         bt->stream));
+    HIPCHK(launch_solute(bt->view, c0.am && !c0.beam_sep, c0.f32, bt->zm_planes > 0 ? bt->zm_by : 0, bt->zm_lds, 1,
+                         1, 0, cur, nxt, bt->stream));
     if (bt->timing) {
         HIPCHK(hipEventRecord(bt->ev[e0 + 1], bt->stream));
         bt->ev_used = e0 + 2;
@@ -1979,6 +2052,10 @@ int mmca_batch_set_tuning(mmca_batch *bt, int key, int value)
             TRY(mmca_set_tuning(d, MMCA_TUNE_TRACK_D, value));
         bt->track_d = value ? 1 : 0;
         break;
+    case MMCA_TUNE_BEAM_SEP:
+        for (mmca_ctx *d : bt->dom)
+            TRY(mmca_set_tuning(d, MMCA_TUNE_BEAM_SEP, value));
+        break;
     case MMCA_TUNE_GRAPH:
         bt->use_graph = value != 0;
         return MMCA_OK;
@@ -2028,6 +2105,11 @@ int mmca_batch_step(mmca_batch *bt, int nsteps, int want_sums, mmca_report *repo
         }
     }
     HIPCHK(hipStreamSynchronize(bt->stream));
+    // a domain that stops on an error keeps the buffers of its last step
+    // (its kernels skip every later step while the batch's parity turns)
+    std::vector<std::pair<int, uint64_t>> start(bt->dom.size());
+    for (size_t d = 0; d < bt->dom.size(); d++)
+        start[d] = { bt->dom[d]->cur, bt->dom[d]->hc.timestep };
     int done = 0;
     while (done < nsteps) {
         // chunk bound: drawn orientations cover every grain the chunk can create
@@ -2059,8 +2141,10 @@ int mmca_batch_step(mmca_batch *bt, int nsteps, int want_sums, mmca_report *repo
         if (done < nsteps)
             TRY(batch_sync(bt));
     }
-    for (mmca_ctx *c : bt->dom) {
-        c->cur = bt->cur;
+    TRY(batch_sync(bt));
+    for (size_t d = 0; d < bt->dom.size(); d++) {
+        mmca_ctx *c = bt->dom[d];
+        c->cur = c->hc.status == MMCA_OK ? bt->cur : start[d].first ^ (int) ((c->hc.timestep - start[d].second) & 1);
         if (want_sums)
             HIPCHK(launch_reduce(c->b, c->c, c->cur, c->d_red, bt->stream));
     }

@@ -114,7 +114,6 @@ temp_model(const Consts &c, double sim_time, int x, int y, int z)
 //   Tl[i] = sqrt(lalpha*lt) * (ls*ls + 4.*lalpha*lt),
 //   Z[i] = lz*lz / (4.*lalpha*lt) for one plane (lz depends on z only),
 // and a copy of the exp table, so that its lookups are LDS reads.
-constexpr int BEAM_NT = 400;
 struct BeamTab {
     double vlt[BEAM_NT], A[BEAM_NT], Tl[BEAM_NT], Z[BEAM_NT];
     uint64_t exp_tab[EXP_TAB_WORDS];

@@ -41,11 +41,16 @@ struct Consts {
     int check_neg;
     // temperature (temperature.c:24-126)
     int am;                 // 1: Gaussian-beam model
+    int beam_sep;           // 1 (FP32-deviation AM only): the beam temperature from its separable
+                            // factors, computed per step into DevBufs::temp
     int am_layers, am_tracks, gnsbx;
     double dt, cell_size, origin[3];
     double t_init, cool_rate, gradient, grad_coef, velocity, velo_coef;
     double lpow, lv, lsd, lab, lhs, liniT, lthcon, rho, lcp, x0, y0, z0;
 };
+
+// quadrature nodes of the Gaussian-beam temperature (temperature.c:104)
+constexpr int BEAM_NT = 400;
 
 // Device-resident counters of one subblock.
 struct Counters {
@@ -99,6 +104,11 @@ struct DevBufs {
     double *fixed_rot;               // 9 per fixed nucleation point
     double *rand_rot;                // 9 per random orientation draw
     int rand_cap;                    // random orientations uploaded
+    // separable beam temperature (Consts::beam_sep): T per padded cell, the
+    // factor tables X[i][x], Y[i][y], Zf[i][z] (BEAM_NT rows each) and
+    // W[(z-1) ny + y-1][i] = Y[i][y] Zf[i][z]
+    double *temp;
+    double *beam_x, *beam_y, *beam_z, *beam_w;
     // single-operator buffers (allocated on first use)
     double *oce;
     double *d;

@@ -22,6 +22,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mmca_device.h"
 #include "mmca_internal.h"
 
@@ -196,7 +198,7 @@ k_nuc_detect(View V, int advance, int cur)
         int rem = (int) (idx - z * c.sxy);
         int y = rem / c.sx;
         int x = rem - y * c.sx;
-        double T = temp_at(c, sim_time, x, y, z);
+        double T = c.beam_sep ? G(b.temp)[idx] : temp_at(c, sim_time, x, y, z);
         int layerno = 1, initnuc = 1;
         if (c.am)
         {
@@ -591,6 +593,21 @@ partition_cell(const Consts &c, TempFn temp, double ce_new, double ce_prev, doub
     *g_out = g;
 }
 
+// The temperature the partitioning of cell idx reads (tempUpdateSB_int,
+// temperature.c:218-262): the model itself, or with beam_sep the separable
+// beam field computed into b.temp before the step (the kernels then run
+// their non-AM form).
+template <bool AM>
+__device__ __forceinline__ double
+temp_step(const DevBufs &b, const Consts &c, double sim_time, long long idx, int x, int y, int z)
+{
+    if constexpr (AM)
+        return temp_model<true>(c, sim_time, x, y, z);
+    if (c.beam_sep)
+        return G(b.temp)[idx];
+    return temp_model<false>(c, sim_time, x, y, z);
+}
+
 // growth.c:163-165: error() ends every rank.  In a coupled batch the status
 // goes to every domain of the view, so that each rank's later kernels (and
 // the halo exchange) stop at the same point; the ranks share one timestep.
@@ -690,7 +707,7 @@ k_solute(View V, int cur, int nxt)
                 const int g_old = G(b.gr)[idx];
                 double cl_out, fs_out;
                 int g, branch;
-                partition_cell(c, [&]() { return temp_model<AM>(c, sim_time, x, j, k); }, ce_new, ce_prev,
+                partition_cell(c, [&]() { return temp_step<AM>(b, c, sim_time, idx, x, j, k); }, ce_new, ce_prev,
                                conc, lspfs, g_old, &cl_out, &fs_out, &g, &branch);
                 br_rec = branch;
                 st_cl<F32>(b, c, nxt, idx, cl_out);
@@ -939,7 +956,7 @@ k_solute_zm(View V, int cur, int nxt)
                     if constexpr (AM)
                         return beam_temp_tab(c, *beam, beam->Z, sim_time, x, j, k);
                     else
-                        return temp_model<false>(c, sim_time, x, j, k);
+                        return temp_step<false>(b, c, sim_time, idx, x, j, k);
                 },
                 ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out, &g_new, &branch);
             st_cl<F32>(b, c, nxt, idx, cl_out);
@@ -1200,7 +1217,7 @@ k_solute_zl(View V, int cur, int nxt)
                     if constexpr (AM)
                         return beam_temp_tab(c, *beam, s_z[s & 1], sim_time, x, j, k);
                     else
-                        return temp_model<false>(c, sim_time, x, j, k);
+                        return temp_step<false>(b, c, sim_time, idx, x, j, k);
                 },
                 ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out, &g_new, &branch);
             st_cl<F32>(b, c, nxt, idx, cl_out);
@@ -1728,7 +1745,246 @@ k_reduce_final(DevBufs b, int nblk, double *out)
 }
 
 // ---------------------------------------------------------------------------
-// Derived fields for download.
+// Separable Gaussian-beam temperature (Consts::beam_sep, FP32-deviation AM).
+// Term i of the quadrature of temp_model<true> (temperature.c:104-112),
+//   ldt * exp(-((lx + lv t_i)^2 + ly^2) / A_i - lz^2 / (4 alpha t_i)) / Tl_i,
+// is the product X_i(x) Y_i(y) Zf_i(z) of
+//   X_i = exp(-(lx + lv t_i)^2 / A_i),   Y_i = ldt / Tl_i * exp(-ly^2 / A_i),
+//   Zf_i = exp(-lz^2 / (4 alpha t_i)),
+// so that over the interior of a block T = liniT + Tfront * sum_i X_i(x)
+// W_i(y, z) with W_i(y, z) = Y_i(y) Zf_i(z): a product of a (BEAM_NT x nx)
+// and a ((nz ny) x BEAM_NT) matrix, 400 multiply-adds per cell and
+// 400 (nx + ny + nz) exps per step in place of 400 exps per cell.  The three
+// factors round separately and the sum runs in another order, so T differs
+// from the quadrature's in its last bits.  Every factor is at most 1 (the
+// exponents are not positive), so a factor that underflows stands for a term
+// below the smallest double.
+
+// beam position and amplitude at sim_time (temperature.c:62-103)
+struct BeamNow {
+    double xnow, ynow, znow, Tfront;
+};
+
+__device__ __forceinline__ BeamNow
+beam_now(const Consts &c, double sim_time)
+{
+    const double invlv = 1. / c.lv;
+    double xlen = c.gnsbx * c.nx * c.cell_size;
+    xlen = xlen + c.lsd;
+    double tline = xlen * invlv;
+    const double tpause = 1.5e-4 * invlv;
+    tline = tline + tpause;
+    double trackno = 0;
+    if (c.am_tracks > 1)
+        trackno = floor(sim_time / tline);
+    BeamNow n;
+    n.xnow = c.x0 + (sim_time - trackno * tline) * c.lv;
+    n.ynow = c.y0 + trackno * c.lhs;
+    n.znow = c.z0;
+    const double lpow = (sim_time < 1e-4) ? 0.0 : c.lpow;
+    n.Tfront = c.lab * lpow * beam_lalpha(c) / (5.568328 * c.lthcon);
+    return n;
+}
+
+// row pitch of the X table: whole BF_COLS-column tiles, zero past nx
+constexpr int BF_COLS = 128;
+__host__ __device__ __forceinline__ int beam_x_pitch(const Consts &c) { return (c.nx + BF_COLS - 1) / BF_COLS * BF_COLS; }
+
+// the factor tables X[i][x-1] (row pitch beam_x_pitch), Y[i][y-1], Zf[i][z-1]
+// at the time of timestep cnt->timestep + adv
+__global__ void __launch_bounds__(256)
+k_beam_tabs(DevBufs b, Consts c, int adv)
+{
+    const MMCA_GLOBAL Counters *cnt = G(b.cnt);
+    if (cnt->status != MMCA_OK)
+        return;
+    const double sim_time = (cnt->timestep + (unsigned long long) adv) * c.dt;
+    const BeamNow bn = beam_now(c, sim_time);
+    const double ls = c.lsd / (2. * 1.414);
+    const double lalpha = beam_lalpha(c);
+    const double ldt = beam_ldt(c);
+    const int xp = beam_x_pitch(c);
+    const long long L = (long long) xp + c.ny + c.nz;
+    for (long long e = (long long) blockIdx.x * blockDim.x + threadIdx.x; e < L * BEAM_NT;
+         e += (long long) gridDim.x * blockDim.x)
+    {
+        const int i = (int) (e / L);
+        const int q = (int) (e - (long long) i * L);
+        const double lt = (0.5 + i) * ldt;
+        const double A = ls * ls + 4. * lalpha * lt;
+        if (q < xp)
+        {
+            double v = 0.0;                       // columns past nx pad the row
+            if (q < c.nx)
+            {
+                const double rx = c.origin[0] + ((c.off[0] + q + 1) + 0.5) * c.cell_size;
+                const double u = (rx - bn.xnow) + c.lv * lt;
+                v = exp_glibc(-(u * u) / A, exp_tab);
+            }
+            G(b.beam_x)[(long long) i * xp + q] = v;
+        }
+        else if (q < xp + c.ny)
+        {
+            const int y = q - xp;
+            const double ry = c.origin[1] + ((c.off[1] + y + 1) + 0.5) * c.cell_size;
+            const double ly = ry - bn.ynow;
+            const double Tl = sqrt(lalpha * lt) * (ls * ls + 4. * lalpha * lt);
+            G(b.beam_y)[(long long) i * c.ny + y] = ldt / Tl * exp_glibc(-(ly * ly) / A, exp_tab);
+        }
+        else
+        {
+            const int z = q - xp - c.ny;
+            const double rz = c.origin[2] + ((c.off[2] + z + 1) + 0.5) * c.cell_size;
+            const double lz = rz - bn.znow;
+            G(b.beam_z)[(long long) i * c.nz + z] = exp_glibc(-(lz * lz) / (4. * lalpha * lt), exp_tab);
+        }
+    }
+}
+
+// W[(z-1) ny + y-1][i] = Y[i][y-1] * Zf[i][z-1]
+__global__ void __launch_bounds__(256)
+k_beam_w(DevBufs b, Consts c)
+{
+    if (G(b.cnt)->status != MMCA_OK)
+        return;
+    const long long n = (long long) c.nz * c.ny * BEAM_NT;
+    for (long long e = (long long) blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long) gridDim.x * blockDim.x)
+    {
+        const long long row = e / BEAM_NT;
+        const int i = (int) (e - row * BEAM_NT);
+        const int z = (int) (row / c.ny);
+        const int y = (int) (row - (long long) z * c.ny);
+        G(b.beam_w)[e] = G(b.beam_y)[(long long) i * c.ny + y] * G(b.beam_z)[(long long) i * c.nz + z];
+    }
+}
+
+// T over the interior into out (padded indexing).  Workgroup: one plane,
+// BF_COLS columns (two per lane) and BF_WAVES x BF_ROWS rows; the X rows of
+// BF_IC quadrature nodes at a time are staged in LDS, the W values of a
+// wave's rows are the same for all its lanes (scalar loads).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+constexpr int BF_ROWS = 8;
+constexpr int BF_WAVES = 4;
+constexpr int BF_IC = 40;
+static_assert(BEAM_NT % BF_IC == 0, "node chunks must tile the quadrature");
+
+__global__ void __launch_bounds__(64 * BF_WAVES)
+k_beam_field(DevBufs b, Consts c, int adv, double *out)
+{
+    __shared__ double sX[BF_IC][BF_COLS];
+    const MMCA_GLOBAL Counters *cnt = G(b.cnt);
+    if (cnt->status != MMCA_OK)
+        return;                                   // uniform over the workgroup
+    const int k = blockIdx.z + 1;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xb = blockIdx.x * BF_COLS;          // first column (0-based interior)
+    const int x = xb + 2 * lane + 1;              // this lane's columns x, x + 1
+    const int jw = blockIdx.y * (BF_WAVES * BF_ROWS) + wave * BF_ROWS + 1;
+    const double sim_time = (cnt->timestep + (unsigned long long) adv) * c.dt;
+    const BeamNow bn = beam_now(c, sim_time);
+    const double rz = c.origin[2] + ((c.off[2] + k) + 0.5) * c.cell_size;
+    auto store = [&](int r, double t0, double t1) {
+        const int j = jw + r;
+        if (j > c.ny)
+            return;
+        const long long idx = (long long) k * c.sxy + (long long) j * c.sx + x;
+        if (x <= c.nx)
+            out[idx] = t0;
+        if (x + 1 <= c.nx)
+            out[idx + 1] = t1;
+    };
+    if (!(rz <= bn.znow + c.cell_size))
+    {
+        for (int r = 0; r < BF_ROWS; ++r)
+            store(r, 0.0, 0.0);
+        return;
+    }
+    if (bn.Tfront == 0.0)
+    {
+        // every term is finite, so liniT + Tfront * sum is liniT + 0
+        const double t = c.liniT + bn.Tfront * 0.0;
+        for (int r = 0; r < BF_ROWS; ++r)
+            store(r, t, t);
+        return;
+    }
+    const MMCA_GLOBAL double *W[BF_ROWS];
+#pragma unroll
+    for (int r = 0; r < BF_ROWS; ++r)
+        W[r] = G(b.beam_w) + ((long long) (k - 1) * c.ny + min(jw + r, c.ny) - 1) * BEAM_NT;
+    double acc0[BF_ROWS], acc1[BF_ROWS];
+#pragma unroll
+    for (int r = 0; r < BF_ROWS; ++r)
+        acc0[r] = acc1[r] = 0.0;
+    // staging: every thread moves STG column pairs of the chunk, all loads
+    // issued before the first LDS write
+    constexpr int STG = BF_IC * BF_COLS / 2 / (64 * BF_WAVES);
+    static_assert(STG * 2 * 64 * BF_WAVES == BF_IC * BF_COLS, "staging must cover the chunk");
+    const int xp = beam_x_pitch(c);
+    const MMCA_GLOBAL double *Xt = G(b.beam_x) + xb;
+    for (int i0 = 0; i0 < BEAM_NT; i0 += BF_IC)
+    {
+        dvec2 v[STG];
+#pragma unroll
+        for (int q = 0; q < STG; ++q)
+        {
+            const int e = threadIdx.x + q * 64 * BF_WAVES;
+            const int ii = e / (BF_COLS / 2), x2 = 2 * (e - ii * (BF_COLS / 2));
+            v[q] = *reinterpret_cast<const MMCA_GLOBAL dvec2 *>(Xt + (long long) (i0 + ii) * xp + x2);
+        }
+        __syncthreads();                          // the previous chunk is consumed
+#pragma unroll
+        for (int q = 0; q < STG; ++q)
+        {
+            const int e = threadIdx.x + q * 64 * BF_WAVES;
+            const int ii = e / (BF_COLS / 2), x2 = 2 * (e - ii * (BF_COLS / 2));
+            *reinterpret_cast<dvec2 *>(&sX[ii][x2]) = v[q];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ii = 0; ii < BF_IC; ++ii)
+        {
+            const dvec2 xv = *reinterpret_cast<const dvec2 *>(&sX[ii][2 * lane]);
+#pragma unroll
+            for (int r = 0; r < BF_ROWS; ++r)
+            {
+                const double w = W[r][i0 + ii];
+                acc0[r] = fma(xv.x, w, acc0[r]);
+                acc1[r] = fma(xv.y, w, acc1[r]);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < BF_ROWS; ++r)
+        store(r, c.liniT + bn.Tfront * acc0[r], c.liniT + bn.Tfront * acc1[r]);
+}
+
+// The separable temperature of timestep cnt->timestep + adv into b.temp
+hipError_t launch_beam_field(const DevBufs &b, const Consts &c, int adv, hipStream_t s)
+{
+    const long long tabs = (long long) BEAM_NT * (beam_x_pitch(c) + c.ny + c.nz);
+    hipLaunchKernelGGL(k_beam_tabs, dim3((unsigned) std::min<long long>((tabs + 255) / 256, 4096)), dim3(256), 0, s,
+                       b, c, adv);
+    const long long nw = (long long) c.nz * c.ny * BEAM_NT;
+    hipLaunchKernelGGL(k_beam_w, dim3((unsigned) std::min<long long>((nw + 255) / 256, 65536)), dim3(256), 0, s, b, c);
+    const dim3 grid((unsigned) ((c.nx + BF_COLS - 1) / BF_COLS),
+                    (unsigned) ((c.ny + BF_WAVES * BF_ROWS - 1) / (BF_WAVES * BF_ROWS)), (unsigned) c.nz);
+    hipLaunchKernelGGL(k_beam_field, grid, dim3(64 * BF_WAVES), 0, s, b, c, adv, b.temp);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Derived fields for download.  With beam_sep the interior temperature is
+// the separable one, refreshed into b.temp by the caller; the halo cells
+// (read by no step) keep the quadrature.
+__device__ __forceinline__ double
+field_temp(const DevBufs &b, const Consts &c, long long i, int x, int y, int z)
+{
+    if (c.beam_sep && x >= 1 && x <= c.nx && y >= 1 && y <= c.ny && z >= 1 && z <= c.nz)
+        return G(b.temp)[i];
+    return temp_at(c, G(b.cnt)->timestep * c.dt, x, y, z);
+}
+
 __global__ void
 k_fill_temp(DevBufs b, Consts c, double *out)
 {
@@ -1739,7 +1995,7 @@ k_fill_temp(DevBufs b, Consts c, double *out)
     long long rem = i - z * c.sxy;
     int y = (int) (rem / c.sx);
     int x = (int) (rem - (long long) y * c.sx);
-    out[i] = temp_at(c, b.cnt->timestep * c.dt, x, y, z);
+    out[i] = field_temp(b, c, i, x, y, z);
 }
 
 // cache_io_data (loop.c:74-113): FP32 copies of fs, ce and the temperature
@@ -1764,7 +2020,7 @@ k_io_fields(DevBufs b, Consts c, int cur, float *fs_io, float *ce_io, float *t_i
         long long rem = i - z * c.sxy;
         int y = (int) (rem / c.sx);
         int x = (int) (rem - (long long) y * c.sx);
-        t_io[i] = (float) temp_at(c, G(b.cnt)->timestep * c.dt, x, y, z);
+        t_io[i] = (float) field_temp(b, c, i, x, y, z);
     }
 }
 

@@ -1,6 +1,11 @@
 import os
 import sys
 
+# Parallel workers (pytest -n) each run the OpenMP oracle: one thread per
+# worker, or the spinning thread teams of the workers starve each other.
+if "PYTEST_XDIST_WORKER" in os.environ:
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

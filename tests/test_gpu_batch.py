@@ -64,8 +64,9 @@ def test_batch_restart_image_continues_bit_for_bit():
 
 def test_corrupted_restart_image_is_refused():
     """A restart image whose interface list names a cell outside the
-    interior is refused (capture would write through it), and a block whose
-    load failed part way refuses to step."""
+    interior, or whose gr names a grain beyond the table, is refused (capture
+    would write, the kernels read, through it), and a block whose load failed
+    part way refuses to step.  An uploaded gr field is checked the same way."""
     from mumatscale_amd.lib import Subblock
     cfg = load_case("c1_directional")[0]
     with Subblock(cfg.params(), cfg.nucleation()) as a, Subblock(cfg.params(), cfg.nucleation()) as b:
@@ -81,3 +82,16 @@ def test_corrupted_restart_image_is_refused():
             b.load_state(bytes(img))
         with pytest.raises(abi.MmcaError):
             b.step(1)
+        gr = a.download(abi.FIELD_GR)
+        img = bytearray(a.save_state())
+        at = bytes(img).find(gr.tobytes())
+        assert at > 0
+        g = np.frombuffer(img, dtype=np.int32, count=gr.size, offset=at).copy()
+        g[gr.size // 2] = cfg.max_total_grains + 1             # one past the grain table
+        img[at:at + 4 * gr.size] = g.tobytes()
+        with Subblock(cfg.params(), cfg.nucleation()) as c:
+            with pytest.raises(abi.MmcaError):
+                c.load_state(bytes(img))
+            with pytest.raises(abi.MmcaError):
+                c.upload(abi.FIELD_GR, g.reshape(gr.shape))
+            c.upload(abi.FIELD_GR, gr)                            # in range: accepted

@@ -3,11 +3,15 @@ batched ensemble, on the GPU.
 
 AM track (Gaussian-beam temperature, temperature.c:62-123, over a Voronoi
 substrate): the FP64 run is bit-identical with the oracle; the FP32-deviation
-run (ce, cl stored as FP32 deviations from c0, every other value and all
-arithmetic FP64) closes its solute balance with the ledger to 1e-12, has the
-FP64 run's grain count, and stays within stated bounds of its solid fraction
-and solute field.  Both at a block the oracle finishes in seconds (300 steps,
-beam melting and regrowing part of the substrate) and at the full
+runs (ce, cl stored as FP32 deviations from c0, every other value and all
+arithmetic FP64) close their solute balance with the ledger to 1e-12, have
+the FP64 run's grain count, and stay within stated bounds of its solid
+fraction and solute field.  The FP32-deviation block takes its beam
+temperature from the separable factors of the quadrature's terms (the
+default there); its temperature field stays within 1e-13 (relative) of the
+quadrature's.  The FP32 form with the per-cell quadrature is checked the same
+way.  Both at a block the oracle finishes in seconds (300 steps, beam
+melting and regrowing part of the substrate) and at the full
 1024 x 512 x 256 track for 40 steps (the beam turns on at step 16).
 
 Ensemble (96 domains of sides in {32..128}, 3 K mean bulk undercooling,
@@ -32,6 +36,13 @@ AUTOTUNE_SHAPES = {(0, 4), (8, 4), (16, 4), (8, 8), (16, 8), (32, 8), (64, 8), (
 def bits(a):
     a = np.ascontiguousarray(a)
     return a.view(np.int64) if a.dtype == np.float64 else a
+
+
+TEMP_REL = 1e-13       # separable beam temperature against the quadrature
+
+
+def _temp_rel(t32, t64):
+    return float(np.max(np.abs(t32 - t64) / np.maximum(np.abs(t64), 1.0)))
 
 
 def _fp32_errors(c64, r64, c32, r32, n, c_init):
@@ -66,14 +77,21 @@ def test_am_track_fp64_bit_exact_and_fp32_bounds():
     fs[1:-1, 1:-1, 1:-1] = 1.0
     orc.upload(abi.FIELD_FS, fs)
     melted = 0
-    with Subblock(p64, cfg.nucleation()) as a, Subblock(p32, cfg.nucleation()) as b:
+    with Subblock(p64, cfg.nucleation()) as a, Subblock(p32, cfg.nucleation()) as b, \
+            Subblock(p32, cfg.nucleation()) as q:
+        with pytest.raises(abi.MmcaError):
+            a.set_tuning(abi.TUNE_BEAM_SEP, 1)    # FP64 storage keeps the quadrature
+        assert b.get_tuning(abi.TUNE_BEAM_SEP) == 1
+        q.set_tuning(abi.TUNE_BEAM_SEP, 0)
         apply_substrate(a, gr, recs)
         apply_substrate(b, gr, recs)
+        apply_substrate(q, gr, recs)
         done = 0
         for t in (10, 17, 20, 25, 30, 40, 50, 60, 80, 120, 300):
             assert orc.step(t - done) == abi.OK
             ra = a.step(t - done, want_sums=True)
             rb = b.step(t - done, want_sums=True)
+            rq = q.step(t - done, want_sums=True)
             done = t
             g = a.download(abi.FIELD_GR)
             melted = max(melted, int(np.sum(g[1:-1, 1:-1, 1:-1] <= 0)))
@@ -84,13 +102,18 @@ def test_am_track_fp64_bit_exact_and_fp32_bounds():
             assert ra.num_grains == orc.num_grains
             assert np.array_equal(a.interface_list(), orc.diff_id()), t
             c64 = a.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1]
-            c32 = b.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1]
-            e = _fp32_errors(c64, ra, c32, rb, n, cfg.c_init)
-            print("  fp32 vs fp64:", e, flush=True)
-            assert e["ledger"] <= 1e-12
-            assert e["grains"][0] == e["grains"][1]
-            assert e["fs"] <= 1e-6
-            assert e["l2"] <= 1e-6
+            t64 = a.download(abi.FIELD_TEMP)[1:-1, 1:-1, 1:-1]
+            tb = _temp_rel(b.download(abi.FIELD_TEMP)[1:-1, 1:-1, 1:-1], t64)
+            print("  separable temperature vs quadrature: %.3e" % tb, flush=True)
+            assert tb <= TEMP_REL
+            for name, blk, rep in (("separable", b, rb), ("quadrature", q, rq)):
+                c32 = blk.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1]
+                e = _fp32_errors(c64, ra, c32, rep, n, cfg.c_init)
+                print("  fp32 (%s) vs fp64:" % name, e, flush=True)
+                assert e["ledger"] <= 1e-12
+                assert e["grains"][0] == e["grains"][1]
+                assert e["fs"] <= 1e-6
+                assert e["l2"] <= 1e-6
     assert melted > n // 4                       # the beam melted part of the substrate
     assert int(np.sum(g[1:-1, 1:-1, 1:-1] <= 0)) < melted // 100     # which regrew
 
@@ -108,10 +131,13 @@ def test_am_track_full_size_fp32_against_fp64():
             apply_substrate(sb, gr, recs)
             rep = sb.step(steps, want_sums=True)
             out[prec] = (sb.download(abi.FIELD_CE)[1:-1, 1:-1, 1:-1], rep,
-                         int(np.sum(sb.download(abi.FIELD_GR)[1:-1, 1:-1, 1:-1] <= 0)))
-    (c64, r64, liq64), (c32, r32, liq32) = out[abi.PREC_FP64], out[abi.PREC_FP32DEV]
+                         int(np.sum(sb.download(abi.FIELD_GR)[1:-1, 1:-1, 1:-1] <= 0)),
+                         sb.download(abi.FIELD_TEMP)[1:-1, 1:-1, 1:-1])
+    (c64, r64, liq64, t64), (c32, r32, liq32, t32) = out[abi.PREC_FP64], out[abi.PREC_FP32DEV]
     e = _fp32_errors(c64, r64, c32, r32, n, cfg.c_init)
-    print("full track: liquid cells", liq64, liq32, "errors", e, flush=True)
+    tr = _temp_rel(t32, t64)
+    print("full track: liquid cells", liq64, liq32, "errors", e, "temperature", tr, flush=True)
+    assert tr <= TEMP_REL
     assert liq64 > 0 and liq32 == liq64
     assert e["ledger"] <= 1e-12
     assert e["grains"][0] == e["grains"][1]

@@ -554,7 +554,7 @@ def main():
                     help="z-marching solute kernel, planes per workgroup (0: cell-parallel)")
     ap.add_argument("--zm-rows", type=int, default=16, help="rows per z-marching workgroup (4, 8, 16)")
     ap.add_argument("--xcd", type=int, default=1, help="1: contiguous tile runs per XCD")
-    ap.add_argument("--lds", type=int, default=0,
+    ap.add_argument("--lds", type=int, default=1,
                     help="1: z-marching kernel with its planes staged in LDS by direct loads (k_solute_zl)")
     ap.add_argument("--capture-chunk", type=int, default=0,
                     help=">0: capture deals its list slots in chunks of this many, pinned to XCDs")

@@ -321,7 +321,7 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
 #define MMCA_TUNE_ZM_ROWS    1   /* rows per z-marching workgroup: 4, 8 or 16 */
 #define MMCA_TUNE_XCD_REMAP  2   /* 1: workgroups of one XCD take contiguous tiles */
 #define MMCA_TUNE_GRAPH      3   /* batches: 1 = replay steps from a captured HIP graph */
-#define MMCA_TUNE_ZM_LDS     4   /* 1: z-marching kernel with its planes staged in LDS by direct loads */
+#define MMCA_TUNE_ZM_LDS     4   /* 1 (default): z-marching kernel with its planes staged in LDS by direct loads; 0: in registers */
 #define MMCA_TUNE_CAPTURE_CHUNK 5 /* >0: capture deals list slots in chunks of this many (rounded up to
                                       256), chunk q on XCD q % 8; 0: 256-slot groups over the whole grid */
 #define MMCA_TUNE_TRACK_D    6   /* 1: keep d as the reference's array holds it (growth.c:392-403 writes

@@ -141,7 +141,7 @@ int build_consts(const mmca_params &p, Consts &c)
     c.zm_planes = 64;                   // z-marching solute kernel by default
     c.zm_by = 16;
     c.xcd_remap = 1;
-    c.zm_lds = 0;
+    c.zm_lds = 1;                       // planes staged in LDS (k_solute_zl)
     c.capture_chunk = 0;
     c.track_d = 0;
     if (p.precision != MMCA_PREC_FP64 && p.precision != MMCA_PREC_FP32DEV)

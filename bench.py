@@ -43,8 +43,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILES = {"fp64": os.path.join(ROOT, "profiles", "k_solute_zm_directional_256x256x512_fp64.md"),
-                    "fp32": os.path.join(ROOT, "profiles", "k_solute_zm_directional_256x256x512_fp32dev.md")}
+TRAFFIC_PROFILES = {"fp64": os.path.join(ROOT, "profiles", "k_solute_zl_directional_256x256x512_fp64.md"),
+                    "fp32": os.path.join(ROOT, "profiles", "k_solute_zl_directional_256x256x512_fp32dev.md")}
 
 
 def bandwidth_probe(cells, reps=20):

@@ -317,7 +317,7 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
                             double *angles_out, double *rotmat_out);
 
 /* --- tuning (no effect on results) --------------------------------------- */
-#define MMCA_TUNE_ZM_PLANES  0   /* >0: z-marching solute kernel, planes per workgroup; 0: cell-parallel */
+#define MMCA_TUNE_ZM_PLANES  0   /* 1..1024: z-marching solute kernel, planes per workgroup; 0: cell-parallel */
 #define MMCA_TUNE_ZM_ROWS    1   /* rows per z-marching workgroup: 4, 8 or 16 */
 #define MMCA_TUNE_XCD_REMAP  2   /* 1: workgroups of one XCD take contiguous tiles */
 #define MMCA_TUNE_GRAPH      3   /* batches: 1 = replay steps from a captured HIP graph */

@@ -231,12 +231,19 @@ int ensure_beam(mmca_ctx *ctx)
     DevBufs &b = ctx->b;
     const Consts &c = ctx->c;
     if (!b.temp) {
+        // X rows of whole 128-column tiles, W rows in whole 64-row tiles (zero past ny nz)
+        const size_t wrows = ((size_t) c.ny * c.nz + 63) / 64 * 64;
         TRY(dev_alloc(&b.temp, (size_t) c.P));
-        TRY(dev_alloc(&b.beam_x, (size_t) BEAM_NT * ((c.nx + 127) / 128 * 128)));   // rows of whole 128-column tiles
+        TRY(dev_alloc(&b.beam_x, (size_t) BEAM_NT * ((c.nx + 127) / 128 * 128)));
         TRY(dev_alloc(&b.beam_y, (size_t) BEAM_NT * c.ny));
         TRY(dev_alloc(&b.beam_z, (size_t) BEAM_NT * c.nz));
-        TRY(dev_alloc(&b.beam_w, (size_t) BEAM_NT * c.ny * c.nz));
+        TRY(dev_alloc(&b.beam_w, (size_t) BEAM_NT * wrows));
+        TRY(dev_alloc(&b.beam_key, 2));
         HIPCHK(hipMemsetAsync(b.temp, 0, (size_t) c.P * sizeof(double), ctx->stream));
+        HIPCHK(hipMemsetAsync(b.beam_w, 0, (size_t) BEAM_NT * wrows * sizeof(double), ctx->stream));
+        const double key[2] = { NAN, 1.0 };      // no position yet: W is built on the first step
+        HIPCHK(hipMemcpyAsync(b.beam_key, key, sizeof(key), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
         ctx->view_dirty = true;
     }
     return MMCA_OK;
@@ -630,7 +637,7 @@ int mmca_destroy(mmca_ctx *ctx)
                      b.diff_id, b.nbr, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
                      b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.g_dom, b.fixed_rot,
                      b.rand_rot,
-                     b.oce, b.d, b.temp, b.beam_x, b.beam_y, b.beam_z, b.beam_w, b.red_part, b.cnt, ctx->d_red,
+                     b.oce, b.d, b.temp, b.beam_x, b.beam_y, b.beam_z, b.beam_w, b.beam_key, b.red_part, b.cnt, ctx->d_red,
                      ctx->d_desc, ctx->d_off, ctx->io_buf };
     for (void *p : ptrs)
         if (p)
@@ -1594,8 +1601,8 @@ int mmca_set_tuning(mmca_ctx *ctx, int key, int value)
         return MMCA_ERR_ARG;
     switch (key) {
     case MMCA_TUNE_ZM_PLANES:
-        if (value < 0 || value > ctx->c.nz * 2 + 1024)
-            return MMCA_ERR_ARG;
+        if (value < 0 || value > 1024)
+            return MMCA_ERR_ARG;   // the LDS-staged kernel keeps one temperature per plane of a tile
         ctx->c.zm_planes = value;
         ctx->view_dirty = true;
         return MMCA_OK;
@@ -2027,7 +2034,7 @@ int mmca_batch_set_tuning(mmca_batch *bt, int key, int value)
         return MMCA_ERR_ARG;
     switch (key) {
     case MMCA_TUNE_ZM_PLANES:
-        if (value < 0 || value > 1 << 20)
+        if (value < 0 || value > 1024)
             return MMCA_ERR_ARG;
         bt->zm_planes = value;
         break;

@@ -105,10 +105,10 @@ struct DevBufs {
     double *rand_rot;                // 9 per random orientation draw
     int rand_cap;                    // random orientations uploaded
     // separable beam temperature (Consts::beam_sep): T per padded cell, the
-    // factor tables X[i][x], Y[i][y], Zf[i][z] (BEAM_NT rows each) and
-    // W[(z-1) ny + y-1][i] = Y[i][y] Zf[i][z]
+    // factor tables X[i][x], Y[y][i], Zf[z][i], W[(z-1) ny + y-1][i] =
+    // Y[y][i] Zf[z][i], and the track position W was built for (2 doubles)
     double *temp;
-    double *beam_x, *beam_y, *beam_z, *beam_w;
+    double *beam_x, *beam_y, *beam_z, *beam_w, *beam_key;
     // single-operator buffers (allocated on first use)
     double *oce;
     double *d;

@@ -1016,6 +1016,7 @@ k_solute_zm(View V, int cur, int nxt)
 // memory operation issued in between makes that wait stricter, never looser.
 // Every load address is clamped into the padded box of its field.
 constexpr int ZL_RING = 4;
+constexpr int ZL_TMAX = 1024;                    // planes per workgroup (MMCA_TUNE_ZM_PLANES at most)
 
 template <int BY, bool F32>
 struct ZlSlot {
@@ -1029,7 +1030,25 @@ struct ZlSlot {
     static constexpr int CL0 = 0, FS0 = CL_DW, CE0 = FS0 + FS_DW, GR0 = CE0 + CE_DW;
     static constexpr int DW = GR0 + GR_DW;         // dwords per slot
     static constexpr int NI = DW / 64;             // load instructions per slot
-    static constexpr int NPW = (NI + BY - 1) / BY; // per wave
+    static constexpr int NPW = (NI + BY - 1) / BY; // per wave: loads NPW w .. NPW w + NPW - 1 of the slot
+    // field region of load i (every region is whole loads: cl, fs, ce, gr)
+    static constexpr int region_of(int i)
+    {
+        return i * 64 < FS0 ? 0 : i * 64 < CE0 ? 1 : i * 64 < GR0 ? 2 : 3;
+    }
+    // a wave's loads span at most two regions, so it keeps two base pointers
+    static constexpr bool two_regions()
+    {
+        for (int w = 0; w < BY; ++w)
+        {
+            const int a = w * NPW < NI - 1 ? w * NPW : NI - 1;
+            const int z = w * NPW + NPW - 1 < NI - 1 ? w * NPW + NPW - 1 : NI - 1;
+            if (region_of(z) - region_of(a) > 1)
+                return false;
+        }
+        return true;
+    }
+    static_assert(two_regions(), "a wave's share of a slot must lie in at most two field regions");
 };
 
 // One dword per lane from global (sbase + voff) into LDS (lds + lane * 4).
@@ -1065,6 +1084,10 @@ k_solute_zl(View V, int cur, int nxt)
     __shared__ int s_stop;
     __shared__ double s_beam[AM ? sizeof(BeamTab) / sizeof(double) : 1];
     __shared__ double s_z[AM ? 2 : 1][AM ? BEAM_NT : 1];      // plane factors of the beam, double-buffered
+    // the other temperature models depend on z only (on y in 2-D runs): one
+    // value per plane of the tile, computed before the walk, so that their
+    // constants need no registers during it
+    __shared__ double s_T[AM ? 1 : ZL_TMAX];
     BeamTab *beam = reinterpret_cast<BeamTab *>(s_beam);
     long long gtile = blockIdx.x;
     if (V.xcd_remap)
@@ -1101,35 +1124,40 @@ k_solute_zl(View V, int cur, int nxt)
     const bool rowok = (j <= c.ny);
     const bool active = rowok && (x <= c.nx);
 
-    // this wave's share of a slot: instruction ins[m] of the slot, from field
-    // region reg[m] at per-lane byte offset off[m] within a plane
+    // this wave's share of a slot: loads i = NPW wv + m (the last load of the
+    // slot repeated past its end), which lie in at most two field regions rA
+    // and rB (rB from load mB on); each load reads at per-lane byte offset
+    // off[m] within a plane of its field.  Few scalar registers hold this
+    // state (two bases, two strides), so the kernel's constants stay in
+    // scalar registers too.
     const char *fbase[4] = { (const char *) (F32 ? (const void *) cl32_buf(b, cur) : (const void *) cl_buf(b, cur)),
                              (const char *) fs_buf(b, cur),
                              (const char *) (F32 ? (const void *) b.ce32 : (const void *) b.ce),
                              (const char *) b.gr };
     const long long fplane[4] = { c.sxy * L::E, c.sxy * 8, c.sxy * L::E, c.sxy * 4 };
-    const char *fb[L::NPW];                                // field base of each load
-    long long fstride[L::NPW];                             // and its bytes per plane
-    unsigned ldso[L::NPW];                                 // LDS byte offset within a slot
+    const int ibase = wv * L::NPW;
+    const int rA = L::region_of(min(ibase, L::NI - 1)), rB = L::region_of(min(ibase + L::NPW - 1, L::NI - 1));
+    const int rstart = rB == 0 ? 0 : rB == 1 ? L::FS0 / 64 : rB == 2 ? L::CE0 / 64 : L::GR0 / 64;
+    const int mB = rA == rB ? 0 : rstart - ibase;              // loads m >= mB are in rB
+    const char *baseA = (rA == 0) ? fbase[0] : (rA == 1) ? fbase[1] : (rA == 2) ? fbase[2] : fbase[3];
+    const char *baseB = (rB == 0) ? fbase[0] : (rB == 1) ? fbase[1] : (rB == 2) ? fbase[2] : fbase[3];
+    const long long strA = (rA == 0) ? fplane[0] : (rA == 1) ? fplane[1] : (rA == 2) ? fplane[2] : fplane[3];
+    const long long strB = (rB == 0) ? fplane[0] : (rB == 1) ? fplane[1] : (rB == 2) ? fplane[2] : fplane[3];
     unsigned off[L::NPW];
 #pragma unroll
     for (int m = 0; m < L::NPW; ++m)
     {
-        const int i = min(wv + m * BY, L::NI - 1);        // waves past the end repeat the last load
-        ldso[m] = (unsigned) i * 256u;
+        const int i = min(ibase + m, L::NI - 1);
         const int d = i * 64 + lane;
-        int r, dw, esz, rows_halo;
+        int dw, esz, rows_halo;
         if (d < L::FS0)
-            r = 0, dw = d - L::CL0, esz = L::E, rows_halo = 1;
+            dw = d - L::CL0, esz = L::E, rows_halo = 1;
         else if (d < L::CE0)
-            r = 1, dw = d - L::FS0, esz = 8, rows_halo = 1;
+            dw = d - L::FS0, esz = 8, rows_halo = 1;
         else if (d < L::GR0)
-            r = 2, dw = d - L::CE0, esz = L::E, rows_halo = 0;
+            dw = d - L::CE0, esz = L::E, rows_halo = 0;
         else
-            r = 3, dw = d - L::GR0, esz = 4, rows_halo = 0;
-        const int ru = __builtin_amdgcn_readfirstlane(r);   // uniform: a load lies in one region
-        fb[m] = (ru == 0) ? fbase[0] : (ru == 1) ? fbase[1] : (ru == 2) ? fbase[2] : fbase[3];
-        fstride[m] = (ru == 0) ? fplane[0] : (ru == 1) ? fplane[1] : (ru == 2) ? fplane[2] : fplane[3];
+            dw = d - L::GR0, esz = 4, rows_halo = 0;
         const int per = esz / 4;                          // dwords per value
         const int width = rows_halo ? L::HW : 64;
         const int nval = rows_halo ? L::HR * L::HW : BY * 64;
@@ -1145,9 +1173,10 @@ k_solute_zl(View V, int cur, int nxt)
     auto issue = [&](int s) {
         const long long p = min(k0 - 1 + s, k1 + 1);
         const unsigned slot = ring_lds + (unsigned) ((s % ZL_RING) * L::DW) * 4u;
+        const char *pa = baseA + p * strA, *pb = baseB + p * strB;
 #pragma unroll
         for (int m = 0; m < L::NPW; ++m)
-            zl_load(fb[m] + p * fstride[m], off[m], slot + ldso[m]);
+            zl_load(m >= mB ? pb : pa, off[m], slot + (unsigned) min(ibase + m, L::NI - 1) * 256u);
     };
     auto cl_at = [&](int q, int row, int col) -> double {
         const unsigned *sl = s_ring + q * L::DW + L::CL0;
@@ -1173,6 +1202,15 @@ k_solute_zl(View V, int cur, int nxt)
     {
         beam_tab_terms(c, *beam, ty * 64 + lane, 64 * BY);
         beam_tab_plane(c, s_z[1], k0, ty * 64 + lane, 64 * BY);
+    }
+    double t_row = 0.0;                          // 2-D runs: the row's temperature (all planes)
+    if constexpr (!AM)
+    {
+        if (c.gdimz == 1)
+            t_row = temp_model<false>(c, sim_time, x, j, k0);
+        else
+            for (int e = ty * 64 + lane; e < min(n, ZL_TMAX); e += 64 * BY)
+                s_T[e] = temp_model<false>(c, sim_time, x0, j0, k0 + e);   // read after the loop's first barrier
     }
     zl_wait<2 * L::NPW>();                       // planes k0 - 1 and k0 have landed
     __syncthreads();
@@ -1216,8 +1254,10 @@ k_solute_zl(View V, int cur, int nxt)
                 [&]() {
                     if constexpr (AM)
                         return beam_temp_tab(c, *beam, s_z[s & 1], sim_time, x, j, k);
+                    else if (c.beam_sep)
+                        return G(b.temp)[idx];
                     else
-                        return temp_step<false>(b, c, sim_time, idx, x, j, k);
+                        return c.gdimz == 1 ? t_row : s_T[min(s - 1, ZL_TMAX - 1)];
                 },
                 ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out, &g_new, &branch);
             st_cl<F32>(b, c, nxt, idx, cl_out);
@@ -1516,7 +1556,8 @@ k_capture(View V, int cb)
 {
     const int dm = find_dom(V.off_list, V.ndom, blockIdx.x);
     const long long blk = blockIdx.x - V.off_list[dm];
-    const long long nthr = (long long) (V.off_list[dm + 1] - V.off_list[dm]) * LIST_BS;
+    const long long nwg = V.off_list[dm + 1] - V.off_list[dm];
+    const long long nthr = nwg * LIST_BS;
     const DevBufs b = V.dom[dm].b;    // private copy: stores cannot alias it
     const Consts c = V.dom[dm].c;
     const MMCA_GLOBAL Counters *cnt = G(b.cnt);
@@ -1642,7 +1683,6 @@ k_capture(View V, int cb)
         G(b.dc)[3 * (long long) idx + 1] = ty;
         G(b.dc)[3 * (long long) idx + 2] = tz;
     };
-    const long long nwg = V.off_list[dm + 1] - V.off_list[dm];
     const int chunk = c.capture_chunk;
     if (chunk > 0 && nwg >= 8)
     {
@@ -1786,12 +1826,22 @@ beam_now(const Consts &c, double sim_time)
     return n;
 }
 
-// row pitch of the X table: whole BF_COLS-column tiles, zero past nx
-constexpr int BF_COLS = 128;
+// Table shapes: X rows (one per node) of beam_x_pitch columns, zero past
+// nx; W rows (one per (y, z) row of the block, q = (z-1) ny + y-1) of
+// BEAM_NT nodes, beam_w_rows of them, zero past ny nz.
+constexpr int BF_COLS = 128;                      // columns per workgroup of k_beam_field
+constexpr int BF_ROWS = 64;                       // W rows per workgroup
 __host__ __device__ __forceinline__ int beam_x_pitch(const Consts &c) { return (c.nx + BF_COLS - 1) / BF_COLS * BF_COLS; }
+__host__ __device__ __forceinline__ long long
+beam_w_rows(const Consts &c)
+{
+    return ((long long) c.ny * c.nz + BF_ROWS - 1) / BF_ROWS * BF_ROWS;
+}
 
-// the factor tables X[i][x-1] (row pitch beam_x_pitch), Y[i][y-1], Zf[i][z-1]
-// at the time of timestep cnt->timestep + adv
+// The factor tables at the time of timestep cnt->timestep + adv:
+// X[i][x-1], Y[y-1][i], Zf[z-1][i].  W depends on the time only through
+// the track's y position; beam_key[0] holds the position W was built for,
+// and beam_key[1] tells k_beam_w whether to build it again.
 __global__ void __launch_bounds__(256)
 k_beam_tabs(DevBufs b, Consts c, int adv)
 {
@@ -1800,20 +1850,26 @@ k_beam_tabs(DevBufs b, Consts c, int adv)
         return;
     const double sim_time = (cnt->timestep + (unsigned long long) adv) * c.dt;
     const BeamNow bn = beam_now(c, sim_time);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        const double held = G(b.beam_key)[0];
+        const bool same = __double_as_longlong(held) == __double_as_longlong(bn.ynow);
+        G(b.beam_key)[1] = same ? 0.0 : 1.0;
+        G(b.beam_key)[0] = bn.ynow;
+    }
     const double ls = c.lsd / (2. * 1.414);
     const double lalpha = beam_lalpha(c);
     const double ldt = beam_ldt(c);
     const int xp = beam_x_pitch(c);
-    const long long L = (long long) xp + c.ny + c.nz;
-    for (long long e = (long long) blockIdx.x * blockDim.x + threadIdx.x; e < L * BEAM_NT;
-         e += (long long) gridDim.x * blockDim.x)
+    const long long nX = (long long) BEAM_NT * xp, nY = (long long) BEAM_NT * c.ny;
+    const long long n = nX + nY + (long long) BEAM_NT * c.nz;
+    for (long long e = (long long) blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long) gridDim.x * blockDim.x)
     {
-        const int i = (int) (e / L);
-        const int q = (int) (e - (long long) i * L);
-        const double lt = (0.5 + i) * ldt;
-        const double A = ls * ls + 4. * lalpha * lt;
-        if (q < xp)
+        if (e < nX)
         {
+            const int i = (int) (e / xp), q = (int) (e - (long long) i * xp);
+            const double lt = (0.5 + i) * ldt;
+            const double A = ls * ls + 4. * lalpha * lt;
             double v = 0.0;                       // columns past nx pad the row
             if (q < c.nx)
             {
@@ -1821,142 +1877,150 @@ k_beam_tabs(DevBufs b, Consts c, int adv)
                 const double u = (rx - bn.xnow) + c.lv * lt;
                 v = exp_glibc(-(u * u) / A, exp_tab);
             }
-            G(b.beam_x)[(long long) i * xp + q] = v;
+            G(b.beam_x)[e] = v;
         }
-        else if (q < xp + c.ny)
+        else if (e < nX + nY)
         {
-            const int y = q - xp;
+            const long long f = e - nX;
+            const int y = (int) (f / BEAM_NT), i = (int) (f - (long long) y * BEAM_NT);
+            const double lt = (0.5 + i) * ldt;
+            const double A = ls * ls + 4. * lalpha * lt;
             const double ry = c.origin[1] + ((c.off[1] + y + 1) + 0.5) * c.cell_size;
             const double ly = ry - bn.ynow;
             const double Tl = sqrt(lalpha * lt) * (ls * ls + 4. * lalpha * lt);
-            G(b.beam_y)[(long long) i * c.ny + y] = ldt / Tl * exp_glibc(-(ly * ly) / A, exp_tab);
+            G(b.beam_y)[f] = ldt / Tl * exp_glibc(-(ly * ly) / A, exp_tab);
         }
         else
         {
-            const int z = q - xp - c.ny;
+            const long long f = e - nX - nY;
+            const int z = (int) (f / BEAM_NT), i = (int) (f - (long long) z * BEAM_NT);
+            const double lt = (0.5 + i) * ldt;
             const double rz = c.origin[2] + ((c.off[2] + z + 1) + 0.5) * c.cell_size;
             const double lz = rz - bn.znow;
-            G(b.beam_z)[(long long) i * c.nz + z] = exp_glibc(-(lz * lz) / (4. * lalpha * lt), exp_tab);
+            G(b.beam_z)[f] = exp_glibc(-(lz * lz) / (4. * lalpha * lt), exp_tab);
         }
     }
 }
 
-// W[(z-1) ny + y-1][i] = Y[i][y-1] * Zf[i][z-1]
+// W[q][i] = Y[y-1][i] * Zf[z-1][i] for q = (z-1) ny + y-1, when the track's
+// y position changed (beam_key[1]); one row per wave at a time
 __global__ void __launch_bounds__(256)
 k_beam_w(DevBufs b, Consts c)
 {
-    if (G(b.cnt)->status != MMCA_OK)
+    if (G(b.cnt)->status != MMCA_OK || G(b.beam_key)[1] == 0.0)
         return;
-    const long long n = (long long) c.nz * c.ny * BEAM_NT;
-    for (long long e = (long long) blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long) gridDim.x * blockDim.x)
+    const long long rows = (long long) c.nz * c.ny;
+    const long long waves = (long long) gridDim.x * (blockDim.x / 64);
+    for (long long q = (long long) blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; q < rows; q += waves)
     {
-        const long long row = e / BEAM_NT;
-        const int i = (int) (e - row * BEAM_NT);
-        const int z = (int) (row / c.ny);
-        const int y = (int) (row - (long long) z * c.ny);
-        G(b.beam_w)[e] = G(b.beam_y)[(long long) i * c.ny + y] * G(b.beam_z)[(long long) i * c.nz + z];
+        const int z = (int) (q / c.ny), y = (int) (q - (long long) z * c.ny);
+        const MMCA_GLOBAL double *Y = G(b.beam_y) + (long long) y * BEAM_NT;
+        const MMCA_GLOBAL double *Z = G(b.beam_z) + (long long) z * BEAM_NT;
+        for (int i = threadIdx.x % 64; i < BEAM_NT; i += 64)
+            G(b.beam_w)[q * BEAM_NT + i] = Y[i] * Z[i];
     }
 }
 
-// T over the interior into out (padded indexing).  Workgroup: one plane,
-// BF_COLS columns (two per lane) and BF_WAVES x BF_ROWS rows; the X rows of
-// BF_IC quadrature nodes at a time are staged in LDS, the W values of a
-// wave's rows are the same for all its lanes (scalar loads).
-typedef double dvec2 __attribute__((ext_vector_type(2)));
-constexpr int BF_ROWS = 8;
-constexpr int BF_WAVES = 4;
-constexpr int BF_IC = 40;
-static_assert(BEAM_NT % BF_IC == 0, "node chunks must tile the quadrature");
+// T over the interior into out (padded indexing): the product of W (rows q)
+// and X on the matrix cores.  v_mfma_f64_16x16x4f64 multiplies a 16 x 4
+// tile of W (lane l: row l % 16, node slot l / 16) by a 4 x 16 tile of X
+// (lane l: node slot l / 16, column l % 16) into a 16 x 16 tile held as four
+// doubles per lane (lane l, register r: row l / 16 + 4 r, column l % 16).
+// The sum over the nodes may take them in any order, so the node slot k of
+// step s in a block of 16 nodes 16 t is node 16 t + 4 k + s % 4: each lane
+// then loads four consecutive W values of its row at once.  A wave owns 64
+// rows x 32 columns (4 x 2 tiles, 32 accumulators per lane); the four waves
+// of a workgroup share its 64 rows and take 32 columns each.
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+static_assert(BEAM_NT % 16 == 0, "nodes come in blocks of 16");
 
-__global__ void __launch_bounds__(64 * BF_WAVES)
+__global__ void __launch_bounds__(256)
 k_beam_field(DevBufs b, Consts c, int adv, double *out)
 {
-    __shared__ double sX[BF_IC][BF_COLS];
     const MMCA_GLOBAL Counters *cnt = G(b.cnt);
     if (cnt->status != MMCA_OK)
         return;                                   // uniform over the workgroup
-    const int k = blockIdx.z + 1;
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int xb = blockIdx.x * BF_COLS;          // first column (0-based interior)
-    const int x = xb + 2 * lane + 1;              // this lane's columns x, x + 1
-    const int jw = blockIdx.y * (BF_WAVES * BF_ROWS) + wave * BF_ROWS + 1;
+    const int wave = threadIdx.x >> 6;
+    const long long q0 = (long long) blockIdx.y * BF_ROWS;
+    const int x0 = blockIdx.x * BF_COLS + wave * 32;      // 0-based interior column
+    const int kk = lane >> 4, lr = lane & 15;
     const double sim_time = (cnt->timestep + (unsigned long long) adv) * c.dt;
     const BeamNow bn = beam_now(c, sim_time);
-    const double rz = c.origin[2] + ((c.off[2] + k) + 0.5) * c.cell_size;
-    auto store = [&](int r, double t0, double t1) {
-        const int j = jw + r;
-        if (j > c.ny)
+    const long long nq = (long long) c.ny * c.nz;
+    // stores tile (a, bb) register r: row q0 + 16 a + kk + 4 r, column x0 + 16 bb + lr
+    auto store = [&](int a, int bb, int r, double sum, bool sep) {
+        const long long q = q0 + 16 * a + kk + 4 * r;
+        const int x = x0 + 16 * bb + lr;
+        if (q >= nq || x >= c.nx)
             return;
-        const long long idx = (long long) k * c.sxy + (long long) j * c.sx + x;
-        if (x <= c.nx)
-            out[idx] = t0;
-        if (x + 1 <= c.nx)
-            out[idx + 1] = t1;
+        const int z = (int) (q / c.ny) + 1, y = (int) (q - (long long) (z - 1) * c.ny) + 1;
+        const double rz = c.origin[2] + ((c.off[2] + z) + 0.5) * c.cell_size;
+        double t;
+        if (!(rz <= bn.znow + c.cell_size))
+            t = 0.0;
+        else if (!sep)
+            t = c.liniT + bn.Tfront * 0.0;        // every term is finite: liniT + 0
+        else
+            t = c.liniT + bn.Tfront * sum;
+        out[(long long) z * c.sxy + (long long) y * c.sx + x + 1] = t;
     };
-    if (!(rz <= bn.znow + c.cell_size))
-    {
-        for (int r = 0; r < BF_ROWS; ++r)
-            store(r, 0.0, 0.0);
-        return;
-    }
     if (bn.Tfront == 0.0)
     {
-        // every term is finite, so liniT + Tfront * sum is liniT + 0
-        const double t = c.liniT + bn.Tfront * 0.0;
-        for (int r = 0; r < BF_ROWS; ++r)
-            store(r, t, t);
+        for (int a = 0; a < 4; ++a)
+            for (int bb = 0; bb < 2; ++bb)
+                for (int r = 0; r < 4; ++r)
+                    store(a, bb, r, 0.0, false);
         return;
     }
-    const MMCA_GLOBAL double *W[BF_ROWS];
-#pragma unroll
-    for (int r = 0; r < BF_ROWS; ++r)
-        W[r] = G(b.beam_w) + ((long long) (k - 1) * c.ny + min(jw + r, c.ny) - 1) * BEAM_NT;
-    double acc0[BF_ROWS], acc1[BF_ROWS];
-#pragma unroll
-    for (int r = 0; r < BF_ROWS; ++r)
-        acc0[r] = acc1[r] = 0.0;
-    // staging: every thread moves STG column pairs of the chunk, all loads
-    // issued before the first LDS write
-    constexpr int STG = BF_IC * BF_COLS / 2 / (64 * BF_WAVES);
-    static_assert(STG * 2 * 64 * BF_WAVES == BF_IC * BF_COLS, "staging must cover the chunk");
     const int xp = beam_x_pitch(c);
-    const MMCA_GLOBAL double *Xt = G(b.beam_x) + xb;
-    for (int i0 = 0; i0 < BEAM_NT; i0 += BF_IC)
+    // this lane's W rows (one per row tile) and X columns (one per column tile)
+    const MMCA_GLOBAL double *Wr[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        Wr[a] = G(b.beam_w) + (q0 + 16 * a + lr) * BEAM_NT + 4 * kk;
+    const MMCA_GLOBAL double *Xc = G(b.beam_x) + x0 + lr + (long long) (4 * kk) * xp;
+    dvec4 acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        for (int bb = 0; bb < 2; ++bb)
+            acc[a][bb] = dvec4{0.0, 0.0, 0.0, 0.0};
+    // one block of 16 nodes: loads, then 4 steps of 8 MFMAs
+    dvec4 wa[4];
+    double xb[2][4];
+    auto load = [&](int t) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            wa[a] = *reinterpret_cast<const MMCA_GLOBAL dvec4 *>(Wr[a] + 16 * t);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb)
+                xb[bb][m] = Xc[(long long) (16 * t + m) * xp + 16 * bb];
+    };
+    load(0);
+    for (int t = 0; t < BEAM_NT / 16; ++t)
     {
-        dvec2 v[STG];
+        const dvec4 ca[4] = { wa[0], wa[1], wa[2], wa[3] };
+        const double cb[2][4] = { { xb[0][0], xb[0][1], xb[0][2], xb[0][3] },
+                                  { xb[1][0], xb[1][1], xb[1][2], xb[1][3] } };
+        if (t + 1 < BEAM_NT / 16)
+            load(t + 1);                          // the next block's loads overlap this block's MFMAs
 #pragma unroll
-        for (int q = 0; q < STG; ++q)
-        {
-            const int e = threadIdx.x + q * 64 * BF_WAVES;
-            const int ii = e / (BF_COLS / 2), x2 = 2 * (e - ii * (BF_COLS / 2));
-            v[q] = *reinterpret_cast<const MMCA_GLOBAL dvec2 *>(Xt + (long long) (i0 + ii) * xp + x2);
-        }
-        __syncthreads();                          // the previous chunk is consumed
+        for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int q = 0; q < STG; ++q)
-        {
-            const int e = threadIdx.x + q * 64 * BF_WAVES;
-            const int ii = e / (BF_COLS / 2), x2 = 2 * (e - ii * (BF_COLS / 2));
-            *reinterpret_cast<dvec2 *>(&sX[ii][x2]) = v[q];
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int ii = 0; ii < BF_IC; ++ii)
-        {
-            const dvec2 xv = *reinterpret_cast<const dvec2 *>(&sX[ii][2 * lane]);
+            for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int r = 0; r < BF_ROWS; ++r)
-            {
-                const double w = W[r][i0 + ii];
-                acc0[r] = fma(xv.x, w, acc0[r]);
-                acc1[r] = fma(xv.y, w, acc1[r]);
-            }
-        }
+                for (int bb = 0; bb < 2; ++bb)
+                    acc[a][bb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[a][m], cb[bb][m], acc[a][bb], 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < BF_ROWS; ++r)
-        store(r, c.liniT + bn.Tfront * acc0[r], c.liniT + bn.Tfront * acc1[r]);
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                store(a, bb, r, acc[a][bb][r], true);
 }
 
 // The separable temperature of timestep cnt->timestep + adv into b.temp
@@ -1965,11 +2029,10 @@ hipError_t launch_beam_field(const DevBufs &b, const Consts &c, int adv, hipStre
     const long long tabs = (long long) BEAM_NT * (beam_x_pitch(c) + c.ny + c.nz);
     hipLaunchKernelGGL(k_beam_tabs, dim3((unsigned) std::min<long long>((tabs + 255) / 256, 4096)), dim3(256), 0, s,
                        b, c, adv);
-    const long long nw = (long long) c.nz * c.ny * BEAM_NT;
-    hipLaunchKernelGGL(k_beam_w, dim3((unsigned) std::min<long long>((nw + 255) / 256, 65536)), dim3(256), 0, s, b, c);
-    const dim3 grid((unsigned) ((c.nx + BF_COLS - 1) / BF_COLS),
-                    (unsigned) ((c.ny + BF_WAVES * BF_ROWS - 1) / (BF_WAVES * BF_ROWS)), (unsigned) c.nz);
-    hipLaunchKernelGGL(k_beam_field, grid, dim3(64 * BF_WAVES), 0, s, b, c, adv, b.temp);
+    const long long rows = (long long) c.ny * c.nz;
+    hipLaunchKernelGGL(k_beam_w, dim3((unsigned) std::min<long long>((rows + 3) / 4, 2048)), dim3(256), 0, s, b, c);
+    const dim3 grid((unsigned) (beam_x_pitch(c) / BF_COLS), (unsigned) (beam_w_rows(c) / BF_ROWS));
+    hipLaunchKernelGGL(k_beam_field, grid, dim3(256), 0, s, b, c, adv, b.temp);
     return hipGetLastError();
 }
 

@@ -118,6 +118,32 @@ def test_am_track_fp64_bit_exact_and_fp32_bounds():
     assert int(np.sum(g[1:-1, 1:-1, 1:-1] <= 0)) < melted // 100     # which regrew
 
 
+def test_am_second_track_rebuilds_the_separable_factors():
+    """Two tracks (NoOfTracks 2, hatch 10 um): the beam moves to the second
+    track after one track time (temperature.c:88-101; about 69 steps here),
+    which changes the y factors of the separable field; the FP32-deviation
+    block's field stays within the bound of the FP64 quadrature on both
+    tracks, and the two runs keep the same grains."""
+    cfg = small_track()
+    cfg.am_tracks = 2
+    cfg.laser["hatch"] = 10e-6
+    cfg.laser["start"][1] = 7e-6
+    gr, recs = voronoi_substrate(cfg, 60, 2)
+    with Subblock(cfg.params(check_negative_ce=False), cfg.nucleation()) as a, \
+            Subblock(cfg.params(check_negative_ce=False, precision=abi.PREC_FP32DEV), cfg.nucleation()) as b:
+        apply_substrate(a, gr, recs)
+        apply_substrate(b, gr, recs)
+        done = 0
+        for t in (40, 60, 75, 100):
+            ra = a.step(t - done)
+            rb = b.step(t - done)
+            done = t
+            tr = _temp_rel(b.download(abi.FIELD_TEMP)[1:-1, 1:-1, 1:-1], a.download(abi.FIELD_TEMP)[1:-1, 1:-1, 1:-1])
+            print("two tracks, step", t, "temperature", tr, "grains", ra.num_grains, rb.num_grains, flush=True)
+            assert tr <= TEMP_REL
+            assert ra.num_grains == rb.num_grains
+
+
 @pytest.mark.timeout(600)
 def test_am_track_full_size_fp32_against_fp64():
     cfg = am_track_config()

@@ -240,10 +240,12 @@ rebuild_halos(const int nsub[3], const int pad[3], const int n[3], size_t esize,
                             const size_t i = (((size_t) z * (n[1] + 2) + y) * (n[0] + 2) + x) * esize;
                             memcpy(cur[r] + i, model[r] + i, esize);
                         }
-    for (int r = 0; r < nr && model && next; r++)
+    for (int r = 0; r < nr; r++)
     {
-        free(model[r]);
-        free(next[r]);
+        if (model)
+            free(model[r]);
+        if (next)
+            free(next[r]);
     }
     free(model);
     free(next);

@@ -311,6 +311,26 @@ hist_alloc(ckpt_hist *h, const mmca_params *ps, int nr)
     return 0;
 }
 
+static void
+hist_free(ckpt_hist *h, int nr)
+{
+    for (int k = 0; k < MMCA_CKPT_HISTORY; k++)
+    {
+        for (int r = 0; h->slot[k] && r < nr; r++)
+        {
+            mmca_ckpt_snap *c = &h->slot[k][r];
+            free(c->ce);
+            free(c->cl);
+            free(c->fs);
+            free(c->dc);
+            free(c->d);
+            free(c->gr);
+        }
+        free(h->slot[k]);
+        h->slot[k] = NULL;
+    }
+}
+
 /* keeps the fields after step t when a checkpoint follows within the depth */
 static int
 hist_take(ckpt_hist *h, mmca_ctx *const *doms, int nr, long long freq, uint64_t t)
@@ -747,6 +767,7 @@ main(int argc, char **argv)
         mmca_batch_destroy(bt);
     else
         mmca_destroy(doms[0]);
+    hist_free(&hist, nr);
     free(ps);
     free(ns);
     free(doms);

@@ -72,6 +72,7 @@ def apply_shape(sb, shape):
     sb.set_tuning(abi.TUNE_XCD_REMAP, shape["xcd"])
     sb.set_tuning(abi.TUNE_ZM_LDS, shape.get("lds", 0))
     sb.set_tuning(abi.TUNE_CAPTURE_CHUNK, shape.get("chunk", 0))
+    sb.set_tuning(abi.TUNE_CAPTURE_BAND, shape.get("band", 0))
 
 
 def measured_traffic(size, shape, prec="fp64"):
@@ -558,6 +559,8 @@ def main():
                     help="1: z-marching kernel with its planes staged in LDS by direct loads (k_solute_zl)")
     ap.add_argument("--capture-chunk", type=int, default=0,
                     help=">0: capture deals its list slots in chunks of this many, pinned to XCDs")
+    ap.add_argument("--capture-band", type=int, default=0,
+                    help=">0: capture works on units of this many rows of one plane, bands pinned to XCDs")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the FP32-deviation, literal-dt, coupled, ensemble and adjoint measurements")
     ap.add_argument("--extra-steps", type=int, default=1000)
@@ -578,7 +581,8 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
     dims = tuple(int(v) for v in args.size.split("x"))
-    shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd, "lds": args.lds, "chunk": args.capture_chunk}
+    shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd, "lds": args.lds, "chunk": args.capture_chunk,
+             "band": args.capture_band}
     if args.capture_footprint:
         capture_footprint(shape)
         return
@@ -676,6 +680,7 @@ def main():
                                                                                   else "plain FP64"),
                    "cells": n, "dt": cfg.dt, "zm_planes": args.zm, "zm_rows": args.zm_rows,
                    "xcd_remap": args.xcd, "zm_lds": args.lds, "capture_chunk": args.capture_chunk,
+                   "capture_band": args.capture_band,
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -333,6 +333,10 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
                                     step for the whole block (within 1e-13 of the quadrature,
                                     not its bits); 0: the quadrature per cell.  FP64 storage
                                     always uses the quadrature (value 1 is refused there) */
+#define MMCA_TUNE_CAPTURE_BAND 8 /* >0: capture works on units of this many rows of one plane, band q
+                                     of every plane on XCD q % 8, units taken plane by plane; used
+                                     when the block has at least 8 bands and the step builds the
+                                     list itself (mmca_step, batches), before CAPTURE_CHUNK */
 int  mmca_set_tuning(mmca_ctx *ctx, int key, int value);
 int  mmca_get_tuning(mmca_ctx *ctx, int key, int *value);
 

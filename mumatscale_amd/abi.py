@@ -55,6 +55,7 @@ TUNE_ZM_LDS = 4
 TUNE_CAPTURE_CHUNK = 5
 TUNE_TRACK_D = 6
 TUNE_BEAM_SEP = 7
+TUNE_CAPTURE_BAND = 8
 
 # partitioning branch codes recorded for the adjoint (fs_change_diffuse, growth.c:221-273)
 BRANCH_LIQUID = 0

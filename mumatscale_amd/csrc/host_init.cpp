@@ -143,6 +143,7 @@ int build_consts(const mmca_params &p, Consts &c)
     c.xcd_remap = 1;
     c.zm_lds = 1;                       // planes staged in LDS (k_solute_zl)
     c.capture_chunk = 0;
+    c.capture_band = 0;
     c.track_d = 0;
     if (p.precision != MMCA_PREC_FP64 && p.precision != MMCA_PREC_FP32DEV)
         return MMCA_ERR_ARG;

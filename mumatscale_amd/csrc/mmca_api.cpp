@@ -573,6 +573,7 @@ int mmca_create(const mmca_params *params, int device, mmca_ctx **out)
     TRY(dev_alloc(&b.dtz, N + 102));
     b.nblk_list = list_blocks(c);
     TRY(dev_alloc(&b.blk_count, (size_t) b.nblk_list));
+    TRY(dev_alloc(&b.row_first, (size_t) c.ny * c.nz));
     b.nblk_red = reduce_blocks(c);
     TRY(dev_alloc(&b.red_part, (size_t) 2 * b.nblk_red));
     TRY(dev_alloc(&ctx->d_red, 2));
@@ -634,7 +635,7 @@ int mmca_destroy(mmca_ctx *ctx)
         b.g_rot = nullptr, b.g_cell = nullptr, b.g_step = nullptr, b.g_src = nullptr, b.g_dom = nullptr;
     void *ptrs[] = { b.ce, b.cl[0], b.cl[1], b.ce32, b.cl32[0], b.cl32[1], b.ledger, b.fs[0], b.fs[1], b.gr, b.dc, b.mask,
                      b.xhalo,
-                     b.diff_id, b.nbr, b.dtx, b.dty, b.dtz, b.blk_count, b.site_idx, b.site_thr, b.site_fixed,
+                     b.diff_id, b.nbr, b.dtx, b.dty, b.dtz, b.blk_count, b.row_first, b.site_idx, b.site_thr, b.site_fixed,
                      b.site_ls, b.events, b.ev_blk, b.ev_cnt, b.g_rot, b.g_cell, b.g_step, b.g_src, b.g_dom, b.fixed_rot,
                      b.rand_rot,
                      b.oce, b.d, b.temp, b.beam_x, b.beam_y, b.beam_z, b.beam_w, b.beam_key, b.red_part, b.cnt, ctx->d_red,
@@ -1626,6 +1627,12 @@ int mmca_set_tuning(mmca_ctx *ctx, int key, int value)
         ctx->c.capture_chunk = (value + 255) / 256 * 256;
         ctx->view_dirty = true;
         return MMCA_OK;
+    case MMCA_TUNE_CAPTURE_BAND:
+        if (value < 0 || value > (1 << 24))
+            return MMCA_ERR_ARG;
+        ctx->c.capture_band = value;
+        ctx->view_dirty = true;
+        return MMCA_OK;
     case MMCA_TUNE_TRACK_D:
         if (value)
             TRY(ensure_d(ctx));
@@ -1664,6 +1671,9 @@ int mmca_get_tuning(mmca_ctx *ctx, int key, int *value)
         return MMCA_OK;
     case MMCA_TUNE_CAPTURE_CHUNK:
         *value = ctx->c.capture_chunk;
+        return MMCA_OK;
+    case MMCA_TUNE_CAPTURE_BAND:
+        *value = ctx->c.capture_band;
         return MMCA_OK;
     case MMCA_TUNE_TRACK_D:
         *value = ctx->c.track_d;
@@ -1710,7 +1720,7 @@ struct mmca_batch {
     View view;
     bool view_dirty = true;
     int cur = 0;
-    int zm_planes = 64, zm_by = 16, xcd_remap = 1, zm_lds = 1, capture_chunk = 0, track_d = 0;
+    int zm_planes = 64, zm_by = 16, xcd_remap = 1, zm_lds = 1, capture_chunk = 0, capture_band = 0, track_d = 0;
     bool timing = false;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
@@ -1735,6 +1745,7 @@ int batch_view(mmca_batch *bt)
         c->c.xcd_remap = bt->xcd_remap;
         c->c.zm_lds = bt->zm_lds;
         c->c.capture_chunk = bt->capture_chunk;
+        c->c.capture_band = bt->capture_band;
         c->c.track_d = bt->track_d;
         c->view_dirty = true;
     }
@@ -2053,6 +2064,11 @@ int mmca_batch_set_tuning(mmca_batch *bt, int key, int value)
         if (value < 0 || value > (1 << 24))
             return MMCA_ERR_ARG;
         bt->capture_chunk = (value + 255) / 256 * 256;
+        break;
+    case MMCA_TUNE_CAPTURE_BAND:
+        if (value < 0 || value > (1 << 24))
+            return MMCA_ERR_ARG;
+        bt->capture_band = value;
         break;
     case MMCA_TUNE_TRACK_D:
         for (mmca_ctx *d : bt->dom)

@@ -28,6 +28,7 @@ struct Consts {
     int xcd_remap;          // 1: contiguous tile runs per XCD
     int zm_lds;             // 1: z-marching kernel with its planes staged in LDS (k_solute_zl)
     int capture_chunk;      // > 0: capture deals list slots in chunks of this many, pinned to XCDs
+    int capture_band;       // > 0: capture works on units of this many rows of one plane, band q on XCD q % 8
     int track_d;            // 1: keep the reference's d array (stale values of cells no longer solid)
     int f32;                // 1: ce, cl stored as FP32 deviations from cinit
     // diffusion (growth.c:38-40)
@@ -85,6 +86,7 @@ struct DevBufs {
     unsigned char *nbr;              // per slot: solid neighbours before capture, bit n = capture direction n
     double *dtx, *dty, *dtz;         // dc_tmp slots 0..capacity
     int *blk_count;                  // per-block member counts -> offsets
+    int *row_first;                  // per interior row (k-1) ny + (j-1): slot of its first list member
     int nblk_list;
     // nucleation sites, sorted by padded index
     int *site_idx;

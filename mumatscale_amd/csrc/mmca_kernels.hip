@@ -1223,7 +1223,7 @@ k_solute_zl(View V, int cur, int nxt)
         const int k = k0 - 1 + s;
         const int q = s % ZL_RING, q1 = (s + 1) % ZL_RING;
         zl_wait<L::NPW>();                       // this wave's share of plane k + 1 has landed
-        __syncthreads();                         // every wave's share; slot of plane k - 2 is free
+        __syncthreads();                         // every wave's share; the slot of plane k - 1 is free
         issue(s + 3);
         if constexpr (AM)
             if (s < n)
@@ -1520,6 +1520,15 @@ k_list_write(View V)
     s_ex[threadIdx.x] = ex;
     __syncthreads();
     const int base = G(b.blk_count)[blk];            // members before this workgroup
+    if (wi < nwords)
+    {
+        // slot of the first member of each row (of the next member after the
+        // row's start when the row has none), for capture's row bands
+        int k, j, w;
+        word_pos(c, wi, &k, &j, &w);
+        if (w == 0)
+            G(b.row_first)[(long long) (k - 1) * c.ny + (j - 1)] = base + ex + 1;
+    }
     for (int i = threadIdx.x; i < tot; i += LIST_BS)
     {
         // word of member i: the last word whose prefix count is <= i
@@ -1548,11 +1557,11 @@ k_list_write(View V)
 // capture_octahedra_diffuse, INDEX_SEP branch (growth.c:640-807), over the
 // slots 1..gindex of the list.  Every member is independent: it reads only
 // solid neighbours (through the pre-capture mask) and writes only itself and
-// its slot, so the slots are dealt round-robin over all the domain's threads
-// whatever their place in the grid.  `cb` is the buffer holding fs/cl after
-// fs_change.
+// its slot, so the slots may be dealt to the threads in any order.  `cb` is
+// the buffer holding fs/cl after fs_change; `rows` is 1 when the list writer
+// of the same list has filled row_first (the row bands need it).
 __global__ void __launch_bounds__(LIST_BS)
-k_capture(View V, int cb)
+k_capture(View V, int cb, int rows)
 {
     const int dm = find_dom(V.off_list, V.ndom, blockIdx.x);
     const long long blk = blockIdx.x - V.off_list[dm];
@@ -1683,8 +1692,39 @@ k_capture(View V, int cb)
         G(b.dc)[3 * (long long) idx + 1] = ty;
         G(b.dc)[3 * (long long) idx + 2] = tz;
     };
+    const int band = c.capture_band;
+    const int nband = band > 0 ? (c.ny + band - 1) / band : 0;
     const int chunk = c.capture_chunk;
-    if (chunk > 0 && nwg >= 8)
+    if (rows && nband >= 8 && nwg >= 8)
+    {
+        // Units of `band` rows of one plane.  Band q of every plane belongs
+        // to the XCD q % 8 (the hardware places workgroup w on XCD w % 8), and
+        // the workgroups of this domain on one XCD take its units plane by
+        // plane, one unit per workgroup at a time.  A member's solid
+        // neighbours lie in its own row, in the rows next to it (mostly in
+        // the same unit: one workgroup, one L1) and in the planes next to it
+        // (units of the same XCD in flight together: one L2).
+        const long long o = V.off_list[dm];
+        const int xcd = (int) (blockIdx.x % 8);
+        const long long first = o + ((xcd - (int) (o % 8)) + 8) % 8;
+        const long long rank = (blockIdx.x - first) / 8;
+        const long long nxcd = (o + nwg - 1 - first) / 8 + 1;
+        const int mine = (nband - 1 - xcd) / 8 + 1;            // bands of this XCD in a plane
+        const long long nunit = (long long) c.nz * mine;
+        const long long nrow = (long long) c.nz * c.ny;
+        for (long long u = rank; u < nunit; u += nxcd)
+        {
+            const int k = (int) (u / mine);                    // plane k + 1
+            const int q = xcd + 8 * (int) (u - (long long) k * mine);
+            const long long r0 = (long long) k * c.ny + (long long) q * band;
+            const long long r1 = (long long) k * c.ny + min((q + 1) * band, c.ny);
+            const int s0 = max(1, min(G(b.row_first)[r0], nmem + 1));
+            const int s1 = r1 < nrow ? max(s0, min(G(b.row_first)[r1], nmem + 1)) : nmem + 1;
+            for (int t = s0 - 1 + (int) threadIdx.x; t < s1 - 1; t += LIST_BS)
+                member(t);
+        }
+    }
+    else if (chunk > 0 && nwg >= 8)
     {
         // Contiguous chunks of `chunk` slots, chunk q on the XCD q % 8: the
         // members of one chunk (about chunk / (nx * ny) planes of the
@@ -2245,15 +2285,15 @@ hipError_t launch_list(const View &v, int cb, int capture, int decide_realloc, h
     if (capture >= 0 && v.tot_list > 0)
         hipLaunchKernelGGL(k_list_write, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v);
     if (capture > 0 && v.tot_list > 0)
-        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb);
+        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb, 1);
     return hipGetLastError();
 }
 
 hipError_t launch_capture_only(const View &v, const DevBufs &b, int cb, hipStream_t s)
 {
     hipLaunchKernelGGL(k_realloc_decide, dim3(1), dim3(1), 0, s, b);
-    if (v.tot_list > 0)
-        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb);
+    if (v.tot_list > 0)   // the list may come from a state image, which holds no row_first
+        hipLaunchKernelGGL(k_capture, dim3((unsigned) v.tot_list), dim3(LIST_BS), 0, s, v, cb, 0);
     return hipGetLastError();
 }
 

@@ -44,6 +44,34 @@ def test_batch_matches_oracle(graph, zm, lds):
                 assert np.array_equal(d.interface_list(), o.diff_id()), (name, t)
 
 
+@pytest.mark.parametrize("band", [8])
+def test_batch_capture_row_bands_match_oracle(band):
+    """Capture on row bands in a batch: two directional blocks of different
+    sizes (their capture workgroups start at different XCDs) and a golden
+    case too small for the bands, each against its oracle bit for bit."""
+    import bench
+    cfgs = [bench.directional_config(100, 72, 64), bench.directional_config(64, 96, 40), load_case("c1_directional")[0]]
+    for c in cfgs[:2]:
+        c.dt *= 0.5
+    orcs = [OracleSubblock(c.params(), c.nucleation()) for c in cfgs]
+    with Batch([c.params() for c in cfgs], [c.nucleation() for c in cfgs]) as bt:
+        bt.set_tuning(abi.TUNE_CAPTURE_BAND, band)
+        done = 0
+        for t in (1, 60, 200, 300):
+            reps = bt.step(t - done)
+            for o in orcs:
+                assert o.step(t - done) == abi.OK
+            done = t
+            for i, (o, d, r) in enumerate(zip(orcs, bt.domains, reps)):
+                assert r.num_grains == o.num_grains, (i, t)
+                for f in (abi.FIELD_CE, abi.FIELD_CL, abi.FIELD_FS, abi.FIELD_GR):
+                    assert np.array_equal(bits(d.download(f)), bits(o.field(f))), (i, t, f)
+                ok = o.dc_taint() == 0
+                assert np.array_equal(bits(d.download(abi.FIELD_DC))[ok], bits(o.field(abi.FIELD_DC))[ok]), (i, t)
+                assert np.array_equal(d.interface_list(), o.diff_id()), (i, t)
+        assert min(o.gindex for o in orcs[:2]) > 1000
+
+
 def test_batch_restart_image_continues_bit_for_bit():
     """mmca_batch_save_state / mmca_batch_load_state on a batch of separate
     blocks: the loaded batch continues exactly as the uninterrupted one."""

@@ -172,6 +172,30 @@ def test_gpu_solid_volume_and_sums():
         assert gpu.solid_volume() == pytest.approx(orc.solid_sum() * cfg.cell_size ** 3, rel=1e-13)
 
 
+@pytest.mark.parametrize("band", [4, 12])
+def test_gpu_capture_row_bands_match_oracle(band):
+    """Capture on row bands pinned to XCDs (MMCA_TUNE_CAPTURE_BAND): a
+    120x100x80 directional block (100 rows: the last band is partial for
+    both band heights; 2 mask words per row, the second partial) against the
+    oracle bit for bit for 400 steps."""
+    import bench
+    cfg = bench.directional_config(120, 100, 80)
+    cfg.dt *= 0.5
+    orc = OracleSubblock(cfg.params(check_negative_ce=True), cfg.nucleation())
+    with Subblock(cfg.params(check_negative_ce=True), cfg.nucleation()) as gpu:
+        gpu.set_tuning(abi.TUNE_CAPTURE_BAND, band)
+        assert gpu.get_tuning(abi.TUNE_CAPTURE_BAND) == band
+        done = 0
+        for t in (1, 50, 150, 300, 400):
+            assert orc.step(t - done) == abi.OK
+            gpu.step(t - done)
+            done = t
+            print("bands", band, "step", t, "list", orc.gindex, flush=True)
+            assert_same_state(gpu, orc, "band %d step %d" % (band, t))
+        assert orc.gindex > 5000
+        orc.close()
+
+
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("lds,chunk", [(1, 16384)], ids=["zl_capture16k"])
 def test_gpu_full_size_directional_block(lds, chunk):

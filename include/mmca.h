@@ -187,8 +187,9 @@ int  mmca_eval_exp(int device, const double *x, double *y, int64_t n);
 
 /*
  * Achievable HBM bandwidth for the solute kernel's field mixes: a streaming
- * kernel with no arithmetic over `cells` cells of freshly allocated buffers,
- * timed with HIP events over `reps` launches after one warm-up launch.
+ * kernel with no arithmetic over `cells` cells (1 .. 2^40) of freshly
+ * allocated buffers, one contiguous run of 1024 cells per workgroup, timed
+ * with HIP events over `reps` launches after one warm-up launch.
  * mix 0: copy of one f64 field (16 B/cell); 1: the FP64 fields (read ce, cl,
  * fs f64 and gr i32, write ce, cl, fs: 52 B/cell); 2: the FP32-deviation
  * fields (read ce, cl f32, fs f64, gr i32, write ce, cl, fs: 36 B/cell).

@@ -37,7 +37,7 @@ hipError_t launch_grow_octahedra(const DevBufs &b, const Consts &c, int cb, hipS
 hipError_t launch_fill_f64(double *p, long long n, double v, hipStream_t s);
 hipError_t launch_eval_exp(const double *x, double *y, long long n, hipStream_t s);
 hipError_t launch_probe(int mix, long long n, double *a, double *b, double *f, float *ea, float *eb, int *g,
-                        int grid, hipStream_t s);
+                        hipStream_t s);
 long long list_words(const Consts &c);
 int list_blocks(const Consts &c);
 int reduce_blocks(const Consts &c);
@@ -455,16 +455,13 @@ int mmca_eval_exp(int device, const double *x, double *y, int64_t n)
 int mmca_bandwidth_probe(int device, int mix, int64_t cells, int reps, double *ms, double *bytes_per_cell)
 {
     static const double bpc[3] = { 16.0, 52.0, 36.0 };
-    if (mix < 0 || mix > 2 || cells <= 0 || reps <= 0 || !ms)
-        return MMCA_ERR_ARG;
+    if (mix < 0 || mix > 2 || cells <= 0 || cells > (1ll << 40) || reps <= 0 || !ms)
+        return MMCA_ERR_ARG;   // one workgroup per 1024 cells: the grid stays below 2^31
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return MMCA_ERR_NO_DEVICE;
     if (device < 0 || device >= ndev || hipSetDevice(device) != hipSuccess)
         return MMCA_ERR_ARG;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
-        return MMCA_ERR_HIP;
     const size_t n = (size_t) cells;
     double *a = nullptr, *b = nullptr, *f = nullptr;
     float *ea = nullptr, *eb = nullptr;
@@ -487,15 +484,12 @@ int mmca_bandwidth_probe(int device, int mix, int64_t cells, int reps, double *m
              (!g || hipMemsetAsync(g, 0, n * sizeof(int), s) == hipSuccess) &&
              (!ea || hipMemsetAsync(ea, 0, n * sizeof(float), s) == hipSuccess) &&
              (!eb || hipMemsetAsync(eb, 0, n * sizeof(float), s) == hipSuccess);
-    // enough workgroups to fill every CU several times over
-    const int grid = (int) std::min<long long>((long long) prop.multiProcessorCount * 32,
-                                               ((long long) n + 255) / 256);
     if (ok)
-        ok = launch_probe(mix, (long long) n, a, b, f, ea, eb, g, grid, s) == hipSuccess;   // warm-up
+        ok = launch_probe(mix, (long long) n, a, b, f, ea, eb, g, s) == hipSuccess;   // warm-up
     if (ok)
         ok = hipEventRecord(e0, s) == hipSuccess;
     for (int r = 0; ok && r < reps; ++r)
-        ok = launch_probe(mix, (long long) n, a, b, f, ea, eb, g, grid, s) == hipSuccess;
+        ok = launch_probe(mix, (long long) n, a, b, f, ea, eb, g, s) == hipSuccess;
     float t = 0.0f;
     if (ok)
         ok = hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&

@@ -301,6 +301,9 @@ int build_view(mmca_ctx *const *ctxs, int n, int xcd_remap, int zm_by, DomainDes
     v->dom = *desc;
     v->ndom = n;
     v->xcd_remap = xcd_remap;
+    v->zm_planes_max = 0;
+    for (int d = 0; d < n; d++)
+        v->zm_planes_max = std::max(v->zm_planes_max, ctxs[d]->c.zm_planes);
     v->off_nuc = *off;
     v->off_zm = *off + (n + 1);
     v->off_cell = *off + 2 * (n + 1);

@@ -139,6 +139,7 @@ struct View {
     const DomainDesc *dom;       // device array, ndom entries
     int ndom;
     int xcd_remap;               // contiguous tile runs per XCD for the solute kernel
+    int zm_planes_max;           // largest zm_planes of the domains (LDS of the per-plane temperatures)
     const long long *off_nuc;    // k_nuc_detect workgroups
     const long long *off_zm;     // z-marching solute tiles
     const long long *off_cell;   // cell-parallel solute workgroups

@@ -1016,7 +1016,6 @@ k_solute_zm(View V, int cur, int nxt)
 // memory operation issued in between makes that wait stricter, never looser.
 // Every load address is clamped into the padded box of its field.
 constexpr int ZL_RING = 4;
-constexpr int ZL_TMAX = 1024;                    // planes per workgroup (MMCA_TUNE_ZM_PLANES at most)
 
 template <int BY, bool F32>
 struct ZlSlot {
@@ -1086,8 +1085,10 @@ k_solute_zl(View V, int cur, int nxt)
     __shared__ double s_z[AM ? 2 : 1][AM ? BEAM_NT : 1];      // plane factors of the beam, double-buffered
     // the other temperature models depend on z only (on y in 2-D runs): one
     // value per plane of the tile, computed before the walk, so that their
-    // constants need no registers during it
-    __shared__ double s_T[AM ? 1 : ZL_TMAX];
+    // constants need no registers during it; dynamic LDS of V.zm_planes_max
+    // doubles (none for the beam), so that short tiles leave room for more
+    // workgroups per CU
+    extern __shared__ double s_T[];
     BeamTab *beam = reinterpret_cast<BeamTab *>(s_beam);
     long long gtile = blockIdx.x;
     if (V.xcd_remap)
@@ -1209,7 +1210,7 @@ k_solute_zl(View V, int cur, int nxt)
         if (c.gdimz == 1)
             t_row = temp_model<false>(c, sim_time, x, j, k0);
         else
-            for (int e = ty * 64 + lane; e < min(n, ZL_TMAX); e += 64 * BY)
+            for (int e = ty * 64 + lane; e < n; e += 64 * BY)
                 s_T[e] = temp_model<false>(c, sim_time, x0, j0, k0 + e);   // read after the loop's first barrier
     }
     zl_wait<2 * L::NPW>();                       // planes k0 - 1 and k0 have landed
@@ -1257,7 +1258,7 @@ k_solute_zl(View V, int cur, int nxt)
                     else if (c.beam_sep)
                         return G(b.temp)[idx];
                     else
-                        return c.gdimz == 1 ? t_row : s_T[min(s - 1, ZL_TMAX - 1)];
+                        return c.gdimz == 1 ? t_row : s_T[s - 1];
                 },
                 ce_new, ce_old, clv, fsv, g_old, &cl_out, &fs_out, &g_new, &branch);
             st_cl<F32>(b, c, nxt, idx, cl_out);
@@ -2209,12 +2210,13 @@ static void launch_solute_t(const View &v, int zm_by, int lds, int diff, int fsc
     if (diff && fsc && zm_by > 0 && lds)
     {
         const long long ng = v.xcd_remap ? ((v.tot_zm + 7) / 8) * 8 : v.tot_zm;
+        const size_t tlds = AM ? 0 : (size_t) v.zm_planes_max * sizeof(double);   // per-plane temperatures
         if (zm_by == 16)
-            hipLaunchKernelGGL((k_solute_zl<16, AM, F32>), dim3((unsigned) ng), dim3(64, 16), 0, s, v, cur, nxt);
+            hipLaunchKernelGGL((k_solute_zl<16, AM, F32>), dim3((unsigned) ng), dim3(64, 16), tlds, s, v, cur, nxt);
         else if (zm_by == 8)
-            hipLaunchKernelGGL((k_solute_zl<8, AM, F32>), dim3((unsigned) ng), dim3(64, 8), 0, s, v, cur, nxt);
+            hipLaunchKernelGGL((k_solute_zl<8, AM, F32>), dim3((unsigned) ng), dim3(64, 8), tlds, s, v, cur, nxt);
         else
-            hipLaunchKernelGGL((k_solute_zl<4, AM, F32>), dim3((unsigned) ng), dim3(64, 4), 0, s, v, cur, nxt);
+            hipLaunchKernelGGL((k_solute_zl<4, AM, F32>), dim3((unsigned) ng), dim3(64, 4), tlds, s, v, cur, nxt);
     }
     else if (diff && fsc && zm_by > 0)
     {

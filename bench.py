@@ -551,10 +551,13 @@ def main():
     ap.add_argument("--size", type=str, default="256x256x512")
     ap.add_argument("--precision", choices=("fp64", "fp32dev"), default="fp64",
                     help="storage of the timed headline run (the line's metric is quoted on fp64)")
-    ap.add_argument("--zm", type=int, default=128,
-                    help="z-marching solute kernel, planes per workgroup (0: cell-parallel)")
-    ap.add_argument("--zm-rows", type=int, default=16, help="rows per z-marching workgroup (4, 8, 16)")
-    ap.add_argument("--xcd", type=int, default=1, help="1: contiguous tile runs per XCD")
+    ap.add_argument("--zm", type=int, default=None,
+                    help="z-marching solute kernel, planes per workgroup (0: cell-parallel; default 128 for FP64, "
+                         "8 for FP32-deviation storage)")
+    ap.add_argument("--zm-rows", type=int, default=None,
+                    help="rows per z-marching workgroup (4, 8, 16; default 16 for FP64, 4 for FP32-deviation)")
+    ap.add_argument("--xcd", type=int, default=None,
+                    help="1: contiguous tile runs per XCD (default 1 for FP64, 0 for FP32-deviation)")
     ap.add_argument("--lds", type=int, default=1,
                     help="1: z-marching kernel with its planes staged in LDS by direct loads (k_solute_zl)")
     ap.add_argument("--capture-chunk", type=int, default=0,
@@ -581,6 +584,14 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
     dims = tuple(int(v) for v in args.size.split("x"))
+    # tile shapes: long 16-row tiles for FP64 storage (the fastest over the
+    # 5000-step run), short 8 x 4 tiles for FP32-deviation storage
+    fp32_tile = {"zm": 8, "rows": 4, "xcd": 0}
+    fp64_tile = {"zm": 128, "rows": 16, "xcd": 1}
+    dflt = fp32_tile if args.precision == "fp32dev" else fp64_tile
+    args.zm = dflt["zm"] if args.zm is None else args.zm
+    args.zm_rows = dflt["rows"] if args.zm_rows is None else args.zm_rows
+    args.xcd = dflt["xcd"] if args.xcd is None else args.xcd
     shape = {"zm": args.zm, "rows": args.zm_rows, "xcd": args.xcd, "lds": args.lds, "chunk": args.capture_chunk,
              "band": args.capture_band}
     if args.capture_footprint:
@@ -638,7 +649,7 @@ def main():
     extra = {"bandwidth_probe": probe}
     if not args.no_extra:
         if not fp32:
-            extra["fp32dev"] = run_fp32_variant(cfg, args.extra_steps, args.warmup, shape)
+            extra["fp32dev"] = run_fp32_variant(cfg, args.extra_steps, args.warmup, dict(shape, **fp32_tile))
             log("fp32dev:", extra["fp32dev"])
         extra["literal_dt"] = run_literal_dt(dims, args.extra_steps, args.warmup, shape)
         log("literal dt:", extra["literal_dt"])

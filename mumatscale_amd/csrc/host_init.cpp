@@ -138,8 +138,8 @@ int build_consts(const mmca_params &p, Consts &c)
     c.sxy = (long long) (p.nx + 2) * (p.ny + 2);
     c.P = padded_cells(p);
     c.w64 = (c.nx + 63) / 64;          // mask words per row (interior x)
-    c.zm_planes = 64;                   // z-marching solute kernel by default
-    c.zm_by = 16;
+    c.zm_planes = 64;                   // z-marching solute kernel by default (FP32 storage:
+    c.zm_by = 16;                       // shorter tiles, below)
     c.xcd_remap = 1;
     c.zm_lds = 1;                       // planes staged in LDS (k_solute_zl)
     c.capture_chunk = 0;
@@ -148,6 +148,14 @@ int build_consts(const mmca_params &p, Consts &c)
     if (p.precision != MMCA_PREC_FP64 && p.precision != MMCA_PREC_FP32DEV)
         return MMCA_ERR_ARG;
     c.f32 = (p.precision == MMCA_PREC_FP32DEV) ? 1 : 0;
+    if (c.f32) {
+        // FP32 storage streams faster in short tiles of 64 x 4 cells x 8 planes
+        // (several workgroups per CU); FP64 storage runs as fast or faster in
+        // the long 16-row tiles over a long run
+        c.zm_planes = 8;
+        c.zm_by = 4;
+        c.xcd_remap = 0;
+    }
     const int n3[3] = { p.nx, p.ny, p.nz };
     for (int a = 0; a < 3; a++) {
         // a face is a mold wall where the global domain ends (one subblock:

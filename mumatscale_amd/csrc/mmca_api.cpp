@@ -1717,7 +1717,7 @@ struct mmca_batch {
     View view;
     bool view_dirty = true;
     int cur = 0;
-    int zm_planes = 64, zm_by = 16, xcd_remap = 1, zm_lds = 1, capture_chunk = 0, capture_band = 0, track_d = 0;
+    int zm_planes = 8, zm_by = 4, xcd_remap = 0, zm_lds = 1, capture_chunk = 0, capture_band = 0, track_d = 0;
     bool timing = false;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;

@@ -172,6 +172,37 @@ def test_gpu_solid_volume_and_sums():
         assert gpu.solid_volume() == pytest.approx(orc.solid_sum() * cfg.cell_size ** 3, rel=1e-13)
 
 
+@pytest.mark.parametrize("variant", [(64, 16, 1, 1), (8, 4, 0, 1), (16, 8, 1, 0), (0, 4, 0, 0)],
+                         ids=["zl64x16", "zl8x4", "zm16x8", "cell"])
+def test_gpu_two_dimensional_block_matches_oracle(variant):
+    """A one-plane directional block, 128x96x1: with a global z extent of 1 the
+    frozen gradient runs along y (temperature.c:41-61), which the LDS-staged
+    kernel evaluates once per row instead of once per plane.  Bit for bit
+    against the oracle at checkpoints over 800 steps (123 grains, an interface
+    of 2 372 cells at the end), for the LDS-staged kernel at two tile shapes,
+    the register z-marching kernel and the cell-parallel kernel."""
+    import bench
+    zm, rows, xcd, lds = variant
+    cfg = bench.directional_config(128, 96, 1)
+    cfg.dt *= 0.5
+    orc = OracleSubblock(cfg.params(check_negative_ce=True), cfg.nucleation())
+    with Subblock(cfg.params(check_negative_ce=True), cfg.nucleation()) as gpu:
+        gpu.set_tuning(abi.TUNE_ZM_PLANES, zm)
+        gpu.set_tuning(abi.TUNE_ZM_ROWS, rows)
+        gpu.set_tuning(abi.TUNE_XCD_REMAP, xcd)
+        gpu.set_tuning(abi.TUNE_ZM_LDS, lds)
+        orc.temp_update()
+        assert_bits_equal(gpu.download(abi.FIELD_TEMP), orc.field(abi.FIELD_TEMP), "2-D temperature")
+        done = 0
+        for t in (1, 50, 200, 400, 800):
+            assert orc.step(t - done) == abi.OK
+            gpu.step(t - done)
+            done = t
+            assert_same_state(gpu, orc, "2-D step %d" % t, check_d=(t == 800))
+        assert orc.num_grains > 100 and orc.gindex > 2000
+        orc.close()
+
+
 @pytest.mark.parametrize("band", [4, 12])
 def test_gpu_capture_row_bands_match_oracle(band):
     """Capture on row bands pinned to XCDs (MMCA_TUNE_CAPTURE_BAND): a

@@ -318,9 +318,12 @@ int  mmca_orientation_draws(const mmca_params *params, const mmca_nucleation_cfg
                             double *angles_out, double *rotmat_out);
 
 /* --- tuning (no effect on results) --------------------------------------- */
-#define MMCA_TUNE_ZM_PLANES  0   /* 1..1024: z-marching solute kernel, planes per workgroup; 0: cell-parallel */
-#define MMCA_TUNE_ZM_ROWS    1   /* rows per z-marching workgroup: 4, 8 or 16 */
-#define MMCA_TUNE_XCD_REMAP  2   /* 1: workgroups of one XCD take contiguous tiles */
+#define MMCA_TUNE_ZM_PLANES  0   /* 1..1024: z-marching solute kernel, planes per workgroup; 0: cell-parallel
+                                    (default 64 for FP64 blocks, 8 for FP32-deviation blocks and batches) */
+#define MMCA_TUNE_ZM_ROWS    1   /* rows per z-marching workgroup: 4, 8 or 16 (default 16 for FP64 blocks,
+                                    4 for FP32-deviation blocks and batches) */
+#define MMCA_TUNE_XCD_REMAP  2   /* 1: workgroups of one XCD take contiguous tiles (default 1 for FP64 blocks,
+                                    0 for FP32-deviation blocks and batches) */
 #define MMCA_TUNE_GRAPH      3   /* batches: 1 = replay steps from a captured HIP graph */
 #define MMCA_TUNE_ZM_LDS     4   /* 1 (default): z-marching kernel with its planes staged in LDS by direct loads; 0: in registers */
 #define MMCA_TUNE_CAPTURE_CHUNK 5 /* >0: capture deals list slots in chunks of this many (rounded up to
